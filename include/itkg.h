@@ -1,0 +1,194 @@
+/*
+ * itkg.h -- C ABI of the MI355X (gfx950) separable Gaussian for the itk-rs
+ * image pipeline.
+ *
+ * The library replaces the body of itk-rs's GaussianFilter::generate_region
+ * (src/filters/gaussian.rs:59-90) and the pass it calls for every axis,
+ * convolve_axis (src/filters/conv.rs:11-41), with HIP kernels.  The region
+ * planning it needs (kernel_radius, build_kernel, input_region_for_output,
+ * split_region) is exported as well, so every caller -- a Rust ImageSource
+ * node, the C++ host library, Python -- shares one definition.
+ *
+ * Conventions
+ *   - Plain C types only; no unwinding crosses the boundary.
+ *   - Status codes: 0 = ITKG_OK, negative = error; itkg_last_error() returns
+ *     a thread-local message for the last failing call of this thread.
+ *   - The caller owns every buffer; the library never frees caller memory.
+ *     Pixel buffers passed as itkg_view are device memory unless the entry
+ *     point's name ends in _host.
+ *   - Pixels: dtype (f32 or f64) times ncomp components, components
+ *     interleaved (VecPixel<T, N> is [T; N], array of structures).
+ *   - Images are axis-0-fastest (src/image.rs:111-120).  A view's stride[d]
+ *     is the distance in PIXELS between neighbours along axis d.
+ *   - Every entry point is reentrant; calls on different streams may run
+ *     concurrently.  Work is enqueued on `stream` (a hipStream_t; NULL = the
+ *     default stream) and is complete when that stream is synchronised.
+ */
+#ifndef ITKG_H
+#define ITKG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ITKG_MAX_DIM 4
+/* Largest per-axis radius the kernels accept: 2r+1 taps travel in the
+ * kernel-argument block. */
+#define ITKG_MAX_RADIUS 127
+
+typedef enum {
+    ITKG_OK = 0,
+    ITKG_ERR_INVALID_ARG = -1,
+    ITKG_ERR_OOM = -2,
+    ITKG_ERR_HIP = -3,
+    ITKG_ERR_UNSUPPORTED = -4,
+    ITKG_ERR_NO_DEVICE = -5
+} itkg_status;
+
+typedef enum { ITKG_F32 = 0, ITKG_F64 = 1 } itkg_dtype;
+
+/* Arithmetic mode.
+ *   ITKG_EXACT: the reference's arithmetic -- every tap fl(x * (T)w_k) added
+ *     to an accumulator that starts at +0, taps k = 0..2r in order, no FMA,
+ *     each pass rounded to T.  Bit-identical to itk-rs.
+ *   ITKG_FAST: the same weights, truncation, clamp and pass order; each tap
+ *     is a fused multiply-add.  Within 1e-5 (max abs error / max abs value)
+ *     of ITKG_EXACT for f32; bit-identical between full-image and streamed
+ *     runs. */
+typedef enum { ITKG_EXACT = 0, ITKG_FAST = 1 } itkg_arith;
+
+/* Kernel selection (testing and tuning).  AUTO picks the fused one-round-trip
+ * kernel where it applies (D = 3 or D = 2, radii within its template set) and
+ * the per-axis kernel otherwise; both give bit-identical results. */
+typedef enum { ITKG_PATH_AUTO = 0, ITKG_PATH_PER_AXIS = 1, ITKG_PATH_FUSED = 2 } itkg_path;
+
+/* Region<D> (src/image.rs:9-15): index may be negative; size >= 0.
+ * Entries at d >= ndim are ignored. */
+typedef struct {
+    int64_t index[ITKG_MAX_DIM];
+    int64_t size[ITKG_MAX_DIM];
+} itkg_region;
+
+/* Strided window onto the pixels of `region`: `data` points at the pixel
+ * whose coordinates are region.index. */
+typedef struct {
+    void* data;
+    itkg_region region;
+    int64_t stride[ITKG_MAX_DIM];
+} itkg_view;
+
+/* ---- region planning (host only, no device work) ----------------------- */
+
+/* GaussianFilter::kernel_radius, src/filters/gaussian.rs:24-27:
+ * ceil(3 * (sigma / spacing)) cast with Rust's saturating float->usize rule
+ * (NaN and negatives give 0). */
+uint64_t itkg_kernel_radius(double sigma, double spacing);
+
+/* GaussianFilter::build_kernel, src/filters/gaussian.rs:29-40: writes the
+ * 2*radius+1 normalised f64 weights to `out` (capacity `cap`). */
+int itkg_build_kernel(double sigma_px, uint64_t radius, double* out, uint64_t cap);
+
+/* GaussianFilter::input_region_for_output, src/filters/gaussian.rs:52-57:
+ * output.padded_per_axis(radii).clipped_to(largest). */
+int itkg_input_region_for_output(int32_t ndim, double sigma, const double* spacing,
+                                 const itkg_region* largest, const itkg_region* output,
+                                 itkg_region* input);
+
+/* split_region, src/region_splitter.rs:8-39: slowest-axis slabs, the first
+ * (size mod n) pieces one row larger.  Writes min(n, size[ndim-1]) pieces
+ * (0 for an empty region) to `pieces` (capacity `cap`), count to `*count`. */
+int itkg_split_region(int32_t ndim, const itkg_region* region, uint64_t num_pieces,
+                      itkg_region* pieces, uint64_t cap, uint64_t* count);
+
+/* ---- forward ----------------------------------------------------------- */
+
+/* GaussianFilter::generate_region, src/filters/gaussian.rs:59-90, with each
+ * pass the convolve_axis of src/filters/conv.rs:11-41.
+ *
+ *   largest : the source's largest_region() (the clamp bounds)
+ *   input   : device view holding at least input_region_for_output(
+ *             output->region); it may be larger (e.g. the whole image)
+ *   output  : device view whose region is the requested region; it must lie
+ *             inside `largest`.  Output and input must not overlap.
+ *   sigma   : physical sigma; spacing[d] per axis (sigma_px = sigma/spacing)
+ *
+ * Streaming (src/source.rs:81-97) is this call once per piece with the
+ * piece as output region; every voxel is computed by the same taps as in the
+ * full-image call, so streamed and full outputs are bit-identical. */
+int itkg_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                                  const double* spacing, const itkg_region* largest,
+                                  const itkg_view* input, const itkg_view* output,
+                                  itkg_arith arith, void* stream);
+
+/* Same contract on dense HOST buffers (what an ImageSource whose Image owns
+ * a host Vec<P> hands over): `input` holds input_region_for_output(requested)
+ * densely, `output` receives `requested` densely.  Copies in, computes and
+ * copies out on an internal stream; returns when the output is written. */
+int itkg_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                       double sigma, const double* spacing,
+                                       const itkg_region* largest, const void* input,
+                                       const itkg_region* requested, void* output,
+                                       itkg_arith arith);
+
+/* ---- adjoint (no counterpart in the reference) ------------------------- */
+
+/* Transpose of itkg_gaussian_generate_region for the same (largest, sigma,
+ * spacing, requested = grad_output->region): the passes are transposed in
+ * reverse axis order; each clamp folds the clamped taps' weights onto the
+ * edge voxel.  grad_input->region must contain input_region_for_output(
+ * requested); only that subregion is written.  With accumulate != 0 the
+ * result is added to grad_input (streamed pieces whose halos overlap). */
+int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                           const double* spacing, const itkg_region* largest,
+                           const itkg_view* grad_output, const itkg_view* grad_input,
+                           int32_t accumulate, void* stream);
+
+/* ---- batched many-volume form (no counterpart in the reference) -------- */
+
+/* One whole 3-D f32 scalar volume: output = GaussianFilter{source: input,
+ * sigma}.generate_region(largest_region) with per-axis spacing.  Both
+ * buffers are dense device memory of size[0]*size[1]*size[2] floats.  For
+ * the adjoint direction `input` is the gradient w.r.t. the output and
+ * `output` receives the gradient w.r.t. the input. */
+typedef struct {
+    const float* input;
+    float* output;
+    int64_t size[3];
+    double sigma;
+    double spacing[3];
+} itkg_volume;
+
+typedef struct itkg_batch itkg_batch;
+
+typedef enum { ITKG_BATCH_FORWARD = 0, ITKG_BATCH_BACKWARD = 1 } itkg_batch_direction;
+
+/* Plans one launch over all volumes: per-volume radii and weights, a packed
+ * tile work list sorted by cost, tile shapes from the autotuner's table.
+ * Allocates the plan's device tables.  The volumes' buffers are bound at
+ * creation; create a new plan for new buffers. */
+int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_direction direction,
+                      itkg_arith arith, itkg_batch** plan);
+/* Runs the plan on `stream`.  The first run on a stream captures the launch
+ * sequence into a HIP graph; later runs on that stream replay it. */
+int itkg_batch_run(itkg_batch* plan, void* stream);
+/* Voxel count covered by the plan (sum of volume sizes). */
+int64_t itkg_batch_voxels(const itkg_batch* plan);
+int itkg_batch_destroy(itkg_batch* plan);
+
+/* ---- control ----------------------------------------------------------- */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* itkg_last_error(void);
+/* Process-wide kernel selection (ITKG_PATH_AUTO by default). */
+int itkg_set_path(itkg_path path);
+/* Library version string. */
+const char* itkg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITKG_H */

@@ -1,0 +1,634 @@
+// C ABI of libitkg: region planning, argument checking and kernel dispatch.
+//
+// The planning restates src/filters/gaussian.rs:24-90 (radius rule, f64
+// weights, per-pass regions) and src/image.rs:28-64 (padded_per_axis,
+// clipped_to); the kernels do the per-pixel work.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "itkg_impl.hpp"
+
+namespace itkg {
+
+namespace {
+thread_local std::string g_err;
+std::atomic<int> g_path{ITKG_PATH_AUTO};
+}  // namespace
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? ITKG_ERR_OOM : ITKG_ERR_HIP, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+void clear_error() { g_err.clear(); }
+
+namespace {
+
+// ---- region algebra (src/image.rs:17-64) --------------------------------
+
+int64_t region_len(int D, const itkg_region& r) {
+    int64_t n = 1;
+    for (int d = 0; d < D; ++d) n *= r.size[d];
+    return n;
+}
+
+itkg_region padded(int D, itkg_region r, const int64_t* radii) {
+    for (int d = 0; d < D; ++d) {
+        r.index[d] -= radii[d];
+        r.size[d] += 2 * radii[d];
+    }
+    return r;
+}
+
+itkg_region clipped(int D, itkg_region r, const itkg_region& b) {
+    for (int d = 0; d < D; ++d) {
+        const int64_t se = r.index[d] + r.size[d], be = b.index[d] + b.size[d];
+        const int64_t s = std::max(r.index[d], b.index[d]);
+        const int64_t e = std::min(se, be);
+        r.index[d] = s;
+        r.size[d] = std::max<int64_t>(e - s, 0);
+    }
+    return r;
+}
+
+bool contains(int D, const itkg_region& outer, const itkg_region& inner) {
+    for (int d = 0; d < D; ++d) {
+        if (inner.size[d] == 0) continue;
+        if (inner.index[d] < outer.index[d]) return false;
+        if (inner.index[d] + inner.size[d] > outer.index[d] + outer.size[d]) return false;
+    }
+    return true;
+}
+
+bool region_ok(int D, const itkg_region& r) {
+    for (int d = 0; d < D; ++d)
+        if (r.size[d] < 0) return false;
+    return true;
+}
+
+// ---- kernel (src/filters/gaussian.rs:24-40) ------------------------------
+
+uint64_t radius_of(double sigma, double spacing) {
+    const double sigma_px = sigma / spacing;
+    const double v = std::ceil(3.0 * sigma_px);
+    if (!(v > 0.0)) return 0;
+    if (v >= 18446744073709551616.0) return UINT64_MAX;
+    return (uint64_t)v;
+}
+
+void weights_of(double sigma_px, uint64_t radius, double* k) {
+    const uint64_t len = 2 * radius + 1;
+    for (uint64_t i = 0; i < len; ++i) {
+        const double x = (double)i - (double)radius;
+        const double t = x / sigma_px;
+        k[i] = std::exp(-0.5 * (t * t));
+    }
+    double sum = 0.0;
+    for (uint64_t i = 0; i < len; ++i) sum += k[i];
+    for (uint64_t i = 0; i < len; ++i) k[i] /= sum;
+}
+
+// ---- per-call plan (src/filters/gaussian.rs:59-90) ----------------------
+
+struct Plan {
+    int D = 0;
+    int64_t radii[ITKG_MAX_DIM] = {0, 0, 0, 0};
+    itkg_region bounds{}, req{}, in_reg{};
+    itkg_region reg[ITKG_MAX_DIM + 1]{};  // reg[a]: input of pass a; reg[D] = req
+    double w[ITKG_MAX_DIM][kMaxTaps];
+};
+
+int make_plan(int D, double sigma, const double* spacing, const itkg_region* largest,
+              const itkg_region& req, Plan* p) {
+    if (D < 1 || D > ITKG_MAX_DIM) return fail(ITKG_ERR_INVALID_ARG, "ndim %d not in [1, %d]", D, ITKG_MAX_DIM);
+    if (!spacing || !largest) return fail(ITKG_ERR_INVALID_ARG, "null spacing or largest region");
+    if (!region_ok(D, *largest) || !region_ok(D, req))
+        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
+    p->D = D;
+    p->bounds = *largest;
+    p->req = req;
+    for (int d = 0; d < D; ++d) {
+        const uint64_t r = radius_of(sigma, spacing[d]);
+        if (r > ITKG_MAX_RADIUS)
+            return fail(ITKG_ERR_UNSUPPORTED, "axis %d radius %llu exceeds %d", d,
+                        (unsigned long long)r, ITKG_MAX_RADIUS);
+        p->radii[d] = (int64_t)r;
+        weights_of(sigma / spacing[d], r, p->w[d]);
+    }
+    if (region_len(D, req) > 0 && !contains(D, *largest, req))
+        return fail(ITKG_ERR_INVALID_ARG, "requested region is not inside the largest region");
+    p->in_reg = clipped(D, padded(D, req, p->radii), *largest);
+    p->reg[0] = p->in_reg;
+    for (int a = 0; a < D; ++a) {
+        if (a + 1 < D) {
+            int64_t rem[ITKG_MAX_DIM] = {0, 0, 0, 0};
+            for (int d = a + 1; d < D; ++d) rem[d] = p->radii[d];
+            p->reg[a + 1] = clipped(D, padded(D, req, rem), *largest);
+        } else {
+            p->reg[a + 1] = req;
+        }
+    }
+    return ITKG_OK;
+}
+
+size_t dtype_size(itkg_dtype t) { return t == ITKG_F64 ? 8 : 4; }
+
+// Pixel offset of `at` inside view v.
+int64_t view_offset(int D, const itkg_view& v, const int64_t* at) {
+    int64_t off = 0;
+    for (int d = 0; d < D; ++d) off += (at[d] - v.region.index[d]) * v.stride[d];
+    return off;
+}
+
+itkg_view dense_view(int D, void* data, const itkg_region& r) {
+    itkg_view v{};
+    v.data = data;
+    v.region = r;
+    int64_t s = 1;
+    for (int d = 0; d < ITKG_MAX_DIM; ++d) {
+        v.stride[d] = s;
+        if (d < D) s *= r.size[d];
+    }
+    return v;
+}
+
+// Scratch needed by the per-axis path: two ping-pong buffers holding the
+// largest intermediate region.
+size_t scratch_bytes(const Plan& p, itkg_dtype dt, int ncomp) {
+    int64_t m = 0;
+    for (int a = 1; a < p.D; ++a) m = std::max(m, region_len(p.D, p.reg[a]));
+    return (size_t)(2 * m) * ncomp * dtype_size(dt);
+}
+
+template <typename T>
+void fill_pass(AxisPassArgs<T>& A, const Plan& p, int axis, const itkg_view& src,
+               const itkg_view& dst, const itkg_region& out_reg) {
+    const int D = p.D;
+    std::memset(&A, 0, sizeof A);
+    for (int d = 0; d < ITKG_MAX_DIM; ++d) {
+        A.n[d] = d < D ? out_reg.size[d] : 1;
+        A.out_lo[d] = d < D ? out_reg.index[d] : 0;
+        A.src_lo[d] = d < D ? src.region.index[d] : 0;
+        A.src_stride[d] = d < D ? src.stride[d] : 0;
+        A.dst_lo[d] = d < D ? dst.region.index[d] : 0;
+        A.dst_stride[d] = d < D ? dst.stride[d] : 0;
+    }
+    A.src = static_cast<const T*>(src.data);
+    A.dst = static_cast<T*>(dst.data);
+    A.total = region_len(D, out_reg);
+    A.axis = axis;
+    A.radius = (int32_t)p.radii[axis];
+    for (int k = 0; k <= 2 * p.radii[axis]; ++k) A.w[k] = (T)p.w[axis][k];
+}
+
+int check_view(int D, const itkg_view* v, const itkg_region& need, const char* what) {
+    if (!v) return fail(ITKG_ERR_INVALID_ARG, "%s view is null", what);
+    if (!region_ok(D, v->region)) return fail(ITKG_ERR_INVALID_ARG, "%s view: negative size", what);
+    if (region_len(D, need) > 0) {
+        if (!v->data) return fail(ITKG_ERR_INVALID_ARG, "%s view has no data", what);
+        if (!contains(D, v->region, need))
+            return fail(ITKG_ERR_INVALID_ARG, "%s view does not cover the region it must hold", what);
+    }
+    return ITKG_OK;
+}
+
+template <typename T>
+int forward_per_axis(const Plan& p, int ncomp, const itkg_view& in, const itkg_view& out,
+                     itkg_arith arith, hipStream_t s, void* scratch) {
+    const int D = p.D;
+    void* own = nullptr;
+    const size_t sb = scratch_bytes(p, sizeof(T) == 8 ? ITKG_F64 : ITKG_F32, ncomp);
+    if (D > 1 && !scratch && sb > 0) {
+        hipError_t e = hipMallocAsync(&own, sb, s);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(per-axis scratch)");
+        scratch = own;
+    }
+    char* buf[2] = {static_cast<char*>(scratch), static_cast<char*>(scratch) + sb / 2};
+    itkg_view cur = in;
+    int rc = ITKG_OK;
+    for (int a = 0; a < D && rc == ITKG_OK; ++a) {
+        const itkg_region& oreg = p.reg[a + 1];
+        itkg_view nxt = (a + 1 == D) ? out : dense_view(D, buf[a & 1], oreg);
+        AxisPassArgs<T> A;
+        fill_pass(A, p, a, cur, nxt, oreg);
+        A.clamp_lo = p.reg[a].index[a];
+        A.clamp_hi = p.reg[a].index[a] + p.reg[a].size[a] - 1;
+        hipError_t e = launch_axis_forward<T>(A, ncomp, arith, s);
+        if (e != hipSuccess) rc = hip_fail(e, "per-axis forward launch");
+        cur = nxt;
+    }
+    if (own) (void)hipFreeAsync(own, s);
+    return rc;
+}
+
+int forward_fused3d(const Plan& p, const itkg_view& in, const itkg_view& out, itkg_arith arith,
+                    hipStream_t s) {
+    Fused3dArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.src = static_cast<const float*>(in.data) + view_offset(3, in, p.in_reg.index);
+    A.dst = static_cast<float*>(out.data);
+    for (int d = 0; d < 3; ++d) {
+        A.src_stride[d] = in.stride[d];
+        A.dst_stride[d] = out.stride[d];
+        A.n_in[d] = (int32_t)p.in_reg.size[d];
+        A.n_out[d] = (int32_t)p.req.size[d];
+        A.off[d] = (int32_t)(p.req.index[d] - p.in_reg.index[d]);
+    }
+    const int r = (int)p.radii[0];
+    for (int k = 0; k <= 2 * r; ++k) {
+        A.wx[k] = (float)p.w[0][k];
+        A.wy[k] = (float)p.w[1][k];
+        A.wz[k] = (float)p.w[2][k];
+    }
+    const int TX = fused3d_tile_x(), TY = fused3d_tile_y(r);
+    A.tiles_x = (A.n_out[0] + TX - 1) / TX;
+    A.tiles_y = (A.n_out[1] + TY - 1) / TY;
+    // z run per workgroup: halve it until the grid holds >= 1024 workgroups
+    // (4 per CU) or the run would be shorter than the ring it fills.
+    int tz = A.n_out[2];
+    if (const char* e = std::getenv("ITKG_FUSED_TZ")) tz = std::max(1, std::atoi(e));
+    else {
+        const int minz = std::max(16, 4 * r);
+        while ((int64_t)A.tiles_x * A.tiles_y * ((A.n_out[2] + tz - 1) / tz) < 1024 && tz / 2 >= minz)
+            tz = (tz + 1) / 2;
+    }
+    A.tz = tz;
+    A.tiles_z = (A.n_out[2] + tz - 1) / tz;
+    hipError_t e = launch_fused3d(A, r, r, r, arith, s);
+    if (e != hipSuccess) return hip_fail(e, "fused 3-D launch");
+    return ITKG_OK;
+}
+
+bool fits_i32(const Plan& p) {
+    for (int d = 0; d < p.D; ++d)
+        if (p.in_reg.size[d] > INT32_MAX || p.req.size[d] > INT32_MAX) return false;
+    return true;
+}
+
+bool use_fused(const Plan& p, itkg_dtype dt, int ncomp) {
+    const int path = g_path.load();
+    if (path == ITKG_PATH_PER_AXIS) return false;
+    return p.D == 3 && dt == ITKG_F32 && ncomp == 1 && fits_i32(p) &&
+           fused3d_supported((int)p.radii[0], (int)p.radii[1], (int)p.radii[2]);
+}
+
+int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
+                 const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch) {
+    if (region_len(p.D, p.req) == 0) return ITKG_OK;
+    if (use_fused(p, dt, ncomp)) return forward_fused3d(p, in, out, arith, s);
+    if (g_path.load() == ITKG_PATH_FUSED)
+        return fail(ITKG_ERR_UNSUPPORTED, "fused path requested but not available for this call");
+    return dt == ITKG_F64 ? forward_per_axis<double>(p, ncomp, in, out, arith, s, scratch)
+                          : forward_per_axis<float>(p, ncomp, in, out, arith, s, scratch);
+}
+
+template <typename T>
+int backward_per_axis(const Plan& p, int ncomp, const itkg_view& gout, const itkg_view& gin,
+                      int accumulate, hipStream_t s, void* scratch) {
+    const int D = p.D;
+    void* own = nullptr;
+    const size_t sb = scratch_bytes(p, sizeof(T) == 8 ? ITKG_F64 : ITKG_F32, ncomp);
+    if (D > 1 && !scratch && sb > 0) {
+        hipError_t e = hipMallocAsync(&own, sb, s);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(adjoint scratch)");
+        scratch = own;
+    }
+    char* buf[2] = {static_cast<char*>(scratch), static_cast<char*>(scratch) + sb / 2};
+    itkg_view cur = gout;
+    int rc = ITKG_OK;
+    for (int a = D - 1; a >= 0 && rc == ITKG_OK; --a) {
+        const itkg_region& dreg = p.reg[a];  // gradient w.r.t. this pass's input
+        itkg_view nxt = (a == 0) ? gin : dense_view(D, buf[a & 1], dreg);
+        AxisPassArgs<T> A;
+        fill_pass(A, p, a, cur, nxt, dreg);
+        A.clamp_lo = p.reg[a].index[a];
+        A.clamp_hi = p.reg[a].index[a] + p.reg[a].size[a] - 1;
+        A.accumulate = (a == 0) ? accumulate : 0;
+        double acc = 0.0;
+        for (int m = 0; m <= p.radii[a]; ++m) {
+            A.tail[m] = (T)acc;
+            acc += p.w[a][m];
+        }
+        const int64_t g_lo = p.reg[a + 1].index[a];
+        const int64_t g_hi = g_lo + p.reg[a + 1].size[a] - 1;
+        hipError_t e = launch_axis_adjoint_range<T>(A, ncomp, g_lo, g_hi, s);
+        if (e != hipSuccess) rc = hip_fail(e, "adjoint launch");
+        cur = nxt;
+    }
+    if (own) (void)hipFreeAsync(own, s);
+    return rc;
+}
+
+bool args_ok(int ndim, itkg_dtype dt, int ncomp) {
+    return ndim >= 1 && ndim <= ITKG_MAX_DIM && (dt == ITKG_F32 || dt == ITKG_F64) && ncomp >= 1 &&
+           ncomp <= 4;
+}
+
+}  // namespace
+
+// ---- batched plan -------------------------------------------------------
+
+struct BatchImpl {
+    std::vector<itkg_volume> vols;
+    std::vector<Plan> plans;
+    itkg_batch_direction dir;
+    itkg_arith arith;
+    void* scratch = nullptr;
+    size_t scratch_sz = 0;
+    int64_t voxels = 0;
+    std::mutex mu;
+    std::unordered_map<hipStream_t, hipGraphExec_t> graphs;
+};
+
+int batch_enqueue(BatchImpl& b, hipStream_t s);
+
+}  // namespace itkg
+
+struct itkg_batch {
+    itkg::BatchImpl impl;
+};
+
+using namespace itkg;
+
+extern "C" {
+
+uint64_t itkg_kernel_radius(double sigma, double spacing) { return radius_of(sigma, spacing); }
+
+int itkg_build_kernel(double sigma_px, uint64_t radius, double* out, uint64_t cap) {
+    clear_error();
+    if (!out) return fail(ITKG_ERR_INVALID_ARG, "null output");
+    if (radius > (uint64_t)ITKG_MAX_RADIUS)
+        return fail(ITKG_ERR_UNSUPPORTED, "radius exceeds %d", ITKG_MAX_RADIUS);
+    if (cap < 2 * radius + 1) return fail(ITKG_ERR_INVALID_ARG, "capacity below 2r+1");
+    weights_of(sigma_px, radius, out);
+    return ITKG_OK;
+}
+
+int itkg_input_region_for_output(int32_t ndim, double sigma, const double* spacing,
+                                 const itkg_region* largest, const itkg_region* output,
+                                 itkg_region* input) {
+    clear_error();
+    if (ndim < 1 || ndim > ITKG_MAX_DIM || !spacing || !largest || !output || !input)
+        return fail(ITKG_ERR_INVALID_ARG, "bad arguments");
+    int64_t radii[ITKG_MAX_DIM] = {0, 0, 0, 0};
+    for (int d = 0; d < ndim; ++d) {
+        const uint64_t r = radius_of(sigma, spacing[d]);
+        radii[d] = r > (uint64_t)INT64_MAX / 4 ? INT64_MAX / 4 : (int64_t)r;
+    }
+    itkg_region out{};
+    out = clipped(ndim, padded(ndim, *output, radii), *largest);
+    for (int d = ndim; d < ITKG_MAX_DIM; ++d) {
+        out.index[d] = 0;
+        out.size[d] = 0;
+    }
+    *input = out;
+    return ITKG_OK;
+}
+
+int itkg_split_region(int32_t ndim, const itkg_region* region, uint64_t num_pieces,
+                      itkg_region* pieces, uint64_t cap, uint64_t* count) {
+    clear_error();
+    if (ndim < 1 || ndim > ITKG_MAX_DIM || !region || !count)
+        return fail(ITKG_ERR_INVALID_ARG, "bad arguments");
+    const uint64_t n = num_pieces < 1 ? 1 : num_pieces;
+    *count = 0;
+    if (region_len(ndim, *region) == 0) return ITKG_OK;
+    const int slow = ndim - 1;
+    const uint64_t total = (uint64_t)region->size[slow];
+    const uint64_t actual = std::max<uint64_t>(std::min(n, total), 1);
+    if (!pieces || cap < actual) return fail(ITKG_ERR_INVALID_ARG, "piece capacity below %llu",
+                                             (unsigned long long)actual);
+    const uint64_t base = total / actual, remainder = total % actual;
+    int64_t start = region->index[slow];
+    for (uint64_t i = 0; i < actual; ++i) {
+        const uint64_t this_size = base + (i < remainder ? 1 : 0);
+        itkg_region p = *region;
+        p.index[slow] = start;
+        p.size[slow] = (int64_t)this_size;
+        pieces[i] = p;
+        start += (int64_t)this_size;
+    }
+    *count = actual;
+    return ITKG_OK;
+}
+
+int itkg_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                                  const double* spacing, const itkg_region* largest,
+                                  const itkg_view* input, const itkg_view* output,
+                                  itkg_arith arith, void* stream) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (!output) return fail(ITKG_ERR_INVALID_ARG, "null output view");
+    Plan p;
+    int rc = make_plan(ndim, sigma, spacing, largest, output->region, &p);
+    if (rc) return rc;
+    if ((rc = check_view(ndim, input, p.in_reg, "input"))) return rc;
+    if ((rc = check_view(ndim, output, p.req, "output"))) return rc;
+    return forward_impl(p, dtype, ncomp, *input, *output, arith,
+                        static_cast<hipStream_t>(stream), nullptr);
+}
+
+int itkg_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                       double sigma, const double* spacing,
+                                       const itkg_region* largest, const void* input,
+                                       const itkg_region* requested, void* output,
+                                       itkg_arith arith) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (!requested) return fail(ITKG_ERR_INVALID_ARG, "null requested region");
+    Plan p;
+    int rc = make_plan(ndim, sigma, spacing, largest, *requested, &p);
+    if (rc) return rc;
+    const int64_t nin = region_len(ndim, p.in_reg), nout = region_len(ndim, p.req);
+    if (nout == 0) return ITKG_OK;
+    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
+    const size_t px = dtype_size(dtype) * ncomp;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    void *din = nullptr, *dout = nullptr;
+    e = hipMallocAsync(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "hipMallocAsync(host staging)");
+    } else {
+        e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy in");
+        if (!rc) {
+            const itkg_view vin = dense_view(ndim, din, p.in_reg);
+            const itkg_view vout = dense_view(ndim, dout, p.req);
+            rc = forward_impl(p, dtype, ncomp, vin, vout, arith, s, nullptr);
+        }
+        if (!rc) {
+            e = hipMemcpyAsync(output, dout, (size_t)nout * px, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) rc = hip_fail(e, "copy out");
+        }
+    }
+    if (din) (void)hipFreeAsync(din, s);
+    if (dout) (void)hipFreeAsync(dout, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                           const double* spacing, const itkg_region* largest,
+                           const itkg_view* grad_output, const itkg_view* grad_input,
+                           int32_t accumulate, void* stream) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (!grad_output) return fail(ITKG_ERR_INVALID_ARG, "null grad_output view");
+    Plan p;
+    int rc = make_plan(ndim, sigma, spacing, largest, grad_output->region, &p);
+    if (rc) return rc;
+    if ((rc = check_view(ndim, grad_output, p.req, "grad_output"))) return rc;
+    if ((rc = check_view(ndim, grad_input, p.in_reg, "grad_input"))) return rc;
+    if (region_len(ndim, p.req) == 0) return ITKG_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return dtype == ITKG_F64
+               ? backward_per_axis<double>(p, ncomp, *grad_output, *grad_input, accumulate, s, nullptr)
+               : backward_per_axis<float>(p, ncomp, *grad_output, *grad_input, accumulate, s, nullptr);
+}
+
+// ---- batched ------------------------------------------------------------
+
+int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_direction direction,
+                      itkg_arith arith, itkg_batch** plan) {
+    clear_error();
+    if (n < 0 || (n > 0 && !volumes) || !plan) return fail(ITKG_ERR_INVALID_ARG, "bad arguments");
+    if (direction != ITKG_BATCH_FORWARD && direction != ITKG_BATCH_BACKWARD)
+        return fail(ITKG_ERR_INVALID_ARG, "bad direction");
+    auto* b = new (std::nothrow) itkg_batch;
+    if (!b) return fail(ITKG_ERR_OOM, "host allocation");
+    BatchImpl& B = b->impl;
+    B.dir = direction;
+    B.arith = arith;
+    size_t sc = 0;
+    for (int i = 0; i < n; ++i) {
+        const itkg_volume& v = volumes[i];
+        itkg_region r{};
+        for (int d = 0; d < 3; ++d) r.size[d] = v.size[d];
+        Plan p;
+        int rc = make_plan(3, v.sigma, v.spacing, &r, r, &p);
+        if (!rc && region_len(3, r) > 0 && (!v.input || !v.output))
+            rc = fail(ITKG_ERR_INVALID_ARG, "volume %d has a null buffer", i);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        B.vols.push_back(v);
+        B.plans.push_back(p);
+        B.voxels += region_len(3, r);
+        const bool fused = direction == ITKG_BATCH_FORWARD && use_fused(p, ITKG_F32, 1);
+        if (!fused) sc = std::max(sc, scratch_bytes(p, ITKG_F32, 1));
+    }
+    if (sc > 0) {
+        hipError_t e = hipMalloc(&B.scratch, sc);
+        if (e != hipSuccess) {
+            delete b;
+            return hip_fail(e, "hipMalloc(batch scratch)");
+        }
+        B.scratch_sz = sc;
+    }
+    *plan = b;
+    return ITKG_OK;
+}
+
+int itkg_batch_run(itkg_batch* plan, void* stream) {
+    clear_error();
+    if (!plan) return fail(ITKG_ERR_INVALID_ARG, "null plan");
+    BatchImpl& B = plan->impl;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!s) return batch_enqueue(B, s);  // the null stream cannot be captured
+    std::lock_guard<std::mutex> lock(B.mu);
+    auto it = B.graphs.find(s);
+    if (it == B.graphs.end()) {
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+        const int rc = batch_enqueue(B, s);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        it = B.graphs.emplace(s, ge).first;
+    }
+    hipError_t e = hipGraphLaunch(it->second, s);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return ITKG_OK;
+}
+
+int64_t itkg_batch_voxels(const itkg_batch* plan) { return plan ? plan->impl.voxels : 0; }
+
+int itkg_batch_destroy(itkg_batch* plan) {
+    clear_error();
+    if (!plan) return ITKG_OK;
+    for (auto& kv : plan->impl.graphs) (void)hipGraphExecDestroy(kv.second);
+    if (plan->impl.scratch) (void)hipFree(plan->impl.scratch);
+    delete plan;
+    return ITKG_OK;
+}
+
+const char* itkg_last_error(void) { return g_err.c_str(); }
+
+int itkg_set_path(itkg_path path) {
+    clear_error();
+    if (path != ITKG_PATH_AUTO && path != ITKG_PATH_PER_AXIS && path != ITKG_PATH_FUSED)
+        return fail(ITKG_ERR_INVALID_ARG, "bad path");
+    g_path.store(path);
+    return ITKG_OK;
+}
+
+const char* itkg_version(void) { return "itkg 0.1 (gfx950)"; }
+
+}  // extern "C"
+
+namespace itkg {
+
+// Loop-of-volumes enqueue used by the batched plan (captured into a graph by
+// itkg_batch_run).
+int batch_enqueue(BatchImpl& B, hipStream_t s) {
+    for (size_t i = 0; i < B.vols.size(); ++i) {
+        const itkg_volume& v = B.vols[i];
+        const Plan& p = B.plans[i];
+        if (region_len(3, p.req) == 0) continue;
+        const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
+        const itkg_view vout = dense_view(3, v.output, p.req);
+        int rc;
+        if (B.dir == ITKG_BATCH_FORWARD)
+            rc = forward_impl(p, ITKG_F32, 1, vin, vout, B.arith, s, B.scratch);
+        else
+            rc = backward_per_axis<float>(p, 1, vin, vout, 0, s, B.scratch);
+        if (rc) return rc;
+    }
+    return ITKG_OK;
+}
+
+}  // namespace itkg

@@ -1,0 +1,408 @@
+/*
+ * gauss_oracle.c -- CPU ORACLE FOR TESTS ONLY.
+ *
+ * This file is test infrastructure.  Only tests/, the smoke check in
+ * __graft_entry__.py and the cpu_baseline leg of bench.py may load it, and
+ * only as the checker (or as the timed CPU baseline), never as a product code
+ * path.  The product library (libitkg.so) does not link or call it.
+ *
+ * It restates, in plain C, the itk-rs separable Gaussian path:
+ *
+ *   kernel_radius          src/filters/gaussian.rs:24-27
+ *   build_kernel           src/filters/gaussian.rs:29-40
+ *   generate_region        src/filters/gaussian.rs:59-90
+ *   convolve_axis          src/filters/conv.rs:11-41
+ *   Region::padded_per_axis / clipped_to   src/image.rs:28-37, 52-64
+ *   Image::flat_index      src/image.rs:111-120   (axis 0 fastest)
+ *   f32 / f64 / VecPixel scale+add         src/pixel.rs:19-31, 71-99
+ *   leaf Image::generate_region (clipped copy)   src/source.rs:42-58
+ *
+ * Arithmetic contract (what makes the restatement bit-exact):
+ *   - weights are computed in f64 with glibc exp (the symbol Rust's f64::exp
+ *     calls on linux-gnu), normalised by a sequential f64 sum;
+ *   - each tap is fl32(x * (float)w_k) and is added to an f32 accumulator
+ *     that starts at +0.0, taps in order k = 0..2r, with no FMA
+ *     (build with -ffp-contract=off);
+ *   - each pass writes f32; passes run in axis order 0..D-1 and clamp to the
+ *     pass input's region on the pass axis only.
+ *
+ * Pinning (each function is checked by tests/test_oracle.py):
+ *   kernel_radius / build_kernel : closed form, symmetry, unit sum, radius
+ *                                  boundary cases (3*(s/sp) evaluation order);
+ *   generate_region (f32)        : bit-exact against an independent numpy
+ *                                  float32 restatement, within 1e-6 of
+ *                                  scipy.ndimage.correlate1d(mode='nearest'),
+ *                                  interior-impulse closed form, the
+ *                                  reference's own property tests, brute-force
+ *                                  dense Kronecker operator in f64;
+ *   generate_region (f64)        : dense Kronecker operator (1e-12);
+ *   adjoint (f64 scatter)        : dense transpose of the Kronecker operator
+ *                                  and <Ax, y> = <x, A^T y>;
+ *   ref_faithful (threaded)      : bit-exact against generate_region.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_MAX_DIM 4
+
+/* ---- region algebra: src/image.rs:17-77 -------------------------------- */
+
+typedef struct {
+    int64_t index[ORC_MAX_DIM];
+    int64_t size[ORC_MAX_DIM];
+} orc_region;
+
+static orc_region region_make(int D, const int64_t* idx, const int64_t* sz) {
+    orc_region r;
+    memset(&r, 0, sizeof r);
+    for (int d = 0; d < D; ++d) { r.index[d] = idx[d]; r.size[d] = sz[d]; }
+    return r;
+}
+
+/* src/image.rs:28-37 */
+static orc_region region_padded(int D, orc_region r, const uint64_t* radii) {
+    for (int d = 0; d < D; ++d) {
+        r.index[d] -= (int64_t)radii[d];
+        r.size[d] += 2 * (int64_t)radii[d];
+    }
+    return r;
+}
+
+/* src/image.rs:52-64 */
+static orc_region region_clipped(int D, orc_region r, orc_region b) {
+    for (int d = 0; d < D; ++d) {
+        int64_t self_end = r.index[d] + r.size[d];
+        int64_t b_end = b.index[d] + b.size[d];
+        int64_t s = r.index[d] > b.index[d] ? r.index[d] : b.index[d];
+        int64_t e = self_end < b_end ? self_end : b_end;
+        r.index[d] = s;
+        r.size[d] = e - s > 0 ? e - s : 0;
+    }
+    return r;
+}
+
+static int64_t region_len(int D, orc_region r) {
+    int64_t n = 1;
+    for (int d = 0; d < D; ++d) n *= r.size[d];
+    return n;
+}
+
+/* src/image.rs:111-120, axis 0 fastest */
+static int64_t flat_index(int D, orc_region r, const int64_t* idx) {
+    int64_t off = 0, stride = 1;
+    for (int d = 0; d < D; ++d) {
+        off += (idx[d] - r.index[d]) * stride;
+        stride *= r.size[d];
+    }
+    return off;
+}
+
+/* ---- kernel: src/filters/gaussian.rs:24-40 ----------------------------- */
+
+/* `(3.0 * (sigma / spacing)).ceil() as usize`; Rust's float->usize cast
+ * saturates: NaN and negatives become 0. */
+uint64_t orc_kernel_radius(double sigma, double spacing) {
+    double sigma_px = sigma / spacing;
+    double v = ceil(3.0 * sigma_px);
+    if (!(v > 0.0)) return 0;
+    if (v >= 18446744073709551616.0) return UINT64_MAX;
+    return (uint64_t)v;
+}
+
+/* w_i = exp(-0.5 * ((i - r) / sigma_px)^2), normalised by the sequential
+ * f64 sum.  powi(2) is a single multiply. */
+void orc_build_kernel(double sigma_px, uint64_t radius, double* k) {
+    uint64_t len = 2 * radius + 1;
+    for (uint64_t i = 0; i < len; ++i) {
+        double x = (double)i - (double)radius;
+        double t = x / sigma_px;
+        k[i] = exp(-0.5 * (t * t));
+    }
+    double sum = 0.0;
+    for (uint64_t i = 0; i < len; ++i) sum += k[i];
+    for (uint64_t i = 0; i < len; ++i) k[i] /= sum;
+}
+
+/* ---- convolve_axis: src/filters/conv.rs:11-41 --------------------------- */
+
+#define DEFINE_CONV(NAME, T, WCAST)                                              \
+static void NAME(int D, int ncomp, const T* src, orc_region sreg, int axis,      \
+                 const double* kern, uint64_t klen, orc_region req, T* out) {    \
+    int64_t n = region_len(D, req);                                              \
+    if (n == 0) return;                                                          \
+    int64_t radius = (int64_t)(klen / 2);                                        \
+    int64_t lo = sreg.index[axis];                                               \
+    int64_t hi = sreg.index[axis] + sreg.size[axis] - 1;                         \
+    int64_t idx[ORC_MAX_DIM];                                                    \
+    for (int d = 0; d < D; ++d) idx[d] = req.index[d];                           \
+    for (int64_t o = 0; o < n; ++o) {                                            \
+        for (int c = 0; c < ncomp; ++c) {                                        \
+            T acc = (T)0.0;                                                      \
+            for (uint64_t k = 0; k < klen; ++k) {                                \
+                int64_t s[ORC_MAX_DIM];                                          \
+                for (int d = 0; d < D; ++d) s[d] = idx[d];                       \
+                int64_t v = idx[axis] + (int64_t)k - radius;                     \
+                if (v < lo) v = lo;                                              \
+                if (v > hi) v = hi;                                              \
+                s[axis] = v;                                                     \
+                T x = src[flat_index(D, sreg, s) * ncomp + c];                   \
+                acc = acc + x * (T)WCAST(kern[k]);                               \
+            }                                                                    \
+            out[o * ncomp + c] = acc;                                            \
+        }                                                                        \
+        for (int d = 0; d < D; ++d) {                                            \
+            if (++idx[d] < req.index[d] + req.size[d]) break;                    \
+            idx[d] = req.index[d];                                               \
+        }                                                                        \
+    }                                                                            \
+}
+
+#define AS_F32(w) ((float)(w))
+#define AS_F64(w) (w)
+DEFINE_CONV(conv_axis_f32, float, AS_F32)
+DEFINE_CONV(conv_axis_f64, double, AS_F64)
+
+/* Leaf source (src/source.rs:42-58): clipped copy of `img` to `want`. */
+#define DEFINE_LEAF(NAME, T)                                                     \
+static T* NAME(int D, int ncomp, const T* img, orc_region ireg, orc_region want, \
+               orc_region* got) {                                                \
+    orc_region c = region_clipped(D, want, ireg);                                \
+    int64_t n = region_len(D, c);                                                \
+    T* out = (T*)malloc((size_t)(n > 0 ? n : 1) * ncomp * sizeof(T));           \
+    int64_t idx[ORC_MAX_DIM];                                                    \
+    for (int d = 0; d < D; ++d) idx[d] = c.index[d];                             \
+    for (int64_t o = 0; o < n; ++o) {                                            \
+        int64_t si = flat_index(D, ireg, idx);                                   \
+        for (int k = 0; k < ncomp; ++k) out[o * ncomp + k] = img[si * ncomp + k];\
+        for (int d = 0; d < D; ++d) {                                            \
+            if (++idx[d] < c.index[d] + c.size[d]) break;                        \
+            idx[d] = c.index[d];                                                 \
+        }                                                                        \
+    }                                                                            \
+    *got = c;                                                                    \
+    return out;                                                                  \
+}
+DEFINE_LEAF(leaf_copy_f32, float)
+DEFINE_LEAF(leaf_copy_f64, double)
+
+/* ---- generate_region: src/filters/gaussian.rs:59-90 -------------------- */
+/* The upstream is a leaf Image whose region is the largest region (bounds).
+ * `out` receives linear_len(requested) * ncomp values.  Returns 0 on success,
+ * -1 on a bad argument. */
+#define DEFINE_GEN(NAME, T, LEAF, CONV)                                          \
+int NAME(int D, int ncomp, const T* img, const int64_t* img_index,               \
+         const int64_t* img_size, const double* spacing, double sigma,           \
+         const int64_t* req_index, const int64_t* req_size, T* out) {            \
+    if (D < 1 || D > ORC_MAX_DIM || ncomp < 1) return -1;                        \
+    orc_region bounds = region_make(D, img_index, img_size);                     \
+    orc_region req = region_make(D, req_index, req_size);                        \
+    uint64_t radii[ORC_MAX_DIM] = {0};                                           \
+    for (int d = 0; d < D; ++d) radii[d] = orc_kernel_radius(sigma, spacing[d]); \
+    orc_region in_reg = region_clipped(D, region_padded(D, req, radii), bounds); \
+    orc_region cur_reg;                                                          \
+    T* cur = LEAF(D, ncomp, img, bounds, in_reg, &cur_reg);                      \
+    for (int axis = 0; axis < D; ++axis) {                                       \
+        double sigma_px = sigma / spacing[axis];                                 \
+        uint64_t klen = 2 * radii[axis] + 1;                                     \
+        double* kern = (double*)malloc(klen * sizeof(double));                   \
+        orc_build_kernel(sigma_px, radii[axis], kern);                           \
+        orc_region out_reg;                                                      \
+        if (axis + 1 < D) {                                                      \
+            uint64_t rem[ORC_MAX_DIM] = {0};                                     \
+            for (int d = axis + 1; d < D; ++d) rem[d] = radii[d];                \
+            out_reg = region_clipped(D, region_padded(D, req, rem), bounds);     \
+        } else {                                                                 \
+            out_reg = req;                                                       \
+        }                                                                        \
+        int64_t n = region_len(D, out_reg);                                      \
+        T* nxt = (T*)malloc((size_t)(n > 0 ? n : 1) * ncomp * sizeof(T));       \
+        CONV(D, ncomp, cur, cur_reg, axis, kern, klen, out_reg, nxt);            \
+        free(kern);                                                              \
+        free(cur);                                                               \
+        cur = nxt;                                                               \
+        cur_reg = out_reg;                                                       \
+    }                                                                            \
+    memcpy(out, cur, (size_t)region_len(D, req) * ncomp * sizeof(T));            \
+    free(cur);                                                                   \
+    return 0;                                                                    \
+}
+DEFINE_GEN(orc_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
+DEFINE_GEN(orc_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
+
+/* ---- adjoint in f64, by scattering the forward taps -------------------- */
+/* No reference counterpart.  The forward operator is A = A_{D-1} ... A_0
+ * (pass 0 first).  A^T = A_0^T ... A_{D-1}^T, so the passes are undone in
+ * reverse axis order; each pass transposes convolve_axis literally: every
+ * (output voxel, tap) pair that read src[clamp(...)] * w_k now adds
+ * g_out * w_k into that same source voxel.  `gin` receives the gradient on
+ * the clipped padded input region (the input_region_for_output of
+ * `requested`), dense, axis 0 fastest. */
+int orc_adjoint_f64(int D, int ncomp, const int64_t* img_index, const int64_t* img_size,
+                    const double* spacing, double sigma, const int64_t* req_index,
+                    const int64_t* req_size, const double* gout, double* gin,
+                    int64_t* in_index_out, int64_t* in_size_out) {
+    if (D < 1 || D > ORC_MAX_DIM || ncomp < 1) return -1;
+    orc_region bounds = region_make(D, img_index, img_size);
+    orc_region req = region_make(D, req_index, req_size);
+    uint64_t radii[ORC_MAX_DIM] = {0};
+    for (int d = 0; d < D; ++d) radii[d] = orc_kernel_radius(sigma, spacing[d]);
+    orc_region in_reg = region_clipped(D, region_padded(D, req, radii), bounds);
+    /* regions of every pass: reg[a] = input of pass a, reg[D] = requested */
+    orc_region reg[ORC_MAX_DIM + 1];
+    reg[0] = in_reg;
+    for (int a = 0; a < D; ++a) {
+        if (a + 1 < D) {
+            uint64_t rem[ORC_MAX_DIM] = {0};
+            for (int d = a + 1; d < D; ++d) rem[d] = radii[d];
+            reg[a + 1] = region_clipped(D, region_padded(D, req, rem), bounds);
+        } else {
+            reg[a + 1] = req;
+        }
+    }
+    int64_t n = region_len(D, req);
+    double* g = (double*)malloc((size_t)(n > 0 ? n : 1) * ncomp * sizeof(double));
+    memcpy(g, gout, (size_t)n * ncomp * sizeof(double));
+    for (int axis = D - 1; axis >= 0; --axis) {
+        orc_region sreg = reg[axis], oreg = reg[axis + 1];
+        uint64_t klen = 2 * radii[axis] + 1;
+        int64_t radius = (int64_t)radii[axis];
+        double* kern = (double*)malloc(klen * sizeof(double));
+        orc_build_kernel(sigma / spacing[axis], radii[axis], kern);
+        int64_t ns = region_len(D, sreg), no = region_len(D, oreg);
+        double* gs = (double*)calloc((size_t)(ns > 0 ? ns : 1) * ncomp, sizeof(double));
+        int64_t lo = sreg.index[axis], hi = sreg.index[axis] + sreg.size[axis] - 1;
+        int64_t idx[ORC_MAX_DIM];
+        for (int d = 0; d < D; ++d) idx[d] = oreg.index[d];
+        for (int64_t o = 0; o < no; ++o) {
+            for (uint64_t k = 0; k < klen; ++k) {
+                int64_t s[ORC_MAX_DIM];
+                for (int d = 0; d < D; ++d) s[d] = idx[d];
+                int64_t v = idx[axis] + (int64_t)k - radius;
+                if (v < lo) v = lo;
+                if (v > hi) v = hi;
+                s[axis] = v;
+                int64_t si = flat_index(D, sreg, s);
+                for (int c = 0; c < ncomp; ++c) gs[si * ncomp + c] += g[o * ncomp + c] * kern[k];
+            }
+            for (int d = 0; d < D; ++d) {
+                if (++idx[d] < oreg.index[d] + oreg.size[d]) break;
+                idx[d] = oreg.index[d];
+            }
+        }
+        free(kern);
+        free(g);
+        g = gs;
+    }
+    memcpy(gin, g, (size_t)region_len(D, in_reg) * ncomp * sizeof(double));
+    free(g);
+    for (int d = 0; d < D; ++d) { in_index_out[d] = in_reg.index[d]; in_size_out[d] = in_reg.size[d]; }
+    return 0;
+}
+
+/* ---- "ref-faithful" threaded variant (CPU baseline) -------------------- */
+/* Same arithmetic as orc_generate_region_f32, with the reference's loop
+ * structure: leaf clipped copy (src/source.rs:42-58), a materialised
+ * Vec<Index<D>> per pass (src/filters/conv.rs:21-22), a parallel map over
+ * output voxels with a per-tap clamp and flat_index (conv.rs:24-37) run on a
+ * pool of `nthreads` workers that pull dynamic chunks (the work-splitting
+ * rayon's par_iter performs), and an order-preserving collect. */
+
+typedef struct {
+    int D, ncomp, axis;
+    const float* src;
+    orc_region sreg;
+    const float* wf;
+    int64_t klen, radius;
+    const int64_t* indices; /* D int64 per voxel */
+    int64_t n;
+    float* out;
+    int64_t chunk;
+    int64_t next; /* atomic */
+} ref_job;
+
+static void* ref_worker(void* arg) {
+    ref_job* j = (ref_job*)arg;
+    const int D = j->D, nc = j->ncomp, axis = j->axis;
+    const int64_t lo = j->sreg.index[axis], hi = j->sreg.index[axis] + j->sreg.size[axis] - 1;
+    for (;;) {
+        int64_t b = __atomic_fetch_add(&j->next, j->chunk, __ATOMIC_RELAXED);
+        if (b >= j->n) break;
+        int64_t e = b + j->chunk < j->n ? b + j->chunk : j->n;
+        for (int64_t o = b; o < e; ++o) {
+            const int64_t* oi = j->indices + o * D;
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int64_t k = 0; k < j->klen; ++k) {
+                int64_t s[ORC_MAX_DIM];
+                for (int d = 0; d < D; ++d) s[d] = oi[d];
+                int64_t v = oi[axis] + k - j->radius;
+                if (v < lo) v = lo;
+                if (v > hi) v = hi;
+                s[axis] = v;
+                int64_t si = flat_index(D, j->sreg, s);
+                for (int c = 0; c < nc; ++c) acc[c] = acc[c] + j->src[si * nc + c] * j->wf[k];
+            }
+            for (int c = 0; c < nc; ++c) j->out[o * nc + c] = acc[c];
+        }
+    }
+    return NULL;
+}
+
+int orc_ref_faithful_f32(int D, int ncomp, const float* img, const int64_t* img_index,
+                         const int64_t* img_size, const double* spacing, double sigma,
+                         const int64_t* req_index, const int64_t* req_size, float* out,
+                         int nthreads) {
+    if (D < 1 || D > ORC_MAX_DIM || ncomp < 1 || ncomp > 4 || nthreads < 1) return -1;
+    orc_region bounds = region_make(D, img_index, img_size);
+    orc_region req = region_make(D, req_index, req_size);
+    uint64_t radii[ORC_MAX_DIM] = {0};
+    for (int d = 0; d < D; ++d) radii[d] = orc_kernel_radius(sigma, spacing[d]);
+    orc_region in_reg = region_clipped(D, region_padded(D, req, radii), bounds);
+    orc_region cur_reg;
+    float* cur = leaf_copy_f32(D, ncomp, img, bounds, in_reg, &cur_reg);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    for (int axis = 0; axis < D; ++axis) {
+        int64_t klen = 2 * (int64_t)radii[axis] + 1;
+        double* kern = (double*)malloc((size_t)klen * sizeof(double));
+        float* wf = (float*)malloc((size_t)klen * sizeof(float));
+        orc_build_kernel(sigma / spacing[axis], radii[axis], kern);
+        for (int64_t k = 0; k < klen; ++k) wf[k] = (float)kern[k];
+        orc_region out_reg;
+        if (axis + 1 < D) {
+            uint64_t rem[ORC_MAX_DIM] = {0};
+            for (int d = axis + 1; d < D; ++d) rem[d] = radii[d];
+            out_reg = region_clipped(D, region_padded(D, req, rem), bounds);
+        } else {
+            out_reg = req;
+        }
+        int64_t n = region_len(D, out_reg);
+        /* materialised index list, built sequentially (conv.rs:21-22) */
+        int64_t* indices = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * D * sizeof(int64_t));
+        int64_t idx[ORC_MAX_DIM];
+        for (int d = 0; d < D; ++d) idx[d] = out_reg.index[d];
+        for (int64_t o = 0; o < n; ++o) {
+            for (int d = 0; d < D; ++d) indices[o * D + d] = idx[d];
+            for (int d = 0; d < D; ++d) {
+                if (++idx[d] < out_reg.index[d] + out_reg.size[d]) break;
+                idx[d] = out_reg.index[d];
+            }
+        }
+        float* nxt = (float*)malloc((size_t)(n > 0 ? n : 1) * ncomp * sizeof(float));
+        ref_job job = {D, ncomp, axis, cur, cur_reg, wf, klen, (int64_t)radii[axis],
+                       indices, n, nxt, 1024, 0};
+        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, ref_worker, &job);
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+        free(indices);
+        free(kern);
+        free(wf);
+        free(cur);
+        cur = nxt;
+        cur_reg = out_reg;
+    }
+    free(th);
+    memcpy(out, cur, (size_t)region_len(D, req) * ncomp * sizeof(float));
+    free(cur);
+    return 0;
+}

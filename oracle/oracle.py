@@ -1,0 +1,271 @@
+"""CPU oracle for the itk-rs separable Gaussian -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check in __graft_entry__.py and bench.py's
+cpu_baseline leg may import this module, and only as the checker (or as the
+timed CPU baseline).  The product path never imports it.
+
+Three independent restatements live here:
+
+* ``generate_region`` / ``adjoint_f64`` / ``ref_faithful``: ctypes calls into
+  ``liboracle.so`` (gauss_oracle.c), the loop-for-loop C restatement of
+  src/filters/gaussian.rs:24-90 and src/filters/conv.rs:11-41.
+* ``np_generate_region``: a numpy float32 restatement written independently
+  (vectorised over every axis but the pass axis, taps in order k = 0..2r,
+  separate multiply and add).  It must agree with the C oracle to the bit.
+* ``dense_operator``: the explicit f64 matrix of one clamp-to-edge pass
+  (A[i, clamp(i + off + k - r)] += w_k), applied as a Kronecker product --
+  the brute-force check for the forward and the explicit transpose for the
+  adjoint.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        f64p = ctypes.POINTER(ctypes.c_double)
+        L.orc_kernel_radius.restype = ctypes.c_uint64
+        L.orc_kernel_radius.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_build_kernel.restype = None
+        L.orc_build_kernel.argtypes = [ctypes.c_double, ctypes.c_uint64, f64p]
+        for name in ("orc_generate_region_f32", "orc_generate_region_f64"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, i64p, i64p, f64p,
+                           ctypes.c_double, i64p, i64p, ctypes.c_void_p]
+        L.orc_ref_faithful_f32.restype = ctypes.c_int
+        L.orc_ref_faithful_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, i64p, i64p,
+                                           f64p, ctypes.c_double, i64p, i64p, ctypes.c_void_p,
+                                           ctypes.c_int]
+        L.orc_adjoint_f64.restype = ctypes.c_int
+        L.orc_adjoint_f64.argtypes = [ctypes.c_int, ctypes.c_int, i64p, i64p, f64p, ctypes.c_double,
+                                      i64p, i64p, ctypes.c_void_p, ctypes.c_void_p, i64p, i64p]
+        _lib = L
+    return _lib
+
+
+def _i64(v):
+    a = (ctypes.c_int64 * len(v))(*[int(x) for x in v])
+    return a
+
+
+def _f64(v):
+    return (ctypes.c_double * len(v))(*[float(x) for x in v])
+
+
+# ---------------------------------------------------------------------------
+# region algebra (src/image.rs:28-64)
+# ---------------------------------------------------------------------------
+
+def kernel_radius(sigma: float, spacing: float) -> int:
+    """src/filters/gaussian.rs:24-27 (via the C oracle)."""
+    return int(lib().orc_kernel_radius(float(sigma), float(spacing)))
+
+
+def build_kernel(sigma_px: float, radius: int) -> np.ndarray:
+    """src/filters/gaussian.rs:29-40 (via the C oracle), f64 weights."""
+    out = np.zeros(2 * radius + 1, dtype=np.float64)
+    lib().orc_build_kernel(float(sigma_px), int(radius),
+                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def padded(index, size, radii):
+    return ([i - r for i, r in zip(index, radii)], [s + 2 * r for s, r in zip(size, radii)])
+
+
+def clipped(index, size, bidx, bsize):
+    ni, ns = [], []
+    for i, s, bi, bs in zip(index, size, bidx, bsize):
+        st = max(i, bi)
+        en = min(i + s, bi + bs)
+        ni.append(st)
+        ns.append(max(en - st, 0))
+    return ni, ns
+
+
+def _shape(size, ncomp):
+    """numpy shape of a dense region buffer: axis 0 fastest -> reversed."""
+    shp = tuple(int(s) for s in reversed(size))
+    return shp + ((ncomp,) if ncomp > 1 else ())
+
+
+# ---------------------------------------------------------------------------
+# C oracle entry points
+# ---------------------------------------------------------------------------
+
+def generate_region(img: np.ndarray, img_index, spacing, sigma, req_index, req_size,
+                    ncomp: int = 1) -> np.ndarray:
+    """GaussianFilter{source: img, sigma}.generate_region(req) via the C oracle.
+
+    ``img`` is a dense numpy array in numpy order (slowest axis first), with a
+    trailing component axis when ncomp > 1.  The dtype selects f32 or f64.
+    """
+    D = len(spacing)
+    img = np.ascontiguousarray(img)
+    img_size = list(reversed(img.shape[:D]))
+    fn = lib().orc_generate_region_f32 if img.dtype == np.float32 else lib().orc_generate_region_f64
+    if img.dtype not in (np.float32, np.float64):
+        raise TypeError("oracle supports float32 and float64 pixels")
+    out = np.zeros(_shape(req_size, ncomp), dtype=img.dtype)
+    rc = fn(D, ncomp, img.ctypes.data, _i64(img_index), _i64(img_size), _f64(spacing), float(sigma),
+            _i64(req_index), _i64(req_size), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    return out
+
+
+def ref_faithful(img: np.ndarray, img_index, spacing, sigma, req_index, req_size, ncomp=1,
+                 nthreads=None) -> np.ndarray:
+    """The reference's loop structure on `nthreads` host threads (f32 only)."""
+    D = len(spacing)
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    img_size = list(reversed(img.shape[:D]))
+    if nthreads is None:
+        nthreads = os.cpu_count() or 1
+    out = np.zeros(_shape(req_size, ncomp), dtype=np.float32)
+    rc = lib().orc_ref_faithful_f32(D, ncomp, img.ctypes.data, _i64(img_index), _i64(img_size),
+                                    _f64(spacing), float(sigma), _i64(req_index), _i64(req_size),
+                                    out.ctypes.data, int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    return out
+
+
+def adjoint_f64(gout: np.ndarray, img_index, img_size, spacing, sigma, req_index, req_size,
+                ncomp=1):
+    """Adjoint (transpose) of generate_region in f64 by scattered taps.
+
+    Returns (gin, in_index, in_size) where gin lives on the clipped padded
+    input region."""
+    D = len(spacing)
+    gout = np.ascontiguousarray(gout, dtype=np.float64)
+    radii = [kernel_radius(sigma, sp) for sp in spacing]
+    pi, ps = padded(req_index, req_size, radii)
+    ii, isz = clipped(pi, ps, img_index, img_size)
+    gin = np.zeros(_shape(isz, ncomp), dtype=np.float64)
+    oi = (ctypes.c_int64 * D)()
+    os_ = (ctypes.c_int64 * D)()
+    rc = lib().orc_adjoint_f64(D, ncomp, _i64(img_index), _i64(img_size), _f64(spacing),
+                               float(sigma), _i64(req_index), _i64(req_size), gout.ctypes.data,
+                               gin.ctypes.data, oi, os_)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    assert list(oi) == ii and list(os_) == isz
+    return gin, ii, isz
+
+
+# ---------------------------------------------------------------------------
+# independent numpy float32 restatement (checks the C oracle bit for bit)
+# ---------------------------------------------------------------------------
+
+def _np_radius(sigma, spacing):
+    v = math.ceil(3.0 * (sigma / spacing)) if not math.isnan(sigma / spacing) else 0
+    return max(int(v), 0)
+
+
+def _np_kernel(sigma_px, r):
+    # math.exp is the C library's exp (what Rust's f64::exp calls); numpy's
+    # vectorised exp can differ in the last ulp.
+    x = np.arange(2 * r + 1, dtype=np.float64) - float(r)
+    t = x / sigma_px
+    k = np.array([math.exp(v) for v in (-0.5 * (t * t))], dtype=np.float64)
+    s = 0.0
+    for v in k:  # sequential sum, as Iterator::sum folds
+        s += float(v)
+    return k / s
+
+
+def np_generate_region(img: np.ndarray, img_index, spacing, sigma, req_index, req_size,
+                       ncomp: int = 1) -> np.ndarray:
+    """numpy restatement; float dtype of `img` is kept for every pass."""
+    D = len(spacing)
+    T = img.dtype.type
+    size = list(reversed(img.shape[:D]))
+    radii = [_np_radius(sigma, sp) for sp in spacing]
+    pi, ps = padded(req_index, req_size, radii)
+    in_i, in_s = clipped(pi, ps, img_index, size)
+    # leaf clipped copy
+    sl = tuple(slice(in_i[d] - img_index[d], in_i[d] - img_index[d] + in_s[d])
+               for d in reversed(range(D)))
+    cur = np.array(img[sl], dtype=img.dtype, copy=True)
+    cur_i, cur_s = list(in_i), list(in_s)
+    for a in range(D):
+        w = _np_kernel(sigma / spacing[a], radii[a])
+        if a + 1 < D:
+            rem = [0] * D
+            for d in range(a + 1, D):
+                rem[d] = radii[d]
+            oi, os_ = clipped(*padded(req_index, req_size, rem), img_index, size)
+        else:
+            oi, os_ = list(req_index), list(req_size)
+        np_axis = D - 1 - a
+        # the other axes of cur already equal the output's (see generate_region)
+        lo, hi = cur_i[a], cur_i[a] + cur_s[a] - 1
+        out_coords = np.arange(oi[a], oi[a] + os_[a])
+        acc = np.zeros(_shape(os_, ncomp), dtype=img.dtype)
+        for k in range(2 * radii[a] + 1):
+            src = np.clip(out_coords + k - radii[a], lo, hi) - cur_i[a]
+            taken = np.take(cur, src, axis=np_axis)
+            acc = acc + taken * T(w[k])
+        cur, cur_i, cur_s = acc, oi, os_
+    return cur
+
+
+# ---------------------------------------------------------------------------
+# explicit operator (brute force forward, explicit transpose for the adjoint)
+# ---------------------------------------------------------------------------
+
+def axis_matrix(w: np.ndarray, out_lo: int, out_n: int, in_lo: int, in_n: int) -> np.ndarray:
+    """Dense f64 matrix of one clamp-to-edge pass along one axis."""
+    r = (len(w) - 1) // 2
+    A = np.zeros((out_n, in_n), dtype=np.float64)
+    for i in range(out_n):
+        for k in range(len(w)):
+            j = min(max(out_lo + i + k - r, in_lo), in_lo + in_n - 1) - in_lo
+            A[i, j] += w[k]
+    return A
+
+
+def dense_operator(img_index, img_size, spacing, sigma, req_index, req_size):
+    """Per-axis matrices [A_0, ..., A_{D-1}] and the input region they act on."""
+    D = len(spacing)
+    radii = [_np_radius(sigma, sp) for sp in spacing]
+    in_i, in_s = clipped(*padded(req_index, req_size, radii), img_index, img_size)
+    mats = []
+    for a in range(D):
+        w = _np_kernel(sigma / spacing[a], radii[a])
+        mats.append(axis_matrix(w, req_index[a], req_size[a], in_i[a], in_s[a]))
+    return mats, in_i, in_s
+
+
+def apply_dense(mats, x: np.ndarray, transpose=False) -> np.ndarray:
+    """Apply the separable operator (or its transpose) to x (numpy order)."""
+    D = len(mats)
+    y = np.asarray(x, dtype=np.float64)
+    for a in range(D):
+        M = mats[a].T if transpose else mats[a]
+        np_axis = D - 1 - a
+        y = np.moveaxis(np.tensordot(M, y, axes=([1], [np_axis])), 0, np_axis)
+    return y

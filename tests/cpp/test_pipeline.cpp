@@ -1,0 +1,182 @@
+// The reference's pipeline tests (tests/pipeline.rs:19-153) and the Gaussian
+// unit tests (src/filters/gaussian.rs:107-127), written against the C++ host
+// API (itk-rs_amd/host/itk.hpp), whose GaussianFilter runs on the GPU.
+// Prints one "ok <name>" line per test; exits non-zero on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "../../itk-rs_amd/host/itk.hpp"
+
+using namespace itk;
+
+#define CHECK(cond, ...)                                           \
+    do {                                                           \
+        if (!(cond)) {                                             \
+            std::printf("FAIL %s:%d: ", __func__, __LINE__);       \
+            std::printf(__VA_ARGS__);                              \
+            std::printf("\n");                                     \
+            std::exit(1);                                          \
+        }                                                          \
+    } while (0)
+
+template <size_t D>
+Image<float, D> solid_image(std::array<uint64_t, D> size, float fill) {
+    std::array<int64_t, D> z{};
+    std::array<double, D> one, zero{};
+    one.fill(1.0);
+    return Image<float, D>::allocate(Region<D>::make(z, size), one, zero, fill);
+}
+
+static Image<float, 2> impulse_image_2d(uint64_t size) {
+    const int64_t mid = (int64_t)(size / 2);
+    auto img = solid_image<2>({size, size}, 0.0f);
+    img.set_pixel(Index<2>{{mid, mid}}, 1.0f);
+    return img;
+}
+
+static float sum(const std::vector<float>& v) {
+    float s = 0.0f;
+    for (float x : v) s += x;
+    return s;
+}
+
+// src/filters/gaussian.rs:107-114
+static void test_gaussian_impulse_energy_conserved() {
+    auto filter = make_gaussian(impulse_image_2d(21), 1.5);
+    auto out = filter.generate_region(filter.largest_region());
+    const float s = sum(out.data);
+    CHECK(std::fabs(s - 1.0f) < 0.01f, "energy not conserved: sum=%g", s);
+}
+
+// src/filters/gaussian.rs:116-127
+static void test_gaussian_peak_at_centre() {
+    auto filter = make_gaussian(impulse_image_2d(21), 1.5);
+    auto out = filter.generate_region(filter.largest_region());
+    const float peak = out.get_pixel(Index<2>{{10, 10}});
+    for (float v : out.data) CHECK(v <= peak + 1e-6f, "pixel %g > peak %g", v, peak);
+}
+
+// tests/pipeline.rs:19-31
+static void test_image_source_identity_2d() {
+    auto img = solid_image<2>({8, 8}, 0.0f);
+    img.set_pixel(Index<2>{{3, 5}}, 42.0f);
+    const auto sub = Region<2>::make({2, 4}, {4, 3});
+    auto cropped = img.generate_region(sub);
+    CHECK(cropped.region == sub, "region");
+    CHECK(cropped.get_pixel(Index<2>{{3, 5}}) == 42.0f, "marked pixel");
+    CHECK(cropped.get_pixel(Index<2>{{2, 4}}) == 0.0f, "corner");
+}
+
+// tests/pipeline.rs:37-54
+static void test_gaussian_impulse_2d() {
+    auto filter = make_gaussian(impulse_image_2d(21), 1.5);
+    auto out = filter.generate_region(filter.largest_region());
+    const float s = sum(out.data);
+    CHECK(std::fabs(s - 1.0f) < 0.01f, "energy not conserved: sum=%g", s);
+    const float peak = out.get_pixel(Index<2>{{10, 10}});
+    for (float v : out.data) CHECK(v <= peak + 1e-6f, "pixel %g > peak %g", v, peak);
+}
+
+// tests/pipeline.rs:60-72
+static void test_gaussian_3d() {
+    auto img = solid_image<3>({11, 11, 11}, 0.0f);
+    img.set_pixel(Index<3>{{5, 5, 5}}, 1.0f);
+    auto filter = make_gaussian(img, 1.0);
+    auto out = filter.generate_region(filter.largest_region());
+    const float s = sum(out.data);
+    CHECK(std::fabs(s - 1.0f) < 0.05f, "3D energy not conserved: sum=%g", s);
+}
+
+// tests/pipeline.rs:78-100
+static void test_vector_pixel_gaussian() {
+    using V = VecPixel<float, 3>;
+    auto img = Image<V, 2>::allocate(Region<2>::make({0, 0}, {11, 11}), {1.0, 1.0}, {0.0, 0.0}, V{});
+    img.set_pixel(Index<2>{{5, 5}}, V{{1.0f, 2.0f, 3.0f}});
+    auto filter = make_gaussian(img, 1.0);
+    auto out = filter.generate_region(filter.largest_region());
+    float s[3] = {0, 0, 0};
+    for (const V& p : out.data)
+        for (int c = 0; c < 3; ++c) s[c] += p.v[c];
+    CHECK(std::fabs(s[0] - 1.0f) < 0.05f, "ch0 sum=%g", s[0]);
+    CHECK(std::fabs(s[1] - 2.0f) < 0.05f, "ch1 sum=%g", s[1]);
+    CHECK(std::fabs(s[2] - 3.0f) < 0.05f, "ch2 sum=%g", s[2]);
+}
+
+// tests/pipeline.rs:106-129 (the values are bit-identical, which the
+// tolerance of the original also accepts)
+static void test_streaming_matches_full() {
+    auto img = solid_image<2>({20, 20}, 0.0f);
+    img.set_pixel(Index<2>{{10, 10}}, 1.0f);
+    auto filter = make_gaussian(img, 1.5);
+    auto full = filter.generate_region(filter.largest_region());
+    auto filter2 = make_gaussian(img, 1.5);
+    auto streamed = stream_full(SourceRef<decltype(filter2)>(filter2), 4);
+    CHECK(full.data.size() == streamed.data.size(), "size");
+    for (size_t i = 0; i < full.data.size(); ++i) {
+        CHECK(std::fabs(full.data[i] - streamed.data[i]) < 1e-6f, "mismatch %g vs %g",
+              full.data[i], streamed.data[i]);
+        CHECK(full.data[i] == streamed.data[i], "not bit-identical at %zu", i);
+    }
+}
+
+// tests/pipeline.rs:135-153
+static void test_unary_filter_pipeline() {
+    auto gaussian = make_gaussian(impulse_image_2d(21), 1.5);
+    auto threshold = make_unary<float>(gaussian, [](float p) { return p > 0.01f ? 1.0f : 0.0f; });
+    auto out = threshold.generate_region(threshold.largest_region());
+    size_t lit = 0;
+    for (float v : out.data) lit += v > 0.5f;
+    CHECK(lit > 0, "no pixels above threshold");
+    CHECK(lit < 21 * 21, "all pixels above threshold");
+}
+
+// A request inside the image (a sub-region) equals the same crop of the
+// full-image output: the halo is pulled from upstream, the clamp only acts at
+// the true image bounds.
+static void test_subregion_matches_full_crop() {
+    auto img = solid_image<3>({19, 17, 23}, 0.0f);
+    uint32_t s = 12345;
+    for (float& v : img.data) {
+        s = s * 1664525u + 1013904223u;
+        v = (float)(s >> 8) / 16777216.0f;
+    }
+    auto filter = make_gaussian(img, 1.3);
+    auto full = filter.generate_region(filter.largest_region());
+    const auto sub = Region<3>::make({3, 2, 5}, {9, 11, 7});
+    auto part = filter.generate_region(sub);
+    Image<float, 3>::for_each_index(sub, [&](const Index<3>& i) {
+        CHECK(part.get_pixel(i) == full.get_pixel(i), "sub-region voxel differs");
+    });
+}
+
+static void test_error_is_thrown() {
+    auto img = solid_image<2>({8, 8}, 0.0f);
+    auto filter = make_gaussian(img, 1.0);
+    bool thrown = false;
+    try {
+        filter.generate_region(Region<2>::make({4, 4}, {8, 8}));  // outside the image
+    } catch (const Error& e) {
+        thrown = e.code == ITKG_ERR_INVALID_ARG;
+    }
+    CHECK(thrown, "request outside the largest region must raise");
+}
+
+int main() {
+#define RUN(t)                  \
+    t();                        \
+    std::printf("ok %s\n", #t); \
+    std::fflush(stdout);
+    RUN(test_image_source_identity_2d);
+    RUN(test_gaussian_impulse_energy_conserved);
+    RUN(test_gaussian_peak_at_centre);
+    RUN(test_gaussian_impulse_2d);
+    RUN(test_gaussian_3d);
+    RUN(test_vector_pixel_gaussian);
+    RUN(test_streaming_matches_full);
+    RUN(test_unary_filter_pipeline);
+    RUN(test_subregion_matches_full_crop);
+    RUN(test_error_is_thrown);
+    std::printf("all passed\n");
+    return 0;
+}

@@ -165,15 +165,24 @@ typedef struct itkg_batch itkg_batch;
 
 typedef enum { ITKG_BATCH_FORWARD = 0, ITKG_BATCH_BACKWARD = 1 } itkg_batch_direction;
 
-/* Plans one launch over all volumes: per-volume radii and weights, a packed
- * tile work list sorted by cost, tile shapes from the autotuner's table.
+/* Plans the launches over all volumes: per-volume radii and weights, packed
+ * tile work lists, tile shapes from the autotuner's cache (or defaults).
  * Allocates the plan's device tables.  The volumes' buffers are bound at
  * creation; create a new plan for new buffers. */
 int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_direction direction,
                       itkg_arith arith, itkg_batch** plan);
 /* Runs the plan on `stream`.  The first run on a stream captures the launch
- * sequence into a HIP graph; later runs on that stream replay it. */
+ * sequence into a HIP graph; later runs on that stream replay it.  Volumes
+ * are grouped by radius; each group is one persistent launch over a packed
+ * (volume, tile) work list. */
 int itkg_batch_run(itkg_batch* plan, void* stream);
+/* Autotuner: times the z-run length of a tile for every radius group of the
+ * plan on `stream` and keeps the fastest (the plan's outputs are written
+ * with valid results while it runs).  With the environment variable
+ * ITKG_TUNE_CACHE naming a file, itkg_batch_create reads earlier choices
+ * from it and this call appends its own (key: direction, arithmetic,
+ * radius).  Captured graphs are dropped and re-captured on the next run. */
+int itkg_batch_autotune(itkg_batch* plan, void* stream);
 /* Voxel count covered by the plan (sum of volume sizes). */
 int64_t itkg_batch_voxels(const itkg_batch* plan);
 int itkg_batch_destroy(itkg_batch* plan);

@@ -35,7 +35,8 @@ BATCH_FORWARD, BATCH_BACKWARD = 0, 1
 EXPORTS = (
     "itkg_kernel_radius", "itkg_build_kernel", "itkg_input_region_for_output",
     "itkg_split_region", "itkg_gaussian_generate_region", "itkg_gaussian_generate_region_host",
-    "itkg_gaussian_backward", "itkg_batch_create", "itkg_batch_run", "itkg_batch_voxels",
+    "itkg_gaussian_backward", "itkg_batch_create", "itkg_batch_run", "itkg_batch_autotune",
+    "itkg_batch_voxels",
     "itkg_batch_destroy", "itkg_last_error", "itkg_set_path", "itkg_version",
 )
 
@@ -87,6 +88,7 @@ def _load():
     L.itkg_batch_create.argtypes = [ctypes.c_int32, P(c_volume), ctypes.c_int, ctypes.c_int,
                                     P(ctypes.c_void_p)]
     L.itkg_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.itkg_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.itkg_batch_voxels.restype = ctypes.c_int64
     L.itkg_batch_voxels.argtypes = [ctypes.c_void_p]
     L.itkg_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -508,6 +510,11 @@ class BatchPlan:
     def run(self, stream: Optional[torch.cuda.Stream] = None) -> None:
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         _check(lib.itkg_batch_run(self.handle, ctypes.c_void_p(s.cuda_stream)))
+
+    def autotune(self, stream: Optional[torch.cuda.Stream] = None) -> None:
+        """Pick the fastest tile z-run per radius group (writes valid outputs)."""
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        _check(lib.itkg_batch_autotune(self.handle, ctypes.c_void_p(s.cuda_stream)))
 
     def close(self) -> None:
         if getattr(self, "handle", None):

@@ -1,0 +1,334 @@
+// Batched many-volume Gaussian (itkg_batch_*): no counterpart in the
+// reference; each volume's result equals GaussianFilter{source: volume,
+// sigma}.generate_region(largest_region) (src/filters/gaussian.rs:59-90), or
+// its adjoint.
+//
+// Plan: volumes whose radii the fused kernel serves are grouped by radius;
+// each group gets a device table of per-volume launch arguments and a packed
+// work list of (volume, tile) items, and runs as ONE persistent launch of the
+// fused kernel over that list.  Other volumes (anisotropic radii, radius
+// beyond the fused set) run the per-axis path with scratch preallocated at
+// plan time.  itkg_batch_run captures the whole sequence into a HIP graph on
+// a stream's first run and replays it afterwards.
+//
+// Autotuner: the z run of a tile (TZ) trades halo recomputation against
+// parallelism; itkg_batch_autotune times TZ candidates per radius group on
+// the plan's own volumes and keeps the fastest, optionally caching the
+// choice in the file named by ITKG_TUNE_CACHE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "itkg_impl.hpp"
+
+namespace itkg {
+
+namespace {
+
+struct BatchClass {
+    int r = 0;
+    int tz = 0;
+    std::vector<int> vol;            // volume indices in this group
+    Fused3dArgs* d_args = nullptr;   // one per volume of the group
+    int2* d_work = nullptr;          // (index into d_args, tile), XCD-interleaved
+    int nwork = 0;
+    int grid = 0;
+};
+
+int default_tz(int r) { return r <= 4 ? 64 : 32; }
+
+using TuneKey = std::tuple<int, int, int>;  // direction, arith, radius
+
+std::map<TuneKey, int> read_cache() {
+    std::map<TuneKey, int> m;
+    const char* path = std::getenv("ITKG_TUNE_CACHE");
+    if (!path) return m;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream ss(line);
+        int d, a, r, tz;
+        if (ss >> d >> a >> r >> tz && tz > 0) m[{d, a, r}] = tz;
+    }
+    return m;
+}
+
+void append_cache(const TuneKey& k, int tz) {
+    const char* path = std::getenv("ITKG_TUNE_CACHE");
+    if (!path) return;
+    std::ofstream f(path, std::ios::app);
+    f << std::get<0>(k) << ' ' << std::get<1>(k) << ' ' << std::get<2>(k) << ' ' << tz << '\n';
+}
+
+}  // namespace
+
+struct BatchImpl {
+    std::vector<itkg_volume> vols;
+    std::vector<Plan> plans;
+    itkg_batch_direction dir = ITKG_BATCH_FORWARD;
+    itkg_arith arith = ITKG_EXACT;
+    int mode = FUSED_FWD_EXACT;
+    std::vector<BatchClass> classes;
+    std::vector<int> rest;           // volumes on the per-axis path
+    void* scratch = nullptr;
+    int64_t voxels = 0;
+    int num_cu = 256;
+    std::mutex mu;
+    std::unordered_map<hipStream_t, hipGraphExec_t> graphs;
+
+    ~BatchImpl() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto& c : classes) {
+            if (c.d_args) (void)hipFree(c.d_args);
+            if (c.d_work) (void)hipFree(c.d_work);
+        }
+        if (scratch) (void)hipFree(scratch);
+    }
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+
+    // (Re)builds the device tables of one group for a z run of `tz`.
+    int build_class(BatchClass& c, int tz) {
+        std::vector<Fused3dArgs> args(c.vol.size());
+        std::vector<int2> items;
+        for (size_t i = 0; i < c.vol.size(); ++i) {
+            const itkg_volume& v = vols[c.vol[i]];
+            const Plan& p = plans[c.vol[i]];
+            const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
+            const itkg_view vout = dense_view(3, v.output, p.req);
+            fill_fused_args(p, vin, vout, mode, 0, tz, &args[i]);
+            const int nt = args[i].tiles_x * args[i].tiles_y * args[i].tiles_z;
+            for (int t = 0; t < nt; ++t) items.push_back(make_int2((int)i, t));
+        }
+        // Workgroup b runs items b, b + G, b + 2G, ... and sits on XCD b mod 8
+        // (G is a multiple of 8): deal the list out in 8 contiguous chunks so
+        // that neighbouring tiles, which share halos, share an XCD's L2.
+        const int n = (int)items.size();
+        const int q = (n + 7) / 8;
+        std::vector<int2> work((size_t)q * 8, make_int2(-1, 0));
+        for (int i = 0; i < n; ++i) work[(size_t)(i % q) * 8 + i / q] = items[i];
+        if (c.d_args) (void)hipFree(c.d_args);
+        if (c.d_work) (void)hipFree(c.d_work);
+        c.d_args = nullptr;
+        c.d_work = nullptr;
+        hipError_t e = hipMalloc(&c.d_args, args.size() * sizeof(Fused3dArgs));
+        if (e == hipSuccess) e = hipMalloc(&c.d_work, work.size() * sizeof(int2));
+        if (e == hipSuccess)
+            e = hipMemcpy(c.d_args, args.data(), args.size() * sizeof(Fused3dArgs),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(c.d_work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "batch tables");
+        c.tz = tz;
+        c.nwork = (int)work.size();
+        const int per_cu = std::max(1, fused3d_batch_occupancy(c.r, mode));
+        c.grid = std::min(c.nwork, ((num_cu * per_cu + 7) / 8) * 8);
+        return ITKG_OK;
+    }
+
+    int enqueue_class(const BatchClass& c, hipStream_t s) const {
+        if (c.nwork == 0) return ITKG_OK;
+        hipError_t e = launch_fused3d_batch(c.d_args, c.d_work, c.nwork, c.grid, c.r, mode, s);
+        if (e != hipSuccess) return hip_fail(e, "batched fused launch");
+        return ITKG_OK;
+    }
+
+    int enqueue(hipStream_t s) {
+        for (const BatchClass& c : classes) {
+            const int rc = enqueue_class(c, s);
+            if (rc) return rc;
+        }
+        for (int i : rest) {
+            const itkg_volume& v = vols[i];
+            const Plan& p = plans[i];
+            const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
+            const itkg_view vout = dense_view(3, v.output, p.req);
+            const int rc = dir == ITKG_BATCH_FORWARD
+                               ? forward_impl(p, ITKG_F32, 1, vin, vout, arith, s, scratch)
+                               : backward_impl(p, ITKG_F32, 1, vin, vout, 0, s, scratch);
+            if (rc) return rc;
+        }
+        return ITKG_OK;
+    }
+};
+
+}  // namespace itkg
+
+struct itkg_batch {
+    itkg::BatchImpl impl;
+};
+
+using namespace itkg;
+
+extern "C" {
+
+int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_direction direction,
+                      itkg_arith arith, itkg_batch** plan) {
+    clear_error();
+    if (n < 0 || (n > 0 && !volumes) || !plan) return fail(ITKG_ERR_INVALID_ARG, "bad arguments");
+    if (direction != ITKG_BATCH_FORWARD && direction != ITKG_BATCH_BACKWARD)
+        return fail(ITKG_ERR_INVALID_ARG, "bad direction");
+    if (arith != ITKG_EXACT && arith != ITKG_FAST) return fail(ITKG_ERR_INVALID_ARG, "bad arith");
+    auto* b = new (std::nothrow) itkg_batch;
+    if (!b) return fail(ITKG_ERR_OOM, "host allocation");
+    BatchImpl& B = b->impl;
+    B.dir = direction;
+    B.arith = arith;
+    B.mode = direction == ITKG_BATCH_BACKWARD ? FUSED_ADJ
+                                              : (arith == ITKG_FAST ? FUSED_FWD_FAST : FUSED_FWD_EXACT);
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            cus > 0)
+            B.num_cu = cus;
+    }
+    size_t sc = 0;
+    std::map<int, size_t> by_r;
+    for (int i = 0; i < n; ++i) {
+        const itkg_volume& v = volumes[i];
+        itkg_region r{};
+        for (int d = 0; d < 3; ++d) r.size[d] = v.size[d];
+        Plan p;
+        int rc = make_plan(3, v.sigma, v.spacing, &r, r, &p);
+        if (!rc && region_len(3, r) > 0 && (!v.input || !v.output))
+            rc = fail(ITKG_ERR_INVALID_ARG, "volume %d has a null buffer", i);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        B.vols.push_back(v);
+        B.plans.push_back(p);
+        B.voxels += region_len(3, r);
+        if (region_len(3, r) == 0) continue;
+        if (use_fused(p, ITKG_F32, 1)) {
+            const int rad = (int)p.radii[0];
+            if (!by_r.count(rad)) {
+                by_r[rad] = B.classes.size();
+                B.classes.emplace_back();
+                B.classes.back().r = rad;
+            }
+            B.classes[by_r[rad]].vol.push_back(i);
+        } else {
+            B.rest.push_back(i);
+            sc = std::max(sc, scratch_bytes(p, ITKG_F32, 1));
+        }
+    }
+    if (sc > 0) {
+        hipError_t e = hipMalloc(&B.scratch, sc);
+        if (e != hipSuccess) {
+            delete b;
+            return hip_fail(e, "hipMalloc(batch scratch)");
+        }
+    }
+    const auto cache = read_cache();
+    for (BatchClass& c : B.classes) {
+        auto it = cache.find({(int)direction, (int)arith, c.r});
+        const int rc = B.build_class(c, it != cache.end() ? it->second : default_tz(c.r));
+        if (rc) {
+            delete b;
+            return rc;
+        }
+    }
+    *plan = b;
+    return ITKG_OK;
+}
+
+int itkg_batch_autotune(itkg_batch* plan, void* stream) {
+    clear_error();
+    if (!plan) return fail(ITKG_ERR_INVALID_ARG, "null plan");
+    BatchImpl& B = plan->impl;
+    std::lock_guard<std::mutex> lock(B.mu);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(ITKG_ERR_HIP, "hipEventCreate");
+    int rc = ITKG_OK;
+    for (BatchClass& c : B.classes) {
+        int maxz = 1;
+        for (int i : c.vol) maxz = std::max<int>(maxz, (int)B.vols[i].size[2]);
+        float best_ms = 1e30f;
+        int best = c.tz;
+        for (int tz : {16, 24, 32, 48, 64, 96, 128, 192, 256}) {
+            if (tz < 2 * c.r || (tz > maxz && tz != 16)) continue;
+            if ((rc = B.build_class(c, tz))) break;
+            if ((rc = B.enqueue_class(c, s))) break;  // warm-up
+            float tot = 0.0f;
+            for (int rep = 0; rep < 3 && !rc; ++rep) {
+                (void)hipEventRecord(e0, s);
+                rc = B.enqueue_class(c, s);
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                tot += ms;
+            }
+            if (rc) break;
+            if (tot < best_ms) {
+                best_ms = tot;
+                best = tz;
+            }
+        }
+        if (rc) break;
+        if ((rc = B.build_class(c, best))) break;
+        append_cache({(int)B.dir, (int)B.arith, c.r}, best);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    B.drop_graphs();
+    return rc;
+}
+
+int itkg_batch_run(itkg_batch* plan, void* stream) {
+    clear_error();
+    if (!plan) return fail(ITKG_ERR_INVALID_ARG, "null plan");
+    BatchImpl& B = plan->impl;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(B.mu);
+    if (!s) return B.enqueue(s);  // the legacy null stream cannot be captured
+    auto it = B.graphs.find(s);
+    if (it == B.graphs.end()) {
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+        const int rc = B.enqueue(s);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        it = B.graphs.emplace(s, ge).first;
+    }
+    hipError_t e = hipGraphLaunch(it->second, s);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return ITKG_OK;
+}
+
+int64_t itkg_batch_voxels(const itkg_batch* plan) { return plan ? plan->impl.voxels : 0; }
+
+int itkg_batch_destroy(itkg_batch* plan) {
+    clear_error();
+    delete plan;
+    return ITKG_OK;
+}
+
+}  // extern "C"

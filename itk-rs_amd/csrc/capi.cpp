@@ -42,8 +42,6 @@ int hip_fail(hipError_t e, const char* what) {
 
 void clear_error() { g_err.clear(); }
 
-namespace {
-
 // ---- region algebra (src/image.rs:17-64) --------------------------------
 
 int64_t region_len(int D, const itkg_region& r) {
@@ -109,14 +107,6 @@ void weights_of(double sigma_px, uint64_t radius, double* k) {
 }
 
 // ---- per-call plan (src/filters/gaussian.rs:59-90) ----------------------
-
-struct Plan {
-    int D = 0;
-    int64_t radii[ITKG_MAX_DIM] = {0, 0, 0, 0};
-    itkg_region bounds{}, req{}, in_reg{};
-    itkg_region reg[ITKG_MAX_DIM + 1]{};  // reg[a]: input of pass a; reg[D] = req
-    double w[ITKG_MAX_DIM][kMaxTaps];
-};
 
 int make_plan(int D, double sigma, const double* spacing, const itkg_region* largest,
               const itkg_region& req, Plan* p) {
@@ -241,18 +231,36 @@ int forward_per_axis(const Plan& p, int ncomp, const itkg_view& in, const itkg_v
     return rc;
 }
 
-int forward_fused3d(const Plan& p, const itkg_view& in, const itkg_view& out, itkg_arith arith,
-                    hipStream_t s) {
-    Fused3dArgs A;
+// z run per workgroup: halve it until the grid holds >= 1024 workgroups (4
+// per CU) or the run would be shorter than twice the ring it fills.
+int fused_tz(int tiles_xy, int nz, int r) {
+    if (const char* e = std::getenv("ITKG_FUSED_TZ")) return std::max(1, std::atoi(e));
+    int tz = std::max(nz, 1);
+    const int minz = std::max(16, 4 * r);
+    while ((int64_t)tiles_xy * ((nz + tz - 1) / tz) < 1024 && tz / 2 >= minz) tz = (tz + 1) / 2;
+    return tz;
+}
+
+// Arguments of one fused launch.  Forward: source = input view at the input
+// region, destination = output (requested).  Adjoint: source = grad_output
+// (requested), destination = grad_input at the input region.  tz <= 0 picks
+// the z run with fused_tz.
+void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
+                     int mode, int accumulate, int tz, Fused3dArgs* out) {
+    const bool adj = mode == FUSED_ADJ;
+    const itkg_region& sreg = adj ? p.req : p.in_reg;
+    const itkg_region& dreg = adj ? p.in_reg : p.req;
+    Fused3dArgs& A = *out;
     std::memset(&A, 0, sizeof A);
-    A.src = static_cast<const float*>(in.data) + view_offset(3, in, p.in_reg.index);
-    A.dst = static_cast<float*>(out.data);
+    A.src = static_cast<const float*>(src_view.data) + view_offset(3, src_view, sreg.index);
+    A.dst = static_cast<float*>(dst_view.data) + view_offset(3, dst_view, dreg.index);
     for (int d = 0; d < 3; ++d) {
-        A.src_stride[d] = in.stride[d];
-        A.dst_stride[d] = out.stride[d];
-        A.n_in[d] = (int32_t)p.in_reg.size[d];
-        A.n_out[d] = (int32_t)p.req.size[d];
-        A.off[d] = (int32_t)(p.req.index[d] - p.in_reg.index[d]);
+        A.src_stride[d] = src_view.stride[d];
+        A.dst_stride[d] = dst_view.stride[d];
+        A.n_src[d] = (int32_t)sreg.size[d];
+        A.n_dst[d] = (int32_t)dreg.size[d];
+        const int32_t off = (int32_t)(p.req.index[d] - p.in_reg.index[d]);
+        A.shift[d] = adj ? -off : off;
     }
     const int r = (int)p.radii[0];
     for (int k = 0; k <= 2 * r; ++k) {
@@ -260,21 +268,27 @@ int forward_fused3d(const Plan& p, const itkg_view& in, const itkg_view& out, it
         A.wy[k] = (float)p.w[1][k];
         A.wz[k] = (float)p.w[2][k];
     }
-    const int TX = fused3d_tile_x(), TY = fused3d_tile_y(r);
-    A.tiles_x = (A.n_out[0] + TX - 1) / TX;
-    A.tiles_y = (A.n_out[1] + TY - 1) / TY;
-    // z run per workgroup: halve it until the grid holds >= 1024 workgroups
-    // (4 per CU) or the run would be shorter than the ring it fills.
-    int tz = A.n_out[2];
-    if (const char* e = std::getenv("ITKG_FUSED_TZ")) tz = std::max(1, std::atoi(e));
-    else {
-        const int minz = std::max(16, 4 * r);
-        while ((int64_t)A.tiles_x * A.tiles_y * ((A.n_out[2] + tz - 1) / tz) < 1024 && tz / 2 >= minz)
-            tz = (tz + 1) / 2;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m <= r; ++m) {
+        A.px[m] = (float)acc[0];
+        A.py[m] = (float)acc[1];
+        A.pz[m] = (float)acc[2];
+        for (int d = 0; d < 3; ++d) acc[d] += p.w[d][m];
     }
-    A.tz = tz;
-    A.tiles_z = (A.n_out[2] + tz - 1) / tz;
-    hipError_t e = launch_fused3d(A, r, r, r, arith, s);
+    A.accumulate = accumulate;
+    const int TX = fused3d_tile_x(), TY = fused3d_tile_y(r);
+    A.tiles_x = (A.n_dst[0] + TX - 1) / TX;
+    A.tiles_y = (A.n_dst[1] + TY - 1) / TY;
+    A.tz = tz > 0 ? tz : fused_tz(A.tiles_x * A.tiles_y, A.n_dst[2], r);
+    A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
+}
+
+int run_fused3d(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view, int mode,
+                int accumulate, hipStream_t s) {
+    Fused3dArgs A;
+    fill_fused_args(p, src_view, dst_view, mode, accumulate, 0, &A);
+    const int r = (int)p.radii[0];
+    hipError_t e = launch_fused3d(A, r, r, r, mode, s);
     if (e != hipSuccess) return hip_fail(e, "fused 3-D launch");
     return ITKG_OK;
 }
@@ -295,7 +309,8 @@ bool use_fused(const Plan& p, itkg_dtype dt, int ncomp) {
 int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
                  const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch) {
     if (region_len(p.D, p.req) == 0) return ITKG_OK;
-    if (use_fused(p, dt, ncomp)) return forward_fused3d(p, in, out, arith, s);
+    if (use_fused(p, dt, ncomp))
+        return run_fused3d(p, in, out, arith == ITKG_FAST ? FUSED_FWD_FAST : FUSED_FWD_EXACT, 0, s);
     if (g_path.load() == ITKG_PATH_FUSED)
         return fail(ITKG_ERR_UNSUPPORTED, "fused path requested but not available for this call");
     return dt == ITKG_F64 ? forward_per_axis<double>(p, ncomp, in, out, arith, s, scratch)
@@ -339,34 +354,22 @@ int backward_per_axis(const Plan& p, int ncomp, const itkg_view& gout, const itk
     return rc;
 }
 
+int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout,
+                  const itkg_view& gin, int accumulate, hipStream_t s, void* scratch) {
+    if (region_len(p.D, p.req) == 0) return ITKG_OK;
+    if (use_fused(p, dt, ncomp)) return run_fused3d(p, gout, gin, FUSED_ADJ, accumulate, s);
+    if (g_path.load() == ITKG_PATH_FUSED)
+        return fail(ITKG_ERR_UNSUPPORTED, "fused path requested but not available for this call");
+    return dt == ITKG_F64 ? backward_per_axis<double>(p, ncomp, gout, gin, accumulate, s, scratch)
+                          : backward_per_axis<float>(p, ncomp, gout, gin, accumulate, s, scratch);
+}
+
 bool args_ok(int ndim, itkg_dtype dt, int ncomp) {
     return ndim >= 1 && ndim <= ITKG_MAX_DIM && (dt == ITKG_F32 || dt == ITKG_F64) && ncomp >= 1 &&
            ncomp <= 4;
 }
 
-}  // namespace
-
-// ---- batched plan -------------------------------------------------------
-
-struct BatchImpl {
-    std::vector<itkg_volume> vols;
-    std::vector<Plan> plans;
-    itkg_batch_direction dir;
-    itkg_arith arith;
-    void* scratch = nullptr;
-    size_t scratch_sz = 0;
-    int64_t voxels = 0;
-    std::mutex mu;
-    std::unordered_map<hipStream_t, hipGraphExec_t> graphs;
-};
-
-int batch_enqueue(BatchImpl& b, hipStream_t s);
-
 }  // namespace itkg
-
-struct itkg_batch {
-    itkg::BatchImpl impl;
-};
 
 using namespace itkg;
 
@@ -504,96 +507,8 @@ int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double
     if (rc) return rc;
     if ((rc = check_view(ndim, grad_output, p.req, "grad_output"))) return rc;
     if ((rc = check_view(ndim, grad_input, p.in_reg, "grad_input"))) return rc;
-    if (region_len(ndim, p.req) == 0) return ITKG_OK;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    return dtype == ITKG_F64
-               ? backward_per_axis<double>(p, ncomp, *grad_output, *grad_input, accumulate, s, nullptr)
-               : backward_per_axis<float>(p, ncomp, *grad_output, *grad_input, accumulate, s, nullptr);
-}
-
-// ---- batched ------------------------------------------------------------
-
-int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_direction direction,
-                      itkg_arith arith, itkg_batch** plan) {
-    clear_error();
-    if (n < 0 || (n > 0 && !volumes) || !plan) return fail(ITKG_ERR_INVALID_ARG, "bad arguments");
-    if (direction != ITKG_BATCH_FORWARD && direction != ITKG_BATCH_BACKWARD)
-        return fail(ITKG_ERR_INVALID_ARG, "bad direction");
-    auto* b = new (std::nothrow) itkg_batch;
-    if (!b) return fail(ITKG_ERR_OOM, "host allocation");
-    BatchImpl& B = b->impl;
-    B.dir = direction;
-    B.arith = arith;
-    size_t sc = 0;
-    for (int i = 0; i < n; ++i) {
-        const itkg_volume& v = volumes[i];
-        itkg_region r{};
-        for (int d = 0; d < 3; ++d) r.size[d] = v.size[d];
-        Plan p;
-        int rc = make_plan(3, v.sigma, v.spacing, &r, r, &p);
-        if (!rc && region_len(3, r) > 0 && (!v.input || !v.output))
-            rc = fail(ITKG_ERR_INVALID_ARG, "volume %d has a null buffer", i);
-        if (rc) {
-            delete b;
-            return rc;
-        }
-        B.vols.push_back(v);
-        B.plans.push_back(p);
-        B.voxels += region_len(3, r);
-        const bool fused = direction == ITKG_BATCH_FORWARD && use_fused(p, ITKG_F32, 1);
-        if (!fused) sc = std::max(sc, scratch_bytes(p, ITKG_F32, 1));
-    }
-    if (sc > 0) {
-        hipError_t e = hipMalloc(&B.scratch, sc);
-        if (e != hipSuccess) {
-            delete b;
-            return hip_fail(e, "hipMalloc(batch scratch)");
-        }
-        B.scratch_sz = sc;
-    }
-    *plan = b;
-    return ITKG_OK;
-}
-
-int itkg_batch_run(itkg_batch* plan, void* stream) {
-    clear_error();
-    if (!plan) return fail(ITKG_ERR_INVALID_ARG, "null plan");
-    BatchImpl& B = plan->impl;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (!s) return batch_enqueue(B, s);  // the null stream cannot be captured
-    std::lock_guard<std::mutex> lock(B.mu);
-    auto it = B.graphs.find(s);
-    if (it == B.graphs.end()) {
-        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-        const int rc = batch_enqueue(B, s);
-        hipGraph_t g = nullptr;
-        e = hipStreamEndCapture(s, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-        hipGraphExec_t ge = nullptr;
-        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-        it = B.graphs.emplace(s, ge).first;
-    }
-    hipError_t e = hipGraphLaunch(it->second, s);
-    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
-    return ITKG_OK;
-}
-
-int64_t itkg_batch_voxels(const itkg_batch* plan) { return plan ? plan->impl.voxels : 0; }
-
-int itkg_batch_destroy(itkg_batch* plan) {
-    clear_error();
-    if (!plan) return ITKG_OK;
-    for (auto& kv : plan->impl.graphs) (void)hipGraphExecDestroy(kv.second);
-    if (plan->impl.scratch) (void)hipFree(plan->impl.scratch);
-    delete plan;
-    return ITKG_OK;
+    return backward_impl(p, dtype, ncomp, *grad_output, *grad_input, accumulate,
+                         static_cast<hipStream_t>(stream), nullptr);
 }
 
 const char* itkg_last_error(void) { return g_err.c_str(); }
@@ -610,25 +525,3 @@ const char* itkg_version(void) { return "itkg 0.1 (gfx950)"; }
 
 }  // extern "C"
 
-namespace itkg {
-
-// Loop-of-volumes enqueue used by the batched plan (captured into a graph by
-// itkg_batch_run).
-int batch_enqueue(BatchImpl& B, hipStream_t s) {
-    for (size_t i = 0; i < B.vols.size(); ++i) {
-        const itkg_volume& v = B.vols[i];
-        const Plan& p = B.plans[i];
-        if (region_len(3, p.req) == 0) continue;
-        const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
-        const itkg_view vout = dense_view(3, v.output, p.req);
-        int rc;
-        if (B.dir == ITKG_BATCH_FORWARD)
-            rc = forward_impl(p, ITKG_F32, 1, vin, vout, B.arith, s, B.scratch);
-        else
-            rc = backward_per_axis<float>(p, 1, vin, vout, 0, s, B.scratch);
-        if (rc) return rc;
-    }
-    return ITKG_OK;
-}
-
-}  // namespace itkg

@@ -47,26 +47,66 @@ template <typename T>
 hipError_t launch_axis_adjoint_range(const AxisPassArgs<T>& a, int ncomp, int64_t g_lo,
                                      int64_t g_hi, hipStream_t s);
 
-// Fused one-round-trip 3-D forward for f32 scalar images.
+// Fused one-round-trip 3-D kernel for f32 scalar pixels (forward or adjoint).
+// Destination pixel j reads source pixels j + shift + k - r, k = 0..2r
+// (forward: clamped into the source; adjoint: zero outside it).
+enum FusedMode { FUSED_FWD_EXACT = 0, FUSED_FWD_FAST = 1, FUSED_ADJ = 2 };
+constexpr int kFusedMaxR = 16;
+
 struct Fused3dArgs {
-    const float* src;        // input view pixel at the pass-0 input region index
-    float* dst;              // output view pixel at the requested region index
+    const float* src;        // source view pixel at the source region index
+    float* dst;              // destination view pixel at the destination region index
     int64_t src_stride[3];
     int64_t dst_stride[3];
-    int32_t n_in[3];         // input region extents (the clamp bounds)
-    int32_t n_out[3];        // requested region extents
-    int32_t off[3];          // requested.index - input.index (0..r)
-    int32_t tz;              // output planes per block
+    int32_t n_src[3];        // source extents (forward: the clamp bounds)
+    int32_t n_dst[3];        // destination extents
+    int32_t shift[3];        // forward: requested.index - input.index; adjoint: its negation
+    int32_t tz;              // destination planes per workgroup
     int32_t tiles_x, tiles_y, tiles_z;
-    float wx[2 * 32 + 1], wy[2 * 32 + 1], wz[2 * 32 + 1];
+    int32_t accumulate;      // adjoint: dst += result
+    float wx[2 * kFusedMaxR + 1], wy[2 * kFusedMaxR + 1], wz[2 * kFusedMaxR + 1];
+    float px[kFusedMaxR + 1], py[kFusedMaxR + 1], pz[kFusedMaxR + 1];  // adjoint edge sums
 };
 
 // True when the fused kernel has an instantiation for these radii.
 bool fused3d_supported(int rx, int ry, int rz);
-// Output tile of one fused workgroup in x and y.
+// Destination tile of one fused workgroup in x and y.
 int fused3d_tile_x();
 int fused3d_tile_y(int rz);
-hipError_t launch_fused3d(const Fused3dArgs& a, int rx, int ry, int rz, itkg_arith arith,
+hipError_t launch_fused3d(const Fused3dArgs& a, int rx, int ry, int rz, int mode,
                           hipStream_t s);
+// Persistent launch over `work` = (index into vols, tile) items, all of
+// isotropic radius r; `vols` and `work` are device arrays.
+hipError_t launch_fused3d_batch(const Fused3dArgs* vols, const int2* work, int nwork,
+                                int grid_blocks, int r, int mode, hipStream_t s);
+// Resident workgroups per CU of the persistent kernel for radius r.
+int fused3d_batch_occupancy(int r, int mode);
+
+// Per-call plan of GaussianFilter::generate_region (src/filters/gaussian.rs:
+// 59-90): radii, f64 weights and the region of every pass.
+struct Plan {
+    int D = 0;
+    int64_t radii[ITKG_MAX_DIM] = {0, 0, 0, 0};
+    itkg_region bounds{}, req{}, in_reg{};
+    itkg_region reg[ITKG_MAX_DIM + 1]{};  // reg[a]: input of pass a; reg[D] = req
+    double w[ITKG_MAX_DIM][kMaxTaps];
+};
+
+int make_plan(int D, double sigma, const double* spacing, const itkg_region* largest,
+              const itkg_region& req, Plan* p);
+int64_t region_len(int D, const itkg_region& r);
+itkg_view dense_view(int D, void* data, const itkg_region& r);
+// Device scratch the per-axis path needs (two ping-pong intermediates).
+size_t scratch_bytes(const Plan& p, itkg_dtype dt, int ncomp);
+// True when the fused kernel serves this plan (and the path setting allows it).
+bool use_fused(const Plan& p, itkg_dtype dt, int ncomp);
+int fused_tz(int tiles_xy, int nz, int r);
+void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
+                     int mode, int accumulate, int tz, Fused3dArgs* out);
+// Enqueue one forward / adjoint; `scratch` (nullable) holds scratch_bytes().
+int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
+                 const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch);
+int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout,
+                  const itkg_view& gin, int accumulate, hipStream_t s, void* scratch);
 
 }  // namespace itkg

@@ -44,3 +44,35 @@ def test_batch_validates(itk, dev):
     x = torch.zeros((4, 4, 4), device=dev)
     with pytest.raises(itk.ItkgError):
         itk.BatchPlan([x], [x.clone()], [200.0], [(1.0, 1.0, 1.0)])
+
+
+def test_batch_autotune_keeps_results(itk, dev, tmp_path, monkeypatch):
+    from itk_rs_amd import workloads as wl
+    monkeypatch.setenv("ITKG_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    vols, params = wl.c4(dev, nvol=6, lo=20, hi=90, seed=8)
+    outs = [torch.empty_like(v) for v in vols]
+    plan = itk.BatchPlan(vols, outs, [p[1] for p in params], [p[2] for p in params])
+    s = torch.cuda.Stream()
+    plan.run(s)
+    s.synchronize()
+    first = [o.clone() for o in outs]
+    plan.autotune(s)
+    s.synchronize()
+    for o in outs:
+        o.zero_()
+    torch.cuda.synchronize()
+    plan.run(s)
+    s.synchronize()
+    for a, b in zip(first, outs):
+        assert torch.equal(a, b)
+    lines = (tmp_path / "tune.txt").read_text().split("\n")
+    assert any(len(ln.split()) == 4 for ln in lines)
+    # a new plan picks the cached choice up and gives the same results
+    outs2 = [torch.empty_like(v) for v in vols]
+    plan2 = itk.BatchPlan(vols, outs2, [p[1] for p in params], [p[2] for p in params])
+    plan2.run(s)
+    s.synchronize()
+    for a, b in zip(first, outs2):
+        assert torch.equal(a, b)
+    plan.close()
+    plan2.close()

@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprof-derived HBM bytes per launch, if measured")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 512 MiB device-to-device copy (a counter reference "
+                         "of known size for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -98,6 +101,9 @@ def main():
     f = itk.GaussianFilter(img, sigma)
     f.arith = arith
     step = lambda: f.generate_into(img, oimg)  # noqa: E731
+    if args.copy_calibration:
+        out.copy_(x)  # 512 MiB read + 512 MiB written by one copy kernel
+        torch.cuda.synchronize()
     wall, ms = timed_loop(step, args.steps, args.warmup, stream)
     ms_avg = float(np.mean(ms))
     gvox = n * args.steps / wall / 1e9

@@ -38,6 +38,7 @@ namespace {
 
 struct BatchClass {
     int r = 0;
+    const FusedEntry* entry = nullptr;  // its persistent launcher
     int tz = 0;
     std::vector<int> vol;            // volume indices in this group
     Fused3dArgs* d_args = nullptr;   // one per volume of the group
@@ -110,7 +111,7 @@ struct BatchImpl {
             const Plan& p = plans[c.vol[i]];
             const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
             const itkg_view vout = dense_view(3, v.output, p.req);
-            fill_fused_args(p, vin, vout, mode, 0, tz, &args[i]);
+            fill_fused_args(p, vin, vout, mode, 0, tz, 0, 1, &args[i]);
             const int nt = args[i].tiles_x * args[i].tiles_y * args[i].tiles_z;
             for (int t = 0; t < nt; ++t) items.push_back(make_int2((int)i, t));
         }
@@ -135,14 +136,14 @@ struct BatchImpl {
         if (e != hipSuccess) return hip_fail(e, "batch tables");
         c.tz = tz;
         c.nwork = (int)work.size();
-        const int per_cu = std::max(1, fused3d_batch_occupancy(c.r, mode));
+        const int per_cu = std::max(1, c.entry->occupancy(mode));
         c.grid = std::min(c.nwork, ((num_cu * per_cu + 7) / 8) * 8);
         return ITKG_OK;
     }
 
     int enqueue_class(const BatchClass& c, hipStream_t s) const {
         if (c.nwork == 0) return ITKG_OK;
-        hipError_t e = launch_fused3d_batch(c.d_args, c.d_work, c.nwork, c.grid, c.r, mode, s);
+        hipError_t e = c.entry->batch(c.d_args, c.d_work, c.nwork, c.grid, mode, s);
         if (e != hipSuccess) return hip_fail(e, "batched fused launch");
         return ITKG_OK;
     }
@@ -215,12 +216,14 @@ int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_directio
         B.plans.push_back(p);
         B.voxels += region_len(3, r);
         if (region_len(3, r) == 0) continue;
-        if (use_fused(p, ITKG_F32, 1)) {
+        const FusedEntry* fe = use_fused(p, ITKG_F32, 1) ? fused_for(p) : nullptr;
+        if (fe && fe->batch && p.D == 3 && fe->rz == fe->rxy) {
             const int rad = (int)p.radii[0];
             if (!by_r.count(rad)) {
                 by_r[rad] = B.classes.size();
                 B.classes.emplace_back();
                 B.classes.back().r = rad;
+                B.classes.back().entry = fe;
             }
             B.classes[by_r[rad]].vol.push_back(i);
         } else {

@@ -241,55 +241,79 @@ int fused_tz(int tiles_xy, int nz, int r) {
     return tz;
 }
 
-// Arguments of one fused launch.  Forward: source = input view at the input
-// region, destination = output (requested).  Adjoint: source = grad_output
-// (requested), destination = grad_input at the input region.  tz <= 0 picks
-// the z run with fused_tz.
+// The fused instantiation serving this plan (2-D images run as one plane
+// with z radius 0), or null.
+const FusedEntry* fused_for(const Plan& p) {
+    if (p.D != 2 && p.D != 3) return nullptr;
+    if (p.radii[0] != p.radii[1]) return nullptr;
+    return fused_entry((int)p.radii[0], p.D == 3 ? (int)p.radii[2] : 0);
+}
+
+// Arguments of one fused launch for component `comp` of `ncomp` interleaved
+// components (a component is a scalar image with strides scaled by ncomp).
+// Forward: source = input view at the input region, destination = output
+// (requested).  Adjoint: source = grad_output (requested), destination =
+// grad_input at the input region.  tz <= 0 picks the z run with fused_tz.
 void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
-                     int mode, int accumulate, int tz, Fused3dArgs* out) {
+                     int mode, int accumulate, int tz, int comp, int ncomp, Fused3dArgs* out) {
     const bool adj = mode == FUSED_ADJ;
     const itkg_region& sreg = adj ? p.req : p.in_reg;
     const itkg_region& dreg = adj ? p.in_reg : p.req;
+    const int D = p.D;
     Fused3dArgs& A = *out;
     std::memset(&A, 0, sizeof A);
-    A.src = static_cast<const float*>(src_view.data) + view_offset(3, src_view, sreg.index);
-    A.dst = static_cast<float*>(dst_view.data) + view_offset(3, dst_view, dreg.index);
+    A.src = static_cast<const float*>(src_view.data) + view_offset(D, src_view, sreg.index) * ncomp +
+            comp;
+    A.dst = static_cast<float*>(dst_view.data) + view_offset(D, dst_view, dreg.index) * ncomp + comp;
     for (int d = 0; d < 3; ++d) {
-        A.src_stride[d] = src_view.stride[d];
-        A.dst_stride[d] = dst_view.stride[d];
-        A.n_src[d] = (int32_t)sreg.size[d];
-        A.n_dst[d] = (int32_t)dreg.size[d];
-        const int32_t off = (int32_t)(p.req.index[d] - p.in_reg.index[d]);
-        A.shift[d] = adj ? -off : off;
+        if (d < D) {
+            A.src_stride[d] = src_view.stride[d] * ncomp;
+            A.dst_stride[d] = dst_view.stride[d] * ncomp;
+            A.n_src[d] = (int32_t)sreg.size[d];
+            A.n_dst[d] = (int32_t)dreg.size[d];
+            const int32_t off = (int32_t)(p.req.index[d] - p.in_reg.index[d]);
+            A.shift[d] = adj ? -off : off;
+        } else {  // a 2-D image is one plane
+            A.n_src[d] = A.n_dst[d] = 1;
+        }
     }
-    const int r = (int)p.radii[0];
-    for (int k = 0; k <= 2 * r; ++k) {
+    const int rxy = (int)p.radii[0];
+    const int rz = D == 3 ? (int)p.radii[2] : 0;
+    for (int k = 0; k <= 2 * rxy; ++k) {
         A.wx[k] = (float)p.w[0][k];
         A.wy[k] = (float)p.w[1][k];
-        A.wz[k] = (float)p.w[2][k];
     }
+    for (int k = 0; k <= 2 * rz; ++k) A.wz[k] = D == 3 ? (float)p.w[2][k] : 1.0f;
+    for (int k = 0; k < 2 * (2 * rz + 1); ++k) A.wz2[k] = A.wz[k % (2 * rz + 1)];
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int m = 0; m <= r; ++m) {
+    for (int m = 0; m <= rxy; ++m) {
         A.px[m] = (float)acc[0];
         A.py[m] = (float)acc[1];
+        acc[0] += p.w[0][m];
+        acc[1] += p.w[1][m];
+    }
+    for (int m = 0; m <= rz && D == 3; ++m) {
         A.pz[m] = (float)acc[2];
-        for (int d = 0; d < 3; ++d) acc[d] += p.w[d][m];
+        acc[2] += p.w[2][m];
     }
     A.accumulate = accumulate;
-    const int TX = fused3d_tile_x(), TY = fused3d_tile_y(r);
-    A.tiles_x = (A.n_dst[0] + TX - 1) / TX;
+    const FusedEntry* fe = fused_for(p);
+    const int TY = fe ? fe->ty : 32;
+    A.tiles_x = (A.n_dst[0] + kFusedTileX - 1) / kFusedTileX;
     A.tiles_y = (A.n_dst[1] + TY - 1) / TY;
-    A.tz = tz > 0 ? tz : fused_tz(A.tiles_x * A.tiles_y, A.n_dst[2], r);
+    A.tz = tz > 0 ? tz : fused_tz(A.tiles_x * A.tiles_y, A.n_dst[2], rz);
     A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
 }
 
-int run_fused3d(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view, int mode,
-                int accumulate, hipStream_t s) {
-    Fused3dArgs A;
-    fill_fused_args(p, src_view, dst_view, mode, accumulate, 0, &A);
-    const int r = (int)p.radii[0];
-    hipError_t e = launch_fused3d(A, r, r, r, mode, s);
-    if (e != hipSuccess) return hip_fail(e, "fused 3-D launch");
+int run_fused(const Plan& p, int ncomp, const itkg_view& src_view, const itkg_view& dst_view,
+              int mode, int accumulate, hipStream_t s) {
+    const FusedEntry* fe = fused_for(p);
+    for (int c = 0; c < ncomp; ++c) {
+        Fused3dArgs A;
+        fill_fused_args(p, src_view, dst_view, mode, accumulate, 0, c, ncomp, &A);
+        hipError_t e = fe->launch(A, mode, s);
+        if (e != hipSuccess) return hip_fail(e, "fused launch");
+    }
     return ITKG_OK;
 }
 
@@ -302,15 +326,15 @@ bool fits_i32(const Plan& p) {
 bool use_fused(const Plan& p, itkg_dtype dt, int ncomp) {
     const int path = g_path.load();
     if (path == ITKG_PATH_PER_AXIS) return false;
-    return p.D == 3 && dt == ITKG_F32 && ncomp == 1 && fits_i32(p) &&
-           fused3d_supported((int)p.radii[0], (int)p.radii[1], (int)p.radii[2]);
+    return dt == ITKG_F32 && ncomp >= 1 && ncomp <= 4 && fits_i32(p) && fused_for(p) != nullptr;
 }
 
 int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
                  const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch) {
     if (region_len(p.D, p.req) == 0) return ITKG_OK;
     if (use_fused(p, dt, ncomp))
-        return run_fused3d(p, in, out, arith == ITKG_FAST ? FUSED_FWD_FAST : FUSED_FWD_EXACT, 0, s);
+        return run_fused(p, ncomp, in, out, arith == ITKG_FAST ? FUSED_FWD_FAST : FUSED_FWD_EXACT, 0,
+                         s);
     if (g_path.load() == ITKG_PATH_FUSED)
         return fail(ITKG_ERR_UNSUPPORTED, "fused path requested but not available for this call");
     return dt == ITKG_F64 ? forward_per_axis<double>(p, ncomp, in, out, arith, s, scratch)
@@ -357,7 +381,7 @@ int backward_per_axis(const Plan& p, int ncomp, const itkg_view& gout, const itk
 int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout,
                   const itkg_view& gin, int accumulate, hipStream_t s, void* scratch) {
     if (region_len(p.D, p.req) == 0) return ITKG_OK;
-    if (use_fused(p, dt, ncomp)) return run_fused3d(p, gout, gin, FUSED_ADJ, accumulate, s);
+    if (use_fused(p, dt, ncomp)) return run_fused(p, ncomp, gout, gin, FUSED_ADJ, accumulate, s);
     if (g_path.load() == ITKG_PATH_FUSED)
         return fail(ITKG_ERR_UNSUPPORTED, "fused path requested but not available for this call");
     return dt == ITKG_F64 ? backward_per_axis<double>(p, ncomp, gout, gin, accumulate, s, scratch)

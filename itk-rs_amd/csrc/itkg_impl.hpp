@@ -65,22 +65,23 @@ struct Fused3dArgs {
     int32_t tiles_x, tiles_y, tiles_z;
     int32_t accumulate;      // adjoint: dst += result
     float wx[2 * kFusedMaxR + 1], wy[2 * kFusedMaxR + 1], wz[2 * kFusedMaxR + 1];
+    float wz2[2 * (2 * kFusedMaxR + 1)];  // wz repeated twice: rotated-weight lookups
     float px[kFusedMaxR + 1], py[kFusedMaxR + 1], pz[kFusedMaxR + 1];  // adjoint edge sums
 };
 
-// True when the fused kernel has an instantiation for these radii.
-bool fused3d_supported(int rx, int ry, int rz);
-// Destination tile of one fused workgroup in x and y.
-int fused3d_tile_x();
-int fused3d_tile_y(int rz);
-hipError_t launch_fused3d(const Fused3dArgs& a, int rx, int ry, int rz, int mode,
-                          hipStream_t s);
-// Persistent launch over `work` = (index into vols, tile) items, all of
-// isotropic radius r; `vols` and `work` are device arrays.
-hipError_t launch_fused3d_batch(const Fused3dArgs* vols, const int2* work, int nwork,
-                                int grid_blocks, int r, int mode, hipStream_t s);
-// Resident workgroups per CU of the persistent kernel for radius r.
-int fused3d_batch_occupancy(int r, int mode);
+// One compiled instantiation of the fused kernel: x/y radius rxy, z radius
+// rz (0 for 2-D images), its y tile height, and its launchers.  `batch` and
+// `occupancy` are null for instantiations without a persistent form.
+struct FusedEntry {
+    int rxy, rz, ty;
+    hipError_t (*launch)(const Fused3dArgs& a, int mode, hipStream_t s);
+    hipError_t (*batch)(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
+                        int mode, hipStream_t s);
+    int (*occupancy)(int mode);  // resident workgroups per CU of the persistent form
+};
+constexpr int kFusedTileX = 64;
+// The instantiation for (rxy, rz), or null.
+const FusedEntry* fused_entry(int rxy, int rz);
 
 // Per-call plan of GaussianFilter::generate_region (src/filters/gaussian.rs:
 // 59-90): radii, f64 weights and the region of every pass.
@@ -100,9 +101,10 @@ itkg_view dense_view(int D, void* data, const itkg_region& r);
 size_t scratch_bytes(const Plan& p, itkg_dtype dt, int ncomp);
 // True when the fused kernel serves this plan (and the path setting allows it).
 bool use_fused(const Plan& p, itkg_dtype dt, int ncomp);
+const FusedEntry* fused_for(const Plan& p);
 int fused_tz(int tiles_xy, int nz, int r);
 void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
-                     int mode, int accumulate, int tz, Fused3dArgs* out);
+                     int mode, int accumulate, int tz, int comp, int ncomp, Fused3dArgs* out);
 // Enqueue one forward / adjoint; `scratch` (nullable) holds scratch_bytes().
 int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
                  const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch);

@@ -12,7 +12,8 @@ CASES = [
     ((17,), [5], [1.0], 1.5, [7], [6]),
     ((7, 8, 9), [0, 0, 0], [1.0, 1.0, 1.0], 2.0, [0, 0, 0], [9, 8, 7]),
     ((3, 2, 4), [0, 0, 0], [1.0, 1.0, 1.0], 2.5, [0, 0, 0], [4, 2, 3]),
-    ((21, 30, 26), [-4, 2, 1], [1.0, 0.8, 1.4], 2.0, [3, 9, 7], [20, 15, 12]),
+    ((21, 30, 26), [-4, 2, 1], [1.0, 0.8, 1.4], 2.0, [3, 9, 7], [18, 15, 12]),
+    ((21, 30, 26), [-4, 2, 1], [1.0, 1.0, 1.0], 1.5, [3, 9, 7], [18, 15, 12]),
     ((4, 3, 5, 6), [0, 0, 0, 0], [1.0, 1.0, 1.0, 2.0], 1.0, [1, 0, 1, 0], [4, 3, 2, 3]),
 ]
 

@@ -216,7 +216,11 @@ int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_directio
         B.plans.push_back(p);
         B.voxels += region_len(3, r);
         if (region_len(3, r) == 0) continue;
-        const FusedEntry* fe = use_fused(p, ITKG_F32, 1) ? fused_for(p) : nullptr;
+        const itkg_view vsrc = dense_view(3, const_cast<float*>(v.input), r);
+        const FusedEntry* fe =
+            use_fused(p, ITKG_F32, 1) && fused_src_ok(p, vsrc, 1, direction == ITKG_BATCH_BACKWARD)
+                ? fused_for(p)
+                : nullptr;
         if (fe && fe->batch && p.D == 3 && fe->rz == fe->rxy) {
             const int rad = (int)p.radii[0];
             if (!by_r.count(rad)) {

@@ -231,14 +231,48 @@ int forward_per_axis(const Plan& p, int ncomp, const itkg_view& in, const itkg_v
     return rc;
 }
 
-// z run per workgroup: halve it until the grid holds >= 1024 workgroups (4
-// per CU) or the run would be shorter than twice the ring it fills.
-int fused_tz(int tiles_xy, int nz, int r) {
+// Compute units of the current device (cached per device).
+int device_cus() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int v = cache[dev].load();
+    if (v > 0) return v;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    cache[dev].store(cus);
+    return cus;
+}
+
+// z run of a fused tile.  A workgroup walks tz + 2r source planes, plus a
+// short prologue; the grid runs in ceil(tiles / slots) rounds where slots =
+// CUs x resident workgroups per CU.  The z runs are the equal splits
+// ceil(nz / k); the one with the fewest steps on the critical path wins (the
+// smaller tz on ties: shorter tails).  ITKG_FUSED_TZ overrides.
+int fused_tz(const FusedEntry* fe, int mode, int tiles_xy, int nz, int r) {
     if (const char* e = std::getenv("ITKG_FUSED_TZ")) return std::max(1, std::atoi(e));
-    int tz = std::max(nz, 1);
-    const int minz = std::max(16, 4 * r);
-    while ((int64_t)tiles_xy * ((nz + tz - 1) / tz) < 1024 && tz / 2 >= minz) tz = (tz + 1) / 2;
-    return tz;
+    nz = std::max(nz, 1);
+    const int occ = fe && fe->plain_occupancy ? fe->plain_occupancy(mode) : 1;
+    const int64_t slots = (int64_t)device_cus() * std::max(occ, 1);
+    int best_tz = nz;
+    int64_t best = INT64_MAX;
+    for (int k = 1; k <= std::min(nz, 1024); ++k) {
+        const int tz = (nz + k - 1) / k;
+        if (k > 1 && tz == (nz + k - 2) / (k - 1)) continue;  // same split as k - 1
+        const int64_t tiles = (int64_t)tiles_xy * ((nz + tz - 1) / tz);
+        const int64_t rounds = (tiles + slots - 1) / slots;
+        const int64_t cost = rounds * (tz + 2 * r + 3);
+        if (cost <= best) {
+            best = cost;
+            best_tz = tz;
+        }
+    }
+    if (std::getenv("ITKG_DEBUG"))
+        std::fprintf(stderr, "itkg: fused r=%d mode=%d occupancy=%d slots=%lld tiles_xy=%d nz=%d -> tz=%d\n",
+                     r, mode, occ, (long long)slots, tiles_xy, nz, best_tz);
+    return best_tz;
 }
 
 // The fused instantiation serving this plan (2-D images run as one plane
@@ -301,7 +335,7 @@ void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& 
     const int TY = fe ? fe->ty : 32;
     A.tiles_x = (A.n_dst[0] + kFusedTileX - 1) / kFusedTileX;
     A.tiles_y = (A.n_dst[1] + TY - 1) / TY;
-    A.tz = tz > 0 ? tz : fused_tz(A.tiles_x * A.tiles_y, A.n_dst[2], rz);
+    A.tz = tz > 0 ? tz : fused_tz(fe, mode, A.tiles_x * A.tiles_y, A.n_dst[2], rz);
     A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
 }
 
@@ -323,6 +357,16 @@ bool fits_i32(const Plan& p) {
     return true;
 }
 
+bool fused_src_ok(const Plan& p, const itkg_view& src, int ncomp, bool adjoint) {
+    const itkg_region& sreg = adjoint ? p.req : p.in_reg;
+    int64_t extent = ncomp - 1;  // largest in-plane element offset
+    for (int d = 0; d < std::min(p.D, 2); ++d) {
+        if (src.stride[d] < 0) return false;
+        if (sreg.size[d] > 0) extent += (sreg.size[d] - 1) * src.stride[d] * ncomp;
+    }
+    return extent < (INT64_C(1) << 30);
+}
+
 bool use_fused(const Plan& p, itkg_dtype dt, int ncomp) {
     const int path = g_path.load();
     if (path == ITKG_PATH_PER_AXIS) return false;
@@ -332,7 +376,7 @@ bool use_fused(const Plan& p, itkg_dtype dt, int ncomp) {
 int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
                  const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch) {
     if (region_len(p.D, p.req) == 0) return ITKG_OK;
-    if (use_fused(p, dt, ncomp))
+    if (use_fused(p, dt, ncomp) && fused_src_ok(p, in, ncomp, false))
         return run_fused(p, ncomp, in, out, arith == ITKG_FAST ? FUSED_FWD_FAST : FUSED_FWD_EXACT, 0,
                          s);
     if (g_path.load() == ITKG_PATH_FUSED)
@@ -381,7 +425,8 @@ int backward_per_axis(const Plan& p, int ncomp, const itkg_view& gout, const itk
 int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout,
                   const itkg_view& gin, int accumulate, hipStream_t s, void* scratch) {
     if (region_len(p.D, p.req) == 0) return ITKG_OK;
-    if (use_fused(p, dt, ncomp)) return run_fused(p, ncomp, gout, gin, FUSED_ADJ, accumulate, s);
+    if (use_fused(p, dt, ncomp) && fused_src_ok(p, gout, ncomp, true))
+        return run_fused(p, ncomp, gout, gin, FUSED_ADJ, accumulate, s);
     if (g_path.load() == ITKG_PATH_FUSED)
         return fail(ITKG_ERR_UNSUPPORTED, "fused path requested but not available for this call");
     return dt == ITKG_F64 ? backward_per_axis<double>(p, ncomp, gout, gin, accumulate, s, scratch)

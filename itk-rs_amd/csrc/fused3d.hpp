@@ -43,11 +43,6 @@
 
 #include "itkg_impl.hpp"
 
-// Minimum resident waves per SIMD the register allocator must keep (1: no
-// constraint).
-#ifndef ITKG_WAVES_PER_EU
-#define ITKG_WAVES_PER_EU 1
-#endif
 
 namespace itkg {
 
@@ -68,17 +63,34 @@ struct Cfg {
     static constexpr int TX = 64;
     static constexpr int CPT = RZ <= 3 ? 8 : (RZ <= 10 ? 4 : 2);
     static constexpr int TY = NW * CPT;
-    static constexpr int WX = TX + 2 * RXY;       // loaded window width
+    static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows
-    static constexpr int RG = (HY + 3) / 4;       // x-pass row groups of 4
-    static constexpr int ROWS = 4 * RG;           // LDS rows allocated
-    static constexpr int SIN = (WX + 3) / 4 * 4;  // sIn row stride (floats)
-    static constexpr int PLANE = ROWS * SIN;      // one half of the sIn double buffer
+    static constexpr int E = WX * HY;             // window elements per plane
+    static constexpr int LJ = (E + NTH - 1) / NTH;  // window loads per lane and plane
+    // x pass: each lane produces Q consecutive outputs of one row; an item
+    // (one wave's 64 lanes) covers Q rows.  Q in {4, 2, 1} is chosen to
+    // balance the ceil(HY / Q) items over the NW waves (every wave waits for
+    // the slowest at the barrier that follows); Q-wide LDS reads need the
+    // row stride WX to be a multiple of Q.
+    static constexpr int xcost(int q) {
+        const int items = (HY + q - 1) / q;
+        const int per_wave = (items + NW - 1) / NW;
+        return (WX % q) ? 1 << 30 : per_wave * (q * (2 * RXY + 1) + 2 * ((q + 2 * RXY + q - 1) / q));
+    }
+    static constexpr int Q = xcost(4) <= xcost(2) ? (xcost(4) <= xcost(1) ? 4 : 1)
+                                                  : (xcost(2) <= xcost(1) ? 2 : 1);
+    static constexpr int RG = (HY + Q - 1) / Q;   // x-pass items
+    static constexpr int XQ = (Q + 2 * RXY + Q - 1) / Q;  // Q-wide reads per x item
+    // One sIn plane: the window (row stride WX), room for the unconditional
+    // stores of the last loads (LJ * NTH) and for the x pass's reads of the
+    // rows past HY in its last item (their outputs are never used).
+    static constexpr int PLANE0 = LJ * NTH > Q * RG * WX ? LJ * NTH : Q * RG * WX;
+    static constexpr int PLANE = (PLANE0 + 3) / 4 * 4;
     static constexpr int SX = TX;                 // sX row stride (floats)
-    static constexpr int XQ = (4 + 2 * RXY + 3) / 4;  // float4 reads per x item
-    static constexpr int LROWS = (HY + NW - 1) / NW;  // rows loaded per wave
+    static constexpr int XPLANE = Q * RG * SX;    // one sX buffer
     static constexpr int P = 2 * RZ + 1;
     static_assert(WX - 64 <= 64, "halo wider than one wave");
+    static_assert((2 * PLANE + 2 * XPLANE) * 4 <= 160 * 1024, "LDS budget of one workgroup");
 };
 
 // Compile-time loop: f(integral_constant<int, I>) for I = 0..N-1.
@@ -125,19 +137,18 @@ __device__ __forceinline__ float tap_sum(const float* v, const float* w) {
 }
 
 // One destination tile (64 x TY x TZ) of one volume.  The caller's previous
-// tile, if any, may still be using the LDS tiles (the tile starts with a
+// tile, if any, may still be using the LDS buffers (the tile starts with a
 // barrier).
 template <int RXY, int RZ, int MODE>
 __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int tile, float* sIn,
                                              float* sX) {
     using C = Cfg<RXY, RZ>;
-    constexpr int CPT = C::CPT, P = C::P;
+    constexpr int CPT = C::CPT, P = C::P, LJ = C::LJ;
     constexpr bool ADJOINT = MODE == FUSED_ADJ;
     const int tx_i = tile % a.tiles_x;
     const int ty_i = (tile / a.tiles_x) % a.tiles_y;
     const int tz_i = tile / (a.tiles_x * a.tiles_y);
     const int x0 = tx_i * C::TX, y0 = ty_i * C::TY, z0 = tz_i * a.tz;
-
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
@@ -145,80 +156,84 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     const int ndx = a.n_dst[0], ndy = a.n_dst[1], ndz = a.n_dst[2];
     auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
 
-    // ---- loads of one source plane (rows wave, wave+NW, ...; two columns per
-    // lane, the second only meaningful for the 2r halo lanes).  The loads are
-    // unconditional, at clamped addresses, so every step issues the same
-    // number and the wait before a register set is used covers that set only;
-    // zeroing (adjoint) and the lanes without a second column are applied
-    // when the set is written to LDS.
-    const int px0 = x0 + a.shift[0] - RXY + lane;
-    const int px1 = px0 + 64;
-    const bool has1 = 64 + lane < C::WX;
-    const bool ok0 = !ADJOINT || (px0 >= 0 && px0 < nsx);
-    const bool ok1 = !ADJOINT || (px1 >= 0 && px1 < nsx);
-    const int64_t sxs = a.src_stride[0], sys = a.src_stride[1], szs = a.src_stride[2];
-    const int64_t xo0 = clampi(px0, 0, nsx - 1) * sxs;
-    const int64_t xo1 = has1 ? clampi(px1, 0, nsx - 1) * sxs : xo0;
+    // ---- window loads.  Window element e = row * WX + col (row-major, the
+    // sIn layout) is loaded by lane e % 64 of wave (e / 64) % NW as its load
+    // j = e / NTH.  The in-plane byte offsets are computed once per tile
+    // (forward: clamped into the source; adjoint: zeros outside, kept as a
+    // bit mask), so a plane's loads are a scalar base plus these offsets and
+    // every step issues the same LJ loads.
+    uint32_t gofs[LJ];
+    uint32_t okmask = 0;
+#pragma unroll
+    for (int j = 0; j < LJ; ++j) {
+        const int e = min((j * C::NW + wave) * 64 + lane, C::E - 1);
+        const int row = e / C::WX, col = e - row * C::WX;
+        const int py = y0 + a.shift[1] - RXY + row;
+        const int px = x0 + a.shift[0] - RXY + col;
+        if (ADJOINT && py >= 0 && py < nsy && px >= 0 && px < nsx) okmask |= 1u << j;
+        gofs[j] = (uint32_t)(clampi(py, 0, nsy - 1) * a.src_stride[1] +
+                             clampi(px, 0, nsx - 1) * a.src_stride[0]) *
+                  (uint32_t)sizeof(float);
+    }
     const int zv0 = z0 + a.shift[2] - RZ;  // source plane of step u = 0
-
-    // Two register sets of one plane each.  Source plane of step u: forward
-    // clamps it into the source, adjoint reads zeros outside.
-    float pa0[C::LROWS], pa1[C::LROWS], pb0[C::LROWS], pb1[C::LROWS];
-    auto load_plane = [&](float* r0p, float* r1p, int u) {
+    const char* src_bytes = reinterpret_cast<const char*>(a.src);
+    auto load_plane = [&](float* r, int u) {
         const int zr = clampi(zv0 + u, 0, nsz - 1);
-        const float* plane = a.src + (int64_t)zr * szs;
+        const char* plane = src_bytes + (int64_t)zr * a.src_stride[2] * (int64_t)sizeof(float);
 #pragma unroll
-        for (int j = 0; j < C::LROWS; ++j) {
-            const int row = min(wave + j * C::NW, C::HY - 1);
-            const int py = clampi(y0 + a.shift[1] - RXY + row, 0, nsy - 1);
-            const float* rp = plane + (int64_t)py * sys;
-            r0p[j] = rp[xo0];
-            r1p[j] = rp[xo1];
-        }
+        for (int j = 0; j < LJ; ++j) r[j] = *reinterpret_cast<const float*>(plane + gofs[j]);
     };
-    auto store_plane = [&](const float* r0p, const float* r1p, float* dstp, int u) {
-        const int zr = zv0 + u;
-        const bool zok = !ADJOINT || (zr >= 0 && zr < nsz);
+    float* const sIn_lane = sIn + wave * 64 + lane;
+    auto store_plane = [&](const float* r, int half, int u) {
+        float* d = sIn_lane + half * C::PLANE;
+        if constexpr (ADJOINT) {
+            const int zr = zv0 + u;
+            const uint32_t m = (zr >= 0 && zr < nsz) ? okmask : 0u;
 #pragma unroll
-        for (int j = 0; j < C::LROWS; ++j) {
-            const int row = wave + j * C::NW;
-            if (row < C::HY) {
-                const int py = y0 + a.shift[1] - RXY + row;
-                const bool rok = zok && (!ADJOINT || (py >= 0 && py < nsy));
-                dstp[row * C::SIN + lane] = (rok && ok0) ? r0p[j] : 0.0f;
-                if (has1) dstp[row * C::SIN + 64 + lane] = (rok && ok1) ? r1p[j] : 0.0f;
-            }
+            for (int j = 0; j < LJ; ++j) d[j * C::NTH] = ((m >> j) & 1u) ? r[j] : 0.0f;
+        } else {
+#pragma unroll
+            for (int j = 0; j < LJ; ++j) d[j * C::NTH] = r[j];
         }
     };
 
     const float* wx = a.wx;
     const float* wy = a.wy;
 
-    // ---- x pass: row groups g = wave, wave+NW, ...; lane -> (row in group, quad)
-    const int xq = lane & 15, xr = lane >> 4;
-    auto x_pass = [&](const float* srcp) {
+    // ---- x pass: items g = wave, wave+NW, ...; lane -> (row in item, Q-group)
+    constexpr int Q = C::Q, LPR = 64 / Q;
+    using vecq = std::conditional_t<Q == 4, float4, std::conditional_t<Q == 2, float2, float>>;
+    const int xq = lane % LPR, xr = lane / LPR;
+    auto x_pass = [&](const float* srcp, float* dstp) {
 #pragma unroll
         for (int g0 = 0; g0 < C::RG; g0 += C::NW) {
             const int g = g0 + wave;
             if (g < C::RG) {
-                const int row = 4 * g + xr;
-                const float4* src4 = reinterpret_cast<const float4*>(&srcp[row * C::SIN + 4 * xq]);
-                float v[4 * C::XQ];
+                const int row = Q * g + xr;
+                const vecq* srcq = reinterpret_cast<const vecq*>(&srcp[row * C::WX + Q * xq]);
+                float v[Q * C::XQ];
 #pragma unroll
                 for (int i = 0; i < C::XQ; ++i) {
-                    const float4 t = src4[i];
-                    v[4 * i + 0] = t.x;
-                    v[4 * i + 1] = t.y;
-                    v[4 * i + 2] = t.z;
-                    v[4 * i + 3] = t.w;
+                    const vecq t = srcq[i];
+                    if constexpr (Q == 4) {
+                        v[4 * i + 0] = t.x;
+                        v[4 * i + 1] = t.y;
+                        v[4 * i + 2] = t.z;
+                        v[4 * i + 3] = t.w;
+                    } else if constexpr (Q == 2) {
+                        v[2 * i + 0] = t.x;
+                        v[2 * i + 1] = t.y;
+                    } else {
+                        v[i] = t;
+                    }
                 }
-                float o[4];
+                float o[Q];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) o[j] = tap_sum<RXY, MODE>(v + j, wx);
+                for (int j = 0; j < Q; ++j) o[j] = tap_sum<RXY, MODE>(v + j, wx);
                 if constexpr (ADJOINT) {
-                    const int xg = x0 + 4 * xq;  // destination x of o[0]
+                    const int xg = x0 + Q * xq;  // destination x of o[0]
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
+                    for (int j = 0; j < Q; ++j) {
                         if (xg + j == 0) {
 #pragma unroll
                             for (int m = 1; m <= RXY; ++m)
@@ -231,8 +246,10 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                         }
                     }
                 }
-                *reinterpret_cast<float4*>(&sX[row * C::SX + 4 * xq]) =
-                    make_float4(o[0], o[1], o[2], o[3]);
+                vecq* dq = reinterpret_cast<vecq*>(&dstp[row * C::SX + Q * xq]);
+                if constexpr (Q == 4) *dq = make_float4(o[0], o[1], o[2], o[3]);
+                else if constexpr (Q == 2) *dq = make_float2(o[0], o[1]);
+                else *dq = o[0];
             }
         }
     };
@@ -240,10 +257,10 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     // ---- y pass: CPT outputs down column `lane`, rows wave*CPT ...
     const int ybase = y0 + wave * CPT;
     float val[CPT];
-    auto y_pass = [&]() {
+    auto y_pass = [&](const float* xp) {
         float v[CPT + 2 * RXY];
 #pragma unroll
-        for (int i = 0; i < CPT + 2 * RXY; ++i) v[i] = sX[(wave * CPT + i) * C::SX + lane];
+        for (int i = 0; i < CPT + 2 * RXY; ++i) v[i] = xp[(wave * CPT + i) * C::SX + lane];
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
             val[c] = tap_sum<RXY, MODE>(v + c, wy);
@@ -282,18 +299,32 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     const int U = tz_act + 2 * RZ;
     const int t_hi = ndz - 1 - z0;  // step-relative index of the last destination plane
 
-    // One step: filter plane u from LDS half `cur`; store plane u+1 (in the
-    // register set `r`) into the other half and reload `r` with plane u+3.
+    // Step u, entered with plane u x-filtered in sX half u % 2, plane u + 1
+    // in sIn half (u + 1) % 2 and plane u + 2 in register set `r`:
+    //   y pass of plane u, one z step, store of the completed output plane;
+    //   plane u + 2 into sIn half u % 2 (its plane u was read before the last
+    //   barrier), `r` reloaded with plane u + 4;
+    //   x pass of plane u + 1 into sX half (u + 1) % 2 (read by step u - 1,
+    //   before the last barrier);
+    //   one barrier.
     int s = 0;
-    auto step = [&](int u, float* ra, float* rb, float* cur, float* nxt) {
+    auto step = [&](int u, float* r, const int h) {
         const int t = u - 2 * RZ;  // output completed at this step (if 0 <= t < tz_act)
-        x_pass(cur);
-        lds_barrier();  // x-filtered tile complete, previous step's readers of nxt done
-        store_plane(ra, rb, nxt, u + 1);
-        load_plane(ra, rb, u + 3);
-        y_pass();
+        y_pass(sX + h * C::XPLANE);
         const int sn = (s + 1 == P) ? 0 : s + 1;
         const float* w = a.wz2 + s + P;  // w[-j] = wz[(s - j) mod P]
+        // Adjoint z-edge weights of this plane (uniform; chosen by compare so
+        // that the argument block is never indexed with a runtime value).
+        float plo = 0.0f, phi = 0.0f;
+        if constexpr (ADJOINT) {
+            const int mlo = (z0 == 0) ? 2 * RZ - u : -1;
+            const int mhi = u - t_hi;
+#pragma unroll
+            for (int m = 1; m <= RZ; ++m) {
+                plo = (mlo == m) ? a.pz[m] : plo;
+                phi = (mhi == m) ? a.pz[m] : phi;
+            }
+        }
         float o[CPT];
         static_for<CPT>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
@@ -310,8 +341,6 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
             }
             o[c] = acc[sn];  // tap 2r has just been added
             if constexpr (ADJOINT) {
-                const float plo = (z0 == 0 && u >= RZ && u < 2 * RZ) ? a.pz[2 * RZ - u] : 0.0f;
-                const float phi = (u > t_hi && u <= t_hi + RZ) ? a.pz[u - t_hi] : 0.0f;
                 tlo[c] = __builtin_fmaf(val[c], plo, tlo[c]);
                 thi[c] = __builtin_fmaf(val[c], phi, thi[c]);
                 if (z0 + t == 0) o[c] = o[c] + tlo[c];
@@ -332,28 +361,52 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                 }
             }
         }
-        lds_barrier();  // next plane stored, x-filtered tile consumed
+        store_plane(r, h, u + 2);
+        load_plane(r, u + 4);
+        x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
+        lds_barrier();
     };
 
+    float ra[LJ], rb[LJ];
     lds_barrier();  // the previous tile of a persistent workgroup is done with LDS
-    load_plane(pa0, pa1, 0);
-    store_plane(pa0, pa1, sIn, 0);
-    load_plane(pa0, pa1, 1);
-    load_plane(pb0, pb1, 2);
+    load_plane(ra, 0);
+    load_plane(rb, 1);
+    store_plane(ra, 0, 0);
+    store_plane(rb, 1, 1);
+    load_plane(ra, 2);
+    load_plane(rb, 3);
+    lds_barrier();
+    x_pass(sIn, sX);
     lds_barrier();
     // An even number of steps: the extra last step (if any) stores nothing.
     for (int u = 0; u < U; u += 2) {
-        step(u, pa0, pa1, sIn, sIn + C::PLANE);
-        step(u + 1, pb0, pb1, sIn + C::PLANE, sIn);
+        step(u, ra, 0);
+        step(u + 1, rb, 1);
     }
 }
 
+// Minimum resident waves per SIMD the register allocator must keep.  For
+// the FMA modes (forward FAST, adjoint) of the radii listed, capping the
+// kernel at 128 VGPRs (two 8-wave workgroups per CU) costs at most a few
+// dwords of spill and pays for itself in latency hiding; the other radii
+// and EXACT (whose ring needs more registers) run unconstrained.
+template <int RZ, int MODE>
+struct WavesPerEu {
+    static constexpr bool cap =
+        MODE != FUSED_FWD_EXACT && RZ >= 4 && (RZ <= 7 || (RZ >= 11 && RZ <= 13));
+    static constexpr int value = cap ? 4 : 1;
+};
+
 template <int RXY, int RZ, int MODE>
-__global__ __launch_bounds__((Cfg<RXY, RZ>::NTH)) __attribute__((amdgpu_waves_per_eu(ITKG_WAVES_PER_EU)))
-void fused3d_kernel(const Fused3dArgs a) {
+__global__ __launch_bounds__((Cfg<RXY, RZ>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
+void fused3d_kernel([[maybe_unused]] const Fused3dArgs args) {
     using C = Cfg<RXY, RZ>;
+    // The argument block is read in place from the kernel-argument segment
+    // (where `args` lies, at offset 0), so that runtime-indexed weights are
+    // scalar loads and the block is never copied to private memory.
+    const Fused3dArgs& a = *(const Fused3dArgs*)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ __attribute__((aligned(16))) float sIn[2 * C::PLANE];
-    __shared__ __attribute__((aligned(16))) float sX[C::ROWS * C::SX];
+    __shared__ __attribute__((aligned(16))) float sX[2 * C::XPLANE];
     // XCD-aware tile order: hardware deals workgroups to the 8 XCDs round
     // robin, so consecutive tiles (which share halos) go to one XCD's L2.
     const int nt = a.tiles_x * a.tiles_y * a.tiles_z;
@@ -367,12 +420,12 @@ void fused3d_kernel(const Fused3dArgs a) {
 // volumes of one radius: workgroup b takes items b, b + G, b + 2G, ...
 // The host interleaves the list so that items b and b + G share an XCD.
 template <int RXY, int RZ, int MODE>
-__global__ __launch_bounds__((Cfg<RXY, RZ>::NTH)) __attribute__((amdgpu_waves_per_eu(ITKG_WAVES_PER_EU)))
+__global__ __launch_bounds__((Cfg<RXY, RZ>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
 void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __restrict__ work,
                           const int nwork) {
     using C = Cfg<RXY, RZ>;
     __shared__ __attribute__((aligned(16))) float sIn[2 * C::PLANE];
-    __shared__ __attribute__((aligned(16))) float sX[C::ROWS * C::SX];
+    __shared__ __attribute__((aligned(16))) float sX[2 * C::XPLANE];
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
         if (it.x < 0) continue;  // padding of the interleaved list
@@ -423,32 +476,38 @@ hipError_t launch_batch_r(const Fused3dArgs* vols, const int2* work, int nwork, 
     return hipGetLastError();
 }
 
-template <int RXY, int RZ>
-int batch_occupancy_r(int mode) {
+// Resident workgroups per CU of the plain (persist = false) or persistent
+// batch (persist = true) kernel.
+template <int RXY, int RZ, bool PERSIST>
+int occupancy_r(int mode) {
     int blocks = 0;
     hipError_t e = hipErrorInvalidValue;
     const int nth = Cfg<RXY, RZ>::NTH;
+    auto query = [&](auto kern) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, nth, 0);
+    };
     if (mode == FUSED_FWD_EXACT)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>, nth, 0);
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT>);
     else if (mode == FUSED_FWD_FAST)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST>, nth, 0);
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST>)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_FAST>);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>, nth, 0);
-    return e == hipSuccess ? blocks : 1;
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_ADJ>);
+    return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
 // Registry entry of one (rxy, rz) instantiation.
 template <int RXY, int RZ>
 constexpr FusedEntry make_entry() {
     return FusedEntry{RXY, RZ, Cfg<RXY, RZ>::TY, &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>,
-                      &batch_occupancy_r<RXY, RZ>};
+                      &occupancy_r<RXY, RZ, true>, &occupancy_r<RXY, RZ, false>};
 }
 template <int RXY, int RZ>
 constexpr FusedEntry make_entry_single() {
-    return FusedEntry{RXY, RZ, Cfg<RXY, RZ>::TY, &launch_r<RXY, RZ>, nullptr, nullptr};
+    return FusedEntry{RXY, RZ, Cfg<RXY, RZ>::TY, &launch_r<RXY, RZ>, nullptr, nullptr,
+                      &occupancy_r<RXY, RZ, false>};
 }
 
 }  // namespace

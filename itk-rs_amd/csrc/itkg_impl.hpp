@@ -77,7 +77,8 @@ struct FusedEntry {
     hipError_t (*launch)(const Fused3dArgs& a, int mode, hipStream_t s);
     hipError_t (*batch)(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                         int mode, hipStream_t s);
-    int (*occupancy)(int mode);  // resident workgroups per CU of the persistent form
+    int (*occupancy)(int mode);        // resident workgroups per CU of the persistent form
+    int (*plain_occupancy)(int mode);  // resident workgroups per CU of the plain form
 };
 constexpr int kFusedTileX = 64;
 // The instantiation for (rxy, rz), or null.
@@ -102,7 +103,11 @@ size_t scratch_bytes(const Plan& p, itkg_dtype dt, int ncomp);
 // True when the fused kernel serves this plan (and the path setting allows it).
 bool use_fused(const Plan& p, itkg_dtype dt, int ncomp);
 const FusedEntry* fused_for(const Plan& p);
-int fused_tz(int tiles_xy, int nz, int r);
+// z run of a fused tile for a grid of tiles_xy tiles per z run over nz planes.
+int fused_tz(const FusedEntry* fe, int mode, int tiles_xy, int nz, int r);
+// True when the fused kernel can address the source view of this call (in-
+// plane offsets are 32-bit).
+bool fused_src_ok(const Plan& p, const itkg_view& src, int ncomp, bool adjoint);
 void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
                      int mode, int accumulate, int tz, int comp, int ncomp, Fused3dArgs* out);
 // Enqueue one forward / adjoint; `scratch` (nullable) holds scratch_bytes().

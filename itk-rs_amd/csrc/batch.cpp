@@ -8,8 +8,11 @@
 // work list of (volume, tile) items, and runs as ONE persistent launch of the
 // fused kernel over that list.  Other volumes (anisotropic radii, radius
 // beyond the fused set) run the per-axis path with scratch preallocated at
-// plan time.  itkg_batch_run captures the whole sequence into a HIP graph on
-// a stream's first run and replays it afterwards.
+// plan time.  The groups and the per-axis remainder are dealt out to a few
+// branches (internal streams forked from and joined back into the caller's
+// stream), so that one group's tail overlaps the next group's start.
+// itkg_batch_run captures the whole forked sequence into a HIP graph on a
+// stream's first run and replays it afterwards.
 //
 // Autotuner: the z run of a tile (TZ) trades halo recomputation against
 // parallelism; itkg_batch_autotune times TZ candidates per radius group on
@@ -47,7 +50,8 @@ struct BatchClass {
     int grid = 0;
 };
 
-int default_tz(int r) { return r <= 4 ? 64 : 32; }
+// Branches (side streams) a plan forks its launches into.
+constexpr int kBatchBranches = 4;
 
 using TuneKey = std::tuple<int, int, int>;  // direction, arith, radius
 
@@ -87,9 +91,18 @@ struct BatchImpl {
     int num_cu = 256;
     std::mutex mu;
     std::unordered_map<hipStream_t, hipGraphExec_t> graphs;
+    hipStream_t side[kBatchBranches] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kBatchBranches] = {};
+    std::vector<std::vector<int>> branch_units;  // per branch: class index, or -1 = per-axis rest
 
     ~BatchImpl() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (int b = 0; b < kBatchBranches; ++b) {
+            if (side[b]) (void)hipStreamDestroy(side[b]);
+            if (join_ev[b]) (void)hipEventDestroy(join_ev[b]);
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
         for (auto& c : classes) {
             if (c.d_args) (void)hipFree(c.d_args);
             if (c.d_work) (void)hipFree(c.d_work);
@@ -100,6 +113,60 @@ struct BatchImpl {
     void drop_graphs() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
+    }
+
+    // Modelled time of one group at z run tz, in plane steps per slot: the
+    // persistent grid deals the tiles out evenly, and the last tile dealt can
+    // finish up to one tile's walk after the others.
+    double model_cost(const BatchClass& c, int tz) const {
+        const int64_t slots = (int64_t)num_cu * std::max(1, c.entry->occupancy(mode));
+        double steps = 0.0, longest = 0.0;
+        for (int i : c.vol) {
+            const itkg_volume& v = vols[i];
+            const int64_t txy = ((v.size[0] + kFusedTileX - 1) / kFusedTileX) *
+                                ((v.size[1] + c.entry->ty - 1) / c.entry->ty);
+            const int64_t nz = v.size[2];
+            const int64_t tzv = std::min<int64_t>(tz, nz);
+            const double walk = (double)(tzv + 2 * c.r + 3);
+            steps += (double)(txy * ((nz + tz - 1) / tz)) * walk;
+            longest = std::max(longest, walk);
+        }
+        return steps / (double)slots + longest;
+    }
+
+    int model_tz(const BatchClass& c) const {
+        int best = 32;
+        double best_cost = 1e300;
+        for (int tz : {16, 24, 32, 48, 64, 96, 128, 192, 256}) {
+            const double cost = model_cost(c, tz);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best = tz;
+            }
+        }
+        return best;
+    }
+
+    // Deals the groups (largest modelled time first) and the per-axis
+    // remainder out to the branches, each to the least loaded one.
+    void assign_branches() {
+        std::vector<std::pair<double, int>> units;
+        for (size_t i = 0; i < classes.size(); ++i)
+            units.push_back({model_cost(classes[i], classes[i].tz), (int)i});
+        if (!rest.empty()) {
+            double w = 0.0;
+            for (int i : rest) w += (double)region_len(3, plans[i].req) * 3.0 / (64.0 * num_cu);
+            units.push_back({w, -1});
+        }
+        std::sort(units.begin(), units.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        const int nb = std::min<int>(kBatchBranches, (int)units.size());
+        branch_units.assign(nb, {});
+        std::vector<double> load(nb, 0.0);
+        for (const auto& u : units) {
+            const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[b] += u.first;
+            branch_units[b].push_back(u.second);
+        }
     }
 
     // (Re)builds the device tables of one group for a z run of `tz`.
@@ -148,11 +215,7 @@ struct BatchImpl {
         return ITKG_OK;
     }
 
-    int enqueue(hipStream_t s) {
-        for (const BatchClass& c : classes) {
-            const int rc = enqueue_class(c, s);
-            if (rc) return rc;
-        }
+    int enqueue_rest(hipStream_t s) const {
         for (int i : rest) {
             const itkg_volume& v = vols[i];
             const Plan& p = plans[i];
@@ -162,6 +225,25 @@ struct BatchImpl {
                                ? forward_impl(p, ITKG_F32, 1, vin, vout, arith, s, scratch)
                                : backward_impl(p, ITKG_F32, 1, vin, vout, 0, s, scratch);
             if (rc) return rc;
+        }
+        return ITKG_OK;
+    }
+
+    // The whole plan on `s`: fork to the branches, run each branch's units in
+    // order, join back into `s`.
+    int enqueue(hipStream_t s) {
+        hipError_t e = hipEventRecord(fork_ev, s);
+        if (e != hipSuccess) return hip_fail(e, "batch fork");
+        for (size_t b = 0; b < branch_units.size(); ++b) {
+            e = hipStreamWaitEvent(side[b], fork_ev, 0);
+            if (e != hipSuccess) return hip_fail(e, "batch fork");
+            for (int u : branch_units[b]) {
+                const int rc = u < 0 ? enqueue_rest(side[b]) : enqueue_class(classes[u], side[b]);
+                if (rc) return rc;
+            }
+            e = hipEventRecord(join_ev[b], side[b]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, join_ev[b], 0);
+            if (e != hipSuccess) return hip_fail(e, "batch join");
         }
         return ITKG_OK;
     }
@@ -242,15 +324,25 @@ int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_directio
             return hip_fail(e, "hipMalloc(batch scratch)");
         }
     }
+    hipError_t e = hipEventCreateWithFlags(&B.fork_ev, hipEventDisableTiming);
+    for (int k = 0; k < kBatchBranches && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&B.side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&B.join_ev[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "batch streams");
+    }
     const auto cache = read_cache();
     for (BatchClass& c : B.classes) {
         auto it = cache.find({(int)direction, (int)arith, c.r});
-        const int rc = B.build_class(c, it != cache.end() ? it->second : default_tz(c.r));
+        const int rc = B.build_class(c, it != cache.end() ? it->second : B.model_tz(c));
         if (rc) {
             delete b;
             return rc;
         }
     }
+    B.assign_branches();
     *plan = b;
     return ITKG_OK;
 }
@@ -296,6 +388,7 @@ int itkg_batch_autotune(itkg_batch* plan, void* stream) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    B.assign_branches();
     B.drop_graphs();
     return rc;
 }

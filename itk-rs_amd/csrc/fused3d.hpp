@@ -200,6 +200,10 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     const float* wx = a.wx;
     const float* wy = a.wy;
 
+    // Adjoint edge terms exist only in tiles that hold an image edge (tile-
+    // uniform tests, so interior tiles skip them).
+    const bool x_edge = x0 == 0 || x0 + C::TX >= ndx;
+
     // ---- x pass: items g = wave, wave+NW, ...; lane -> (row in item, Q-group)
     constexpr int Q = C::Q, LPR = 64 / Q;
     using vecq = std::conditional_t<Q == 4, float4, std::conditional_t<Q == 2, float2, float>>;
@@ -232,6 +236,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                 for (int j = 0; j < Q; ++j) o[j] = tap_sum<RXY, MODE>(v + j, wx);
                 if constexpr (ADJOINT) {
                     const int xg = x0 + Q * xq;  // destination x of o[0]
+                    if (x_edge) {
 #pragma unroll
                     for (int j = 0; j < Q; ++j) {
                         if (xg + j == 0) {
@@ -244,6 +249,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                             for (int m = 1; m <= RXY; ++m)
                                 o[j] = __builtin_fmaf(a.px[m], v[j + m], o[j]);
                         }
+                    }
                     }
                 }
                 vecq* dq = reinterpret_cast<vecq*>(&dstp[row * C::SX + Q * xq]);
@@ -298,6 +304,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     const int tz_act = min(a.tz, ndz - z0);
     const int U = tz_act + 2 * RZ;
     const int t_hi = ndz - 1 - z0;  // step-relative index of the last destination plane
+    const bool z_edge = z0 == 0 || z0 + tz_act >= ndz;
 
     // Step u, entered with plane u x-filtered in sX half u % 2, plane u + 1
     // in sIn half (u + 1) % 2 and plane u + 2 in register set `r`:
@@ -316,7 +323,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         // Adjoint z-edge weights of this plane (uniform; chosen by compare so
         // that the argument block is never indexed with a runtime value).
         float plo = 0.0f, phi = 0.0f;
-        if constexpr (ADJOINT) {
+        if (ADJOINT && z_edge) {
             const int mlo = (z0 == 0) ? 2 * RZ - u : -1;
             const int mhi = u - t_hi;
 #pragma unroll
@@ -341,10 +348,12 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
             }
             o[c] = acc[sn];  // tap 2r has just been added
             if constexpr (ADJOINT) {
-                tlo[c] = __builtin_fmaf(val[c], plo, tlo[c]);
-                thi[c] = __builtin_fmaf(val[c], phi, thi[c]);
-                if (z0 + t == 0) o[c] = o[c] + tlo[c];
-                if (t == t_hi) o[c] = o[c] + thi[c];
+                if (z_edge) {
+                    tlo[c] = __builtin_fmaf(val[c], plo, tlo[c]);
+                    thi[c] = __builtin_fmaf(val[c], phi, thi[c]);
+                    if (z0 + t == 0) o[c] = o[c] + tlo[c];
+                    if (t == t_hi) o[c] = o[c] + thi[c];
+                }
             }
         });
         s = sn;

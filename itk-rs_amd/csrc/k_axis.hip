@@ -1,68 +1,209 @@
 // Per-axis clamp-to-edge pass (forward) and its transpose (adjoint).
 //
 // Forward: the convolve_axis of src/filters/conv.rs:11-41 for any ndim <= 4,
-// f32/f64, 1..4 interleaved components, on strided views.  One thread per
-// output pixel; it is the general path (any D, any radius up to
-// ITKG_MAX_RADIUS) and the arithmetic reference the fused kernels match.
+// f32/f64, 1..4 interleaved components, on strided views.  It is the
+// general path (any D, any radius up to ITKG_MAX_RADIUS, anisotropic radii)
+// and the arithmetic reference the fused kernels match.
 //
 // EXACT arithmetic reproduces conv.rs:27-35 with pixel.rs:21: the
 // accumulator starts at +0 and each tap adds fl(x * (T)w_k), taps in order,
 // no FMA (this file is compiled with -ffp-contract=off and the multiply and
-// add are written as separate statements).
+// add are written as separate statements).  FAST: one FMA per tap.
+//
+// Work decomposition (one wave = one work item, no workgroup barriers):
+//   - pass axis != 0 ("strided"): the 64 lanes take 64 consecutive pixels
+//     along axis 0 and each walks a run of CPT outputs along the pass axis,
+//     reading the CPT + 2r inputs of the run once (coalesced across lanes,
+//     clamped to the pass input extent) and adding each input into every
+//     output of the run that has it as a tap (a sliding window);
+//   - pass axis 0 ("row"): the wave stages the 64 * CPT + 2r inputs of a row
+//     segment in LDS (coalesced), then each lane runs the same sliding window
+//     over CPT consecutive outputs, reading LDS through a padded index (one
+//     pad word per 32) that keeps the lanes' strided reads conflict-free.
+// Every output still receives its taps k = 0..2r in increasing order.
 #include "itkg_impl.hpp"
+
+#include <type_traits>
+#include <utility>
 
 namespace itkg {
 
 namespace {
 
-template <typename T, int NC, bool FAST>
-__global__ __launch_bounds__(256) void axis_forward_kernel(const AxisPassArgs<T> a) {
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F&& f) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// One tap of one output component.
+template <typename T, bool FAST>
+__device__ __forceinline__ T tap(T acc, T x, T w) {
+    if constexpr (FAST) {
+        return __builtin_fma(x, w, acc);
+    } else {
+        const T prod = x * w;
+        return acc + prod;
+    }
+}
+
+// Sliding window over a run of CPT outputs: input k (k = 0 .. CPT - 1 + 2r)
+// is tap k - c of output c.  Requires CPT <= 2r + 1, so that the middle
+// phase (every output of the run takes the input) separates the two ends.
+template <typename T, int NC, bool FAST, int CPT, class Load>
+__device__ __forceinline__ void conv_run(const T* w, const int r, Load&& load, T (&acc)[CPT][NC]) {
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[c][q] = T(0);
+    static_for<CPT - 1>([&](auto kk) {  // head: k < CPT - 1
+        constexpr int k = decltype(kk)::value;
+        T v[NC];
+        load(k, v);
+#pragma unroll
+        for (int c = 0; c <= k; ++c)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) acc[c][q] = tap<T, FAST>(acc[c][q], v[q], w[k - c]);
+    });
+#pragma unroll 2
+    for (int k = CPT - 1; k <= 2 * r; ++k) {  // middle: every output takes input k
+        T v[NC];
+        load(k, v);
+#pragma unroll
+        for (int c = 0; c < CPT; ++c)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) acc[c][q] = tap<T, FAST>(acc[c][q], v[q], w[k - c]);
+    }
+    static_for<CPT - 1>([&](auto dd) {  // tail: k = 2r + d, outputs c >= d
+        constexpr int d = decltype(dd)::value + 1;
+        const int k = 2 * r + d;
+        T v[NC];
+        load(k, v);
+#pragma unroll
+        for (int c = d; c < CPT; ++c)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) acc[c][q] = tap<T, FAST>(acc[c][q], v[q], w[k - c]);
+    });
+}
+
+// Pass along a strided axis (a.axis >= 1).
+template <typename T, int NC, bool FAST, int CPT>
+__global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] const AxisPassArgs<T> args) {
+    // Read in place from the kernel-argument segment: the weights are indexed
+    // with runtime values (scalar loads, no private copy of the block).
+    const AxisPassArgs<T>& a = *(const AxisPassArgs<T>*)__builtin_amdgcn_kernarg_segment_ptr();
     const int ax = a.axis;
-    const int64_t step = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < a.total;
-         lin += step) {
-        int64_t rem = lin;
-        int64_t c[ITKG_MAX_DIM];
+    const int lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the two axes that are neither 0 nor the pass axis
+    const int b1 = ax == 1 ? 2 : 1;
+    const int b2 = ax == 3 ? 2 : 3;
+    const int64_t n0c = (a.n[0] + 63) / 64;
+    const int64_t nac = (a.n[ax] + CPT - 1) / CPT;
+    int64_t it = item;
+    const int64_t i0 = it % n0c;
+    it /= n0c;
+    const int64_t ia = it % nac;
+    it /= nac;
+    const int64_t o1 = it % a.n[b1];
+    const int64_t o2 = it / a.n[b1];
+    if (o2 >= a.n[b2]) return;  // whole wave: past the last item
+    const int64_t col = i0 * 64 + lane;
+    const bool col_ok = col < a.n[0];
+    const int64_t colc = col_ok ? col : a.n[0] - 1;
+    const int64_t p0 = a.out_lo[0] + colc, p1 = a.out_lo[b1] + o1, p2 = a.out_lo[b2] + o2;
+    const T* src = a.src + ((p0 - a.src_lo[0]) * a.src_stride[0] + (p1 - a.src_lo[b1]) * a.src_stride[b1] +
+                            (p2 - a.src_lo[b2]) * a.src_stride[b2] - a.src_lo[ax] * a.src_stride[ax]) * NC;
+    const int64_t sa = a.src_stride[ax] * NC;
+    const int r = a.radius;
+    const int64_t first = a.out_lo[ax] + ia * CPT - r;  // pass-axis position of input k = 0
+    auto load = [&](int k, T (&v)[NC]) {
+        int64_t pos = first + k;
+        pos = pos < a.clamp_lo ? a.clamp_lo : (pos > a.clamp_hi ? a.clamp_hi : pos);
+        const T* p = src + pos * sa;
 #pragma unroll
-        for (int d = 0; d < ITKG_MAX_DIM - 1; ++d) {
-            c[d] = rem % a.n[d];
-            rem /= a.n[d];
+        for (int q = 0; q < NC; ++q) v[q] = p[q];
+    };
+    T acc[CPT][NC];
+    conv_run<T, NC, FAST, CPT>(a.w, r, load, acc);
+    if (!col_ok) return;
+    T* dst = a.dst + ((a.out_lo[0] + col - a.dst_lo[0]) * a.dst_stride[0] +
+                      (p1 - a.dst_lo[b1]) * a.dst_stride[b1] + (p2 - a.dst_lo[b2]) * a.dst_stride[b2]) * NC;
+    const int64_t oa0 = a.out_lo[ax] + ia * CPT - a.dst_lo[ax];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        if (ia * CPT + c < a.n[ax]) {
+            T* o = dst + (oa0 + c) * a.dst_stride[ax] * NC;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) o[q] = acc[c][q];
         }
-        c[ITKG_MAX_DIM - 1] = rem;
-        int64_t soff = 0, doff = 0, oax = 0;
+    }
+}
+
+// LDS index of staged input i (one pad word per 32).
+__device__ __forceinline__ int padded(int i) { return i + (i >> 5); }
+
+// Pass along axis 0.  Dynamic LDS: 4 waves x row_lds_elems(r) pixels.
+template <typename T, int NC, bool FAST, int CPT>
+__global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const AxisPassArgs<T> args) {
+    const AxisPassArgs<T>& a = *(const AxisPassArgs<T>*)__builtin_amdgcn_kernarg_segment_ptr();
+    extern __shared__ __attribute__((aligned(16))) unsigned char row_lds_raw[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int r = a.radius;
+    const int span = 64 * CPT + 2 * r;  // inputs of one segment
+    const int per_wave = padded(span - 1) + 1;
+    T* buf = reinterpret_cast<T*>(row_lds_raw) + (int64_t)wave * per_wave * NC;
+    const int64_t item = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t nseg = (a.n[0] + 64 * CPT - 1) / (64 * CPT);
+    int64_t it = item;
+    const int64_t is = it % nseg;
+    it /= nseg;
+    const int64_t o1 = it % a.n[1];
+    it /= a.n[1];
+    const int64_t o2 = it % a.n[2];
+    const int64_t o3 = it / a.n[2];
+    const bool live = o3 < a.n[3];  // wave-uniform; dead waves only skip memory
+    const int64_t q1 = a.out_lo[1] + o1, q2 = a.out_lo[2] + o2, q3 = a.out_lo[3] + o3;
+    const T* src = a.src + ((q1 - a.src_lo[1]) * a.src_stride[1] + (q2 - a.src_lo[2]) * a.src_stride[2] +
+                            (q3 - a.src_lo[3]) * a.src_stride[3] - a.src_lo[0] * a.src_stride[0]) * NC;
+    const int64_t first = a.out_lo[0] + is * 64 * CPT - r;  // axis-0 position of input 0
+    if (live) {
+        for (int i = lane; i < span; i += 64) {
+            int64_t pos = first + i;
+            pos = pos < a.clamp_lo ? a.clamp_lo : (pos > a.clamp_hi ? a.clamp_hi : pos);
+            const T* p = src + pos * a.src_stride[0] * NC;
 #pragma unroll
-        for (int d = 0; d < ITKG_MAX_DIM; ++d) {
-            const int64_t o = a.out_lo[d] + c[d];
-            doff += (o - a.dst_lo[d]) * a.dst_stride[d];
-            if (d == ax) {
-                oax = o;
-            } else {
-                soff += (o - a.src_lo[d]) * a.src_stride[d];
-            }
+            for (int q = 0; q < NC; ++q) buf[padded(i) * NC + q] = p[q];
         }
-        T acc[NC];
+    }
+    // The staged segment is private to this wave: order its LDS writes
+    // before the other lanes' reads.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!live) return;
+    const int base = lane * CPT;
+    auto load = [&](int k, T (&v)[NC]) {
+        const T* p = buf + padded(base + k) * NC;
 #pragma unroll
-        for (int q = 0; q < NC; ++q) acc[q] = T(0);
-        const int taps = 2 * a.radius + 1;
-        for (int k = 0; k < taps; ++k) {
-            int64_t v = oax + k - a.radius;
-            v = v < a.clamp_lo ? a.clamp_lo : v;
-            v = v > a.clamp_hi ? a.clamp_hi : v;
-            const T* p = a.src + (soff + (v - a.src_lo[ax]) * a.src_stride[ax]) * NC;
-            const T w = a.w[k];
+        for (int q = 0; q < NC; ++q) v[q] = p[q];
+    };
+    T acc[CPT][NC];
+    conv_run<T, NC, FAST, CPT>(a.w, r, load, acc);
+    const int64_t x0 = is * 64 * CPT + base;  // output index of acc[0] along axis 0
+    T* dst = a.dst + ((a.out_lo[0] + x0 - a.dst_lo[0]) * a.dst_stride[0] +
+                      (q1 - a.dst_lo[1]) * a.dst_stride[1] + (q2 - a.dst_lo[2]) * a.dst_stride[2] +
+                      (q3 - a.dst_lo[3]) * a.dst_stride[3]) * NC;
 #pragma unroll
-            for (int q = 0; q < NC; ++q) {
-                if constexpr (FAST) {
-                    acc[q] = __builtin_fma(p[q], w, acc[q]);
-                } else {
-                    const T prod = p[q] * w;
-                    acc[q] = acc[q] + prod;
-                }
-            }
+    for (int c = 0; c < CPT; ++c) {
+        if (x0 + c < a.n[0]) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) dst[c * a.dst_stride[0] * NC + q] = acc[c][q];
         }
-        T* out = a.dst + doff * NC;
-#pragma unroll
-        for (int q = 0; q < NC; ++q) out[q] = acc[q];
     }
 }
 
@@ -140,17 +281,55 @@ inline unsigned grid_for(int64_t total) {
     return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-template <typename T, bool FAST>
-hipError_t fwd_nc(const AxisPassArgs<T>& a, int ncomp, hipStream_t s) {
-    const dim3 grid(grid_for(a.total)), block(256);
-    switch (ncomp) {
-        case 1: hipLaunchKernelGGL((axis_forward_kernel<T, 1, FAST>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((axis_forward_kernel<T, 2, FAST>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((axis_forward_kernel<T, 3, FAST>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((axis_forward_kernel<T, 4, FAST>), grid, block, 0, s, a); break;
-        default: return hipErrorInvalidValue;
+// Outputs per lane of a run: the largest of 8, 4, 2, 1 that is <= 2r + 1.
+inline int run_cpt(int r) { return 2 * r + 1 >= 8 ? 8 : (2 * r + 1 >= 4 ? 4 : (2 * r + 1 >= 2 ? 2 : 1)); }
+
+// Dynamic LDS of the row kernel (4 waves).
+inline size_t row_lds_bytes(int cpt, int r, int nc, size_t elem) {
+    const int span = 64 * cpt + 2 * r;
+    return (size_t)4 * ((span - 1) + ((span - 1) >> 5) + 1) * nc * elem;
+}
+
+template <typename T, int NC, bool FAST, int CPT>
+hipError_t fwd_launch(const AxisPassArgs<T>& a, hipStream_t s) {
+    const int64_t other = a.total / a.n[a.axis];
+    int64_t items;
+    if (a.axis == 0) {
+        const int64_t seg = 64 * CPT;
+        items = ((a.n[0] + seg - 1) / seg) * other;
+        const size_t lds = row_lds_bytes(CPT, a.radius, NC, sizeof(T));
+        const dim3 grid((unsigned)((items + 3) / 4)), block(256);
+        hipLaunchKernelGGL((axis_row_kernel<T, NC, FAST, CPT>), grid, block, lds, s, a);
+    } else {
+        items = ((a.n[0] + 63) / 64) * ((a.n[a.axis] + CPT - 1) / CPT) * (other / a.n[0]);
+        const dim3 grid((unsigned)((items + 3) / 4)), block(256);
+        hipLaunchKernelGGL((axis_strided_kernel<T, NC, FAST, CPT>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+template <typename T, int NC, bool FAST>
+hipError_t fwd_cpt(const AxisPassArgs<T>& a, hipStream_t s) {
+    int cpt = run_cpt(a.radius);
+    // the row kernel's staging must fit the 64 KiB a workgroup gets by default
+    while (a.axis == 0 && cpt > 1 && row_lds_bytes(cpt, a.radius, NC, sizeof(T)) > 65536) cpt /= 2;
+    switch (cpt) {
+        case 8: return fwd_launch<T, NC, FAST, 8>(a, s);
+        case 4: return fwd_launch<T, NC, FAST, 4>(a, s);
+        case 2: return fwd_launch<T, NC, FAST, 2>(a, s);
+        default: return fwd_launch<T, NC, FAST, 1>(a, s);
+    }
+}
+
+template <typename T, bool FAST>
+hipError_t fwd_nc(const AxisPassArgs<T>& a, int ncomp, hipStream_t s) {
+    switch (ncomp) {
+        case 1: return fwd_cpt<T, 1, FAST>(a, s);
+        case 2: return fwd_cpt<T, 2, FAST>(a, s);
+        case 3: return fwd_cpt<T, 3, FAST>(a, s);
+        case 4: return fwd_cpt<T, 4, FAST>(a, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace

@@ -132,7 +132,7 @@ def c3(device, shape=(160, 256, 256)):
             sh = [1] * 3
             sh[ax] = len(kk)
             tr = tr * g.reshape(sh)
-        sm = np.fft.irfftn(f * tr, s=shape)
+        sm = np.fft.irfftn(f * tr, s=shape, axes=(0, 1, 2))
         sm *= 5.0 / np.abs(sm).max()
         comps.append(sm.astype(np.float32))
     v = torch.from_numpy(np.stack(comps, axis=-1)).to(device)

@@ -189,3 +189,61 @@ def test_cpp_host_pipeline():
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "all passed" in res.stdout
+
+
+PER_AXIS_CASES = [
+    # rows longer than one staged segment (64 * 8 outputs), large radius, clamp at both ends
+    ((1500,), [0], [1.0], 7.0, [100], [1200]),
+    ((1300,), [-40], [0.5], 2.0, [-40], [1300]),
+    # radius beyond the fused set along every axis of a 2-D and a 3-D image
+    ((70, 600), [0, 0], [1.0, 1.0], 9.0, [0, 0], [600, 70]),
+    ((40, 50, 30), [2, -3, 1], [1.0, 0.5, 2.0], 6.0, [3, -1, 4], [25, 44, 33]),
+    # anisotropic in-plane radii (no fused instantiation)
+    ((20, 33, 90), [0, 0, 0], [0.6, 1.3, 1.0], 2.2, [0, 0, 0], [90, 33, 20]),
+    # 4-D, largest radius along the slowest axis; tiny extents
+    ((9, 5, 6, 70), [0, 0, 0, 0], [1.0, 1.0, 1.0, 0.25], 2.0, [3, 0, 1, 2], [60, 6, 4, 7]),
+    ((2, 1, 3), [0, 0, 0], [1.0, 1.0, 1.0], 11.0, [0, 0, 0], [3, 1, 2]),
+]
+
+
+@pytest.mark.parametrize("case", PER_AXIS_CASES)
+@pytest.mark.parametrize("ncomp", [1, 2, 4])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_per_axis_runs_exact_and_fast(itk, orc, dev, case, ncomp, dtype):
+    """The per-axis kernels (strided-axis runs and staged rows) against the
+    oracle: EXACT bit-exact, FAST within tolerance."""
+    shape, ii, sp, sigma, ri, rs = case
+    rng = np.random.default_rng(len(shape) * 100 + ncomp)
+    arr = rng.random(shape + ((ncomp,) if ncomp > 1 else ())).astype(dtype)
+    ref = orc.generate_region(arr, ii, sp, sigma, ri, rs, ncomp)
+    itk.set_path(itk.PATH_PER_AXIS)
+    try:
+        for arith in (itk.EXACT, itk.FAST):
+            f = itk.GaussianFilter(_img(itk, arr, ii, sp, dev), sigma)
+            f.arith = arith
+            out = f.generate_region(itk.Region(ri, rs)).data.cpu().numpy()
+            if arith == itk.EXACT:
+                assert _bits_equal(out, ref)
+            else:
+                assert _rel_err(out, ref) <= (FAST_TOL if dtype == np.float32 else 1e-12)
+    finally:
+        itk.set_path(itk.PATH_AUTO)
+
+
+def test_per_axis_strided_output_view(itk, orc, dev):
+    """Output written through a window of a larger buffer on the per-axis path."""
+    rng = np.random.default_rng(9)
+    arr = rng.random((12, 20, 300)).astype(np.float32)
+    img = _img(itk, arr, [0, 0, 0], [1.0, 1.0, 1.0], dev)
+    f = itk.GaussianFilter(img, 6.0)        # radius 18: per-axis
+    req = itk.Region([10, 2, 1], [270, 15, 9])
+    big = torch.full((14, 19, 280), float("nan"), device=dev)
+    win = big[2:11, 1:16, 5:275]
+    out = itk.Image(req, (1.0,) * 3, (0.0,) * 3, win)
+    f.generate_into(img, out)
+    torch.cuda.synchronize()
+    ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, 6.0, [10, 2, 1], [270, 15, 9])
+    assert _bits_equal(win.cpu().numpy(), ref)
+    rest = big.clone()
+    rest[2:11, 1:16, 5:275] = 0.0
+    assert torch.isnan(rest).sum().item() == rest.numel() - win.numel()  # nothing else written

@@ -8,7 +8,7 @@
 // work list of (volume, tile) items, and runs as ONE persistent launch of the
 // fused kernel over that list.  Other volumes (anisotropic radii, radius
 // beyond the fused set) run the per-axis path with scratch preallocated at
-// plan time.  The groups and the per-axis remainder are dealt out to a few
+// plan time.  The groups and the per-axis volumes are dealt out to a few
 // branches (internal streams forked from and joined back into the caller's
 // stream), so that one group's tail overlaps the next group's start.
 // itkg_batch_run captures the whole forked sequence into a HIP graph on a
@@ -50,8 +50,10 @@ struct BatchClass {
     int grid = 0;
 };
 
-// Branches (side streams) a plan forks its launches into.
-constexpr int kBatchBranches = 4;
+// Branches (side streams) a plan forks its launches into.  A process gets
+// four hardware queues by default and the caller's stream holds one; a
+// fourth side stream would share a queue and run after the others.
+constexpr int kBatchBranches = 3;
 
 using TuneKey = std::tuple<int, int, int>;  // direction, arith, radius
 
@@ -86,7 +88,7 @@ struct BatchImpl {
     int mode = FUSED_FWD_EXACT;
     std::vector<BatchClass> classes;
     std::vector<int> rest;           // volumes on the per-axis path
-    void* scratch = nullptr;
+    void* scratch[kBatchBranches] = {};  // per-axis scratch of each branch
     int64_t voxels = 0;
     int num_cu = 256;
     std::mutex mu;
@@ -94,7 +96,8 @@ struct BatchImpl {
     hipStream_t side[kBatchBranches] = {};
     hipEvent_t fork_ev = nullptr;
     hipEvent_t join_ev[kBatchBranches] = {};
-    std::vector<std::vector<int>> branch_units;  // per branch: class index, or -1 = per-axis rest
+    // per branch: a group (index >= 0) or a per-axis volume v (encoded -1 - v)
+    std::vector<std::vector<int>> branch_units;
 
     ~BatchImpl() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
@@ -107,7 +110,8 @@ struct BatchImpl {
             if (c.d_args) (void)hipFree(c.d_args);
             if (c.d_work) (void)hipFree(c.d_work);
         }
-        if (scratch) (void)hipFree(scratch);
+        for (void* p : scratch)
+            if (p) (void)hipFree(p);
     }
 
     void drop_graphs() {
@@ -153,11 +157,8 @@ struct BatchImpl {
         std::vector<std::pair<double, int>> units;
         for (size_t i = 0; i < classes.size(); ++i)
             units.push_back({model_cost(classes[i], classes[i].tz), (int)i});
-        if (!rest.empty()) {
-            double w = 0.0;
-            for (int i : rest) w += (double)region_len(3, plans[i].req) * 3.0 / (64.0 * num_cu);
-            units.push_back({w, -1});
-        }
+        for (int i : rest)  // three passes over the volume, ~64 voxels per CU and step
+            units.push_back({(double)region_len(3, plans[i].req) * 3.0 / (64.0 * num_cu), -1 - i});
         std::sort(units.begin(), units.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
         const int nb = std::min<int>(kBatchBranches, (int)units.size());
         branch_units.assign(nb, {});
@@ -215,18 +216,13 @@ struct BatchImpl {
         return ITKG_OK;
     }
 
-    int enqueue_rest(hipStream_t s) const {
-        for (int i : rest) {
-            const itkg_volume& v = vols[i];
-            const Plan& p = plans[i];
-            const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
-            const itkg_view vout = dense_view(3, v.output, p.req);
-            const int rc = dir == ITKG_BATCH_FORWARD
-                               ? forward_impl(p, ITKG_F32, 1, vin, vout, arith, s, scratch)
-                               : backward_impl(p, ITKG_F32, 1, vin, vout, 0, s, scratch);
-            if (rc) return rc;
-        }
-        return ITKG_OK;
+    int enqueue_volume(int i, hipStream_t s, void* scr) const {
+        const itkg_volume& v = vols[i];
+        const Plan& p = plans[i];
+        const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
+        const itkg_view vout = dense_view(3, v.output, p.req);
+        return dir == ITKG_BATCH_FORWARD ? forward_impl(p, ITKG_F32, 1, vin, vout, arith, s, scr)
+                                         : backward_impl(p, ITKG_F32, 1, vin, vout, 0, s, scr);
     }
 
     // The whole plan on `s`: fork to the branches, run each branch's units in
@@ -238,7 +234,8 @@ struct BatchImpl {
             e = hipStreamWaitEvent(side[b], fork_ev, 0);
             if (e != hipSuccess) return hip_fail(e, "batch fork");
             for (int u : branch_units[b]) {
-                const int rc = u < 0 ? enqueue_rest(side[b]) : enqueue_class(classes[u], side[b]);
+                const int rc = u < 0 ? enqueue_volume(-1 - u, side[b], scratch[b])
+                                     : enqueue_class(classes[u], side[b]);
                 if (rc) return rc;
             }
             e = hipEventRecord(join_ev[b], side[b]);
@@ -317,8 +314,9 @@ int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_directio
             sc = std::max(sc, scratch_bytes(p, ITKG_F32, 1));
         }
     }
-    if (sc > 0) {
-        hipError_t e = hipMalloc(&B.scratch, sc);
+    // every branch may receive per-axis volumes (the autotuner re-deals them)
+    for (int k = 0; k < kBatchBranches && sc > 0 && !B.rest.empty(); ++k) {
+        hipError_t e = hipMalloc(&B.scratch[k], sc);
         if (e != hipSuccess) {
             delete b;
             return hip_fail(e, "hipMalloc(batch scratch)");

@@ -14,12 +14,11 @@
 //   - pass axis != 0 ("strided"): the 64 lanes take 64 consecutive pixels
 //     along axis 0 and each walks a run of CPT outputs along the pass axis,
 //     reading the CPT + 2r inputs of the run once (coalesced across lanes,
-//     clamped to the pass input extent) and adding each input into every
-//     output of the run that has it as a tap (a sliding window);
+//     clamped to the pass input extent);
 //   - pass axis 0 ("row"): the wave stages the 64 * CPT + 2r inputs of a row
-//     segment in LDS (coalesced), then each lane runs the same sliding window
-//     over CPT consecutive outputs, reading LDS through a padded index (one
-//     pad word per 32) that keeps the lanes' strided reads conflict-free.
+//     segment in LDS (coalesced), then each lane computes a run of CPT
+//     consecutive outputs, reading LDS through a padded layout (one pad
+//     pixel per CPT) that keeps the lanes' strided reads conflict-free.
 // Every output still receives its taps k = 0..2r in increasing order.
 #include "itkg_impl.hpp"
 
@@ -43,50 +42,51 @@ __device__ __forceinline__ void static_for(F&& f) {
 template <typename T, bool FAST>
 __device__ __forceinline__ T tap(T acc, T x, T w) {
     if constexpr (FAST) {
-        return __builtin_fma(x, w, acc);
+        if constexpr (std::is_same_v<T, float>)
+            return __builtin_fmaf(x, w, acc);  // single rounding in f32
+        else
+            return __builtin_fma(x, w, acc);
     } else {
         const T prod = x * w;
         return acc + prod;
     }
 }
 
-// Sliding window over a run of CPT outputs: input k (k = 0 .. CPT - 1 + 2r)
-// is tap k - c of output c.  Requires CPT <= 2r + 1, so that the middle
-// phase (every output of the run takes the input) separates the two ends.
+// A run of CPT consecutive outputs: output c takes input c + j as its tap j
+// (j = 0..2r, increasing, so each output's sum has the reference order).
+// The loop runs over taps; a ring of CPT registers holds inputs j .. j+CPT-1
+// (input i in slot i mod CPT), and a block of CPT taps is unrolled so every
+// slot index is a constant: per tap one weight (uniform), one input load
+// (the input CPT ahead, into the slot just consumed) and CPT multiply-adds.
+// load(jb, off, v) fetches input jb + off, jb a multiple of CPT (runtime),
+// off < 2 CPT (compile-time); inputs 0 .. 2r + CPT are fetched.
 template <typename T, int NC, bool FAST, int CPT, class Load>
 __device__ __forceinline__ void conv_run(const T* w, const int r, Load&& load, T (&acc)[CPT][NC]) {
-#pragma unroll
-    for (int c = 0; c < CPT; ++c)
+    const int taps = 2 * r + 1;
+    T ring[CPT][NC];
+    static_for<CPT>([&](auto c_) {
+        constexpr int c = decltype(c_)::value;
 #pragma unroll
         for (int q = 0; q < NC; ++q) acc[c][q] = T(0);
-    static_for<CPT - 1>([&](auto kk) {  // head: k < CPT - 1
-        constexpr int k = decltype(kk)::value;
-        T v[NC];
-        load(k, v);
-#pragma unroll
-        for (int c = 0; c <= k; ++c)
-#pragma unroll
-            for (int q = 0; q < NC; ++q) acc[c][q] = tap<T, FAST>(acc[c][q], v[q], w[k - c]);
+        load(0, std::integral_constant<int, c>{}, ring[c]);
     });
-#pragma unroll 2
-    for (int k = CPT - 1; k <= 2 * r; ++k) {  // middle: every output takes input k
-        T v[NC];
-        load(k, v);
+    auto block = [&](const int jb, const int limit) {
+        static_for<CPT>([&](auto jj_) {
+            constexpr int jj = decltype(jj_)::value;
+            if (jj < limit) {
+                const T wj = w[jb + jj];
 #pragma unroll
-        for (int c = 0; c < CPT; ++c)
+                for (int c = 0; c < CPT; ++c)
 #pragma unroll
-            for (int q = 0; q < NC; ++q) acc[c][q] = tap<T, FAST>(acc[c][q], v[q], w[k - c]);
-    }
-    static_for<CPT - 1>([&](auto dd) {  // tail: k = 2r + d, outputs c >= d
-        constexpr int d = decltype(dd)::value + 1;
-        const int k = 2 * r + d;
-        T v[NC];
-        load(k, v);
-#pragma unroll
-        for (int c = d; c < CPT; ++c)
-#pragma unroll
-            for (int q = 0; q < NC; ++q) acc[c][q] = tap<T, FAST>(acc[c][q], v[q], w[k - c]);
-    });
+                    for (int q = 0; q < NC; ++q)
+                        acc[c][q] = tap<T, FAST>(acc[c][q], ring[(jj + c) % CPT][q], wj);
+                load(jb, std::integral_constant<int, jj + CPT>{}, ring[jj]);
+            }
+        });
+    };
+    int jb = 0;
+    for (; jb + CPT <= taps; jb += CPT) block(jb, CPT);
+    if (jb < taps) block(jb, taps - jb);
 }
 
 // Pass along a strided axis (a.axis >= 1).
@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
     const AxisPassArgs<T>& a = *(const AxisPassArgs<T>*)__builtin_amdgcn_kernarg_segment_ptr();
     const int ax = a.axis;
     const int lane = threadIdx.x & 63;
-    const int64_t item = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t item = (int64_t)blockIdx.x * 4 + wave;  // wave-uniform from here on
     // the two axes that are neither 0 nor the pass axis
     const int b1 = ax == 1 ? 2 : 1;
     const int b2 = ax == 3 ? 2 : 3;
@@ -115,15 +116,17 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
     const bool col_ok = col < a.n[0];
     const int64_t colc = col_ok ? col : a.n[0] - 1;
     const int64_t p0 = a.out_lo[0] + colc, p1 = a.out_lo[b1] + o1, p2 = a.out_lo[b2] + o2;
-    const T* src = a.src + ((p0 - a.src_lo[0]) * a.src_stride[0] + (p1 - a.src_lo[b1]) * a.src_stride[b1] +
-                            (p2 - a.src_lo[b2]) * a.src_stride[b2] - a.src_lo[ax] * a.src_stride[ax]) * NC;
-    const int64_t sa = a.src_stride[ax] * NC;
     const int r = a.radius;
-    const int64_t first = a.out_lo[ax] + ia * CPT - r;  // pass-axis position of input k = 0
-    auto load = [&](int k, T (&v)[NC]) {
-        int64_t pos = first + k;
-        pos = pos < a.clamp_lo ? a.clamp_lo : (pos > a.clamp_hi ? a.clamp_hi : pos);
-        const T* p = src + pos * sa;
+    const int64_t first = a.out_lo[ax] + ia * CPT - r;  // pass-axis position of input 0
+    // clamp bounds relative to input 0 (the extent along one axis fits int)
+    const int lo_k = (int)(a.clamp_lo - first), hi_k = (int)(a.clamp_hi - first);
+    const int64_t sa = a.src_stride[ax] * NC;
+    const T* src = a.src + ((p0 - a.src_lo[0]) * a.src_stride[0] + (p1 - a.src_lo[b1]) * a.src_stride[b1] +
+                            (p2 - a.src_lo[b2]) * a.src_stride[b2] + (first - a.src_lo[ax]) * a.src_stride[ax]) * NC;
+    auto load = [&](int jb, auto off, T (&v)[NC]) {
+        int i = jb + decltype(off)::value;
+        i = i < lo_k ? lo_k : (i > hi_k ? hi_k : i);
+        const T* p = src + (int64_t)i * sa;
 #pragma unroll
         for (int q = 0; q < NC; ++q) v[q] = p[q];
     };
@@ -133,29 +136,34 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
     T* dst = a.dst + ((a.out_lo[0] + col - a.dst_lo[0]) * a.dst_stride[0] +
                       (p1 - a.dst_lo[b1]) * a.dst_stride[b1] + (p2 - a.dst_lo[b2]) * a.dst_stride[b2]) * NC;
     const int64_t oa0 = a.out_lo[ax] + ia * CPT - a.dst_lo[ax];
+    const int64_t da = a.dst_stride[ax] * NC;
+    const int nval = (int)(a.n[ax] - ia * CPT);  // outputs of this run inside the region
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-        if (ia * CPT + c < a.n[ax]) {
-            T* o = dst + (oa0 + c) * a.dst_stride[ax] * NC;
+        if (c < nval) {
+            T* o = dst + (oa0 + c) * da;
 #pragma unroll
             for (int q = 0; q < NC; ++q) o[q] = acc[c][q];
         }
     }
 }
 
-// LDS index of staged input i (one pad word per 32).
-__device__ __forceinline__ int padded(int i) { return i + (i >> 5); }
+// Staged row layout: runs of CPT inputs, each followed by one pad pixel, so
+// that lane l (reading input l * CPT + k) hits bank l * (CPT + 1) + const.
+template <int CPT>
+__device__ __forceinline__ int staged(int i) { return (i % CPT) + (i / CPT) * (CPT + 1); }
 
-// Pass along axis 0.  Dynamic LDS: 4 waves x row_lds_elems(r) pixels.
+// Pass along axis 0.  Dynamic LDS: 4 waves x row_lds_bytes() / 4.
 template <typename T, int NC, bool FAST, int CPT>
 __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const AxisPassArgs<T> args) {
     const AxisPassArgs<T>& a = *(const AxisPassArgs<T>*)__builtin_amdgcn_kernarg_segment_ptr();
     extern __shared__ __attribute__((aligned(16))) unsigned char row_lds_raw[];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int r = a.radius;
-    const int span = 64 * CPT + 2 * r;  // inputs of one segment
-    const int per_wave = padded(span - 1) + 1;
+    // inputs of one segment, plus the ring's read-ahead past the last tap
+    const int span = 64 * CPT + 2 * r + CPT;
+    const int per_wave = ((span - 1) / CPT + 1) * (CPT + 1);
     T* buf = reinterpret_cast<T*>(row_lds_raw) + (int64_t)wave * per_wave * NC;
     const int64_t item = (int64_t)blockIdx.x * 4 + wave;
     const int64_t nseg = (a.n[0] + 64 * CPT - 1) / (64 * CPT);
@@ -168,16 +176,18 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     const int64_t o3 = it / a.n[2];
     const bool live = o3 < a.n[3];  // wave-uniform; dead waves only skip memory
     const int64_t q1 = a.out_lo[1] + o1, q2 = a.out_lo[2] + o2, q3 = a.out_lo[3] + o3;
-    const T* src = a.src + ((q1 - a.src_lo[1]) * a.src_stride[1] + (q2 - a.src_lo[2]) * a.src_stride[2] +
-                            (q3 - a.src_lo[3]) * a.src_stride[3] - a.src_lo[0] * a.src_stride[0]) * NC;
     const int64_t first = a.out_lo[0] + is * 64 * CPT - r;  // axis-0 position of input 0
+    const int lo_k = (int)(a.clamp_lo - first), hi_k = (int)(a.clamp_hi - first);
+    const int64_t s0 = a.src_stride[0] * NC;
+    const T* src = a.src + ((q1 - a.src_lo[1]) * a.src_stride[1] + (q2 - a.src_lo[2]) * a.src_stride[2] +
+                            (q3 - a.src_lo[3]) * a.src_stride[3] + (first - a.src_lo[0]) * a.src_stride[0]) * NC;
     if (live) {
         for (int i = lane; i < span; i += 64) {
-            int64_t pos = first + i;
-            pos = pos < a.clamp_lo ? a.clamp_lo : (pos > a.clamp_hi ? a.clamp_hi : pos);
-            const T* p = src + pos * a.src_stride[0] * NC;
+            const int ic = i < lo_k ? lo_k : (i > hi_k ? hi_k : i);
+            const T* p = src + (int64_t)ic * s0;
+            T* d = buf + staged<CPT>(i) * NC;
 #pragma unroll
-            for (int q = 0; q < NC; ++q) buf[padded(i) * NC + q] = p[q];
+            for (int q = 0; q < NC; ++q) d[q] = p[q];
         }
     }
     // The staged segment is private to this wave: order its LDS writes
@@ -187,8 +197,11 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (!live) return;
     const int base = lane * CPT;
-    auto load = [&](int k, T (&v)[NC]) {
-        const T* p = buf + padded(base + k) * NC;
+    const T* lane_buf = buf + lane * (CPT + 1) * NC;
+    auto load = [&](int jb, auto off, T (&v)[NC]) {
+        constexpr int o = decltype(off)::value;
+        // input base + jb + o, jb a multiple of CPT
+        const T* p = lane_buf + ((jb / CPT + o / CPT) * (CPT + 1) + o % CPT) * NC;
 #pragma unroll
         for (int q = 0; q < NC; ++q) v[q] = p[q];
     };
@@ -281,13 +294,10 @@ inline unsigned grid_for(int64_t total) {
     return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-// Outputs per lane of a run: the largest of 8, 4, 2, 1 that is <= 2r + 1.
-inline int run_cpt(int r) { return 2 * r + 1 >= 8 ? 8 : (2 * r + 1 >= 4 ? 4 : (2 * r + 1 >= 2 ? 2 : 1)); }
-
 // Dynamic LDS of the row kernel (4 waves).
 inline size_t row_lds_bytes(int cpt, int r, int nc, size_t elem) {
-    const int span = 64 * cpt + 2 * r;
-    return (size_t)4 * ((span - 1) + ((span - 1) >> 5) + 1) * nc * elem;
+    const int span = 64 * cpt + 2 * r + cpt;
+    return (size_t)4 * ((span - 1) / cpt + 1) * (cpt + 1) * nc * elem;
 }
 
 template <typename T, int NC, bool FAST, int CPT>
@@ -310,7 +320,7 @@ hipError_t fwd_launch(const AxisPassArgs<T>& a, hipStream_t s) {
 
 template <typename T, int NC, bool FAST>
 hipError_t fwd_cpt(const AxisPassArgs<T>& a, hipStream_t s) {
-    int cpt = run_cpt(a.radius);
+    int cpt = 8;  // outputs per lane of a run
     // the row kernel's staging must fit the 64 KiB a workgroup gets by default
     while (a.axis == 0 && cpt > 1 && row_lds_bytes(cpt, a.radius, NC, sizeof(T)) > 65536) cpt /= 2;
     switch (cpt) {

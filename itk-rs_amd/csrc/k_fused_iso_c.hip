@@ -5,8 +5,7 @@ namespace itkg {
 // Host-side table (the device pass compiles no host function bodies).
 const FusedEntry* fused_table_iso_c(int* count) {
     static const FusedEntry table[] = {make_entry<12, 12>(), make_entry<13, 13>(),
-                                        make_entry<14, 14>(), make_entry<15, 15>(),
-                                        make_entry<16, 16>()};
+                                        make_entry<14, 14>(), make_entry<15, 15>()};
     *count = (int)(sizeof(table) / sizeof(table[0]));
     return table;
 }

@@ -68,10 +68,10 @@ def test_exact_matches_oracle(itk, orc, dev, case, ncomp, dtype, path):
     assert _bits_equal(out.data.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("r", list(range(1, 17)))
+@pytest.mark.parametrize("r", list(range(1, 16)))
 @pytest.mark.parametrize("arith", ["exact", "fast"])
 def test_fused_radii_match_per_axis(itk, orc, dev, r, arith):
-    """Every fused instantiation (isotropic radius 1..16) against the per-axis
+    """Every fused 3-D instantiation (isotropic radius 1..15) against the per-axis
     kernel on a volume with ragged tile edges and a sub-region request."""
     sigma = r / 3.0 - 1e-3
     assert itk.kernel_radius(sigma, 1.0) == r

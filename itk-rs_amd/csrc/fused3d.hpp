@@ -58,10 +58,13 @@ __device__ __forceinline__ void lds_barrier() {
 // CPT * (2rz+1) accumulators per lane; the y tile is TY = NW * CPT rows.
 template <int RXY, int RZ>
 struct Cfg {
-    static constexpr int NW = RZ <= 3 ? 4 : 8;
+#ifndef ITKG_BIG_NW
+#define ITKG_BIG_NW 16
+#endif
+    static constexpr int NW = RZ <= 3 ? 4 : (RZ <= 7 ? 8 : ITKG_BIG_NW);
     static constexpr int NTH = 64 * NW;
     static constexpr int TX = 64;
-    static constexpr int CPT = RZ <= 3 ? 8 : (RZ <= 10 ? 4 : 2);
+    static constexpr int CPT = RZ <= 3 ? 8 : (RZ <= 7 ? 4 : (NW == 16 ? 2 : (RZ <= 10 ? 4 : 2)));
     static constexpr int TY = NW * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows

@@ -128,7 +128,7 @@ struct BatchImpl {
         for (int i : c.vol) {
             const itkg_volume& v = vols[i];
             const int64_t txy = ((v.size[0] + kFusedTileX - 1) / kFusedTileX) *
-                                ((v.size[1] + c.entry->ty - 1) / c.entry->ty);
+                                ((v.size[1] + c.entry->ty[mode] - 1) / c.entry->ty[mode]);
             const int64_t nz = v.size[2];
             const int64_t tzv = std::min<int64_t>(tz, nz);
             const double walk = (double)(tzv + 2 * c.r + 3);

@@ -332,7 +332,7 @@ void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& 
     }
     A.accumulate = accumulate;
     const FusedEntry* fe = fused_for(p);
-    const int TY = fe ? fe->ty : 32;
+    const int TY = fe ? fe->ty[mode] : 32;
     A.tiles_x = (A.n_dst[0] + kFusedTileX - 1) / kFusedTileX;
     A.tiles_y = (A.n_dst[1] + TY - 1) / TY;
     A.tz = tz > 0 ? tz : fused_tz(fe, mode, A.tiles_x * A.tiles_y, A.n_dst[2], rz);

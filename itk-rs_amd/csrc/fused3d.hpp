@@ -54,17 +54,23 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Waves and output columns per lane, by z radius: the z ring holds
+// Waves and output columns per lane, by z radius and mode: the z ring holds
 // CPT * (2rz+1) accumulators per lane; the y tile is TY = NW * CPT rows.
-template <int RXY, int RZ>
+//   rz <= 3: 4 waves x 8 columns;
+//   rz 4..7: FAST 16 waves x 4 columns (64-row tiles: less y halo), EXACT
+//            and the adjoint 8 x 4 (their larger register need would spill
+//            under the 128-VGPR cap of a 16-wave workgroup);
+//   rz >= 8: 16 waves x 2 columns, except EXACT at rz 9..14, which spills
+//            there and runs 8 waves x 4 (rz <= 10) or x 2 columns.
+template <int RXY, int RZ, int MODE>
 struct Cfg {
-#ifndef ITKG_BIG_NW
-#define ITKG_BIG_NW 16
-#endif
-    static constexpr int NW = RZ <= 3 ? 4 : (RZ <= 7 ? 8 : ITKG_BIG_NW);
+    static constexpr bool EXACT_MID = MODE == FUSED_FWD_EXACT && RZ >= 9 && RZ <= 14;
+    static constexpr int NW =
+        RZ <= 3 ? 4 : (RZ <= 7 ? (MODE == FUSED_FWD_FAST ? 16 : 8) : (EXACT_MID ? 8 : 16));
     static constexpr int NTH = 64 * NW;
     static constexpr int TX = 64;
-    static constexpr int CPT = RZ <= 3 ? 8 : (RZ <= 7 ? 4 : (NW == 16 ? 2 : (RZ <= 10 ? 4 : 2)));
+    static constexpr int CPT =
+        RZ <= 3 ? 8 : (RZ <= 7 ? 4 : (EXACT_MID && RZ <= 10 ? 4 : 2));
     static constexpr int TY = NW * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows
@@ -145,7 +151,7 @@ __device__ __forceinline__ float tap_sum(const float* v, const float* w) {
 template <int RXY, int RZ, int MODE>
 __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int tile, float* sIn,
                                              float* sX) {
-    using C = Cfg<RXY, RZ>;
+    using C = Cfg<RXY, RZ, MODE>;
     constexpr int CPT = C::CPT, P = C::P, LJ = C::LJ;
     constexpr bool ADJOINT = MODE == FUSED_ADJ;
     const int tx_i = tile % a.tiles_x;
@@ -369,7 +375,10 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                     if constexpr (ADJOINT) {
                         if (a.accumulate) q = dst[c * dys] + q;
                     }
-                    dst[c * dys] = q;
+                    if constexpr (ADJOINT)
+                        dst[c * dys] = q;
+                    else  // streamed output: keep L2 for the input halos
+                        __builtin_nontemporal_store(q, &dst[c * dys]);
                 }
             }
         }
@@ -410,9 +419,9 @@ struct WavesPerEu {
 };
 
 template <int RXY, int RZ, int MODE>
-__global__ __launch_bounds__((Cfg<RXY, RZ>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
+__global__ __launch_bounds__((Cfg<RXY, RZ, MODE>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
 void fused3d_kernel([[maybe_unused]] const Fused3dArgs args) {
-    using C = Cfg<RXY, RZ>;
+    using C = Cfg<RXY, RZ, MODE>;
     // The argument block is read in place from the kernel-argument segment
     // (where `args` lies, at offset 0), so that runtime-indexed weights are
     // scalar loads and the block is never copied to private memory.
@@ -432,10 +441,10 @@ void fused3d_kernel([[maybe_unused]] const Fused3dArgs args) {
 // volumes of one radius: workgroup b takes items b, b + G, b + 2G, ...
 // The host interleaves the list so that items b and b + G share an XCD.
 template <int RXY, int RZ, int MODE>
-__global__ __launch_bounds__((Cfg<RXY, RZ>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
+__global__ __launch_bounds__((Cfg<RXY, RZ, MODE>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
 void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __restrict__ work,
                           const int nwork) {
-    using C = Cfg<RXY, RZ>;
+    using C = Cfg<RXY, RZ, MODE>;
     __shared__ __attribute__((aligned(16))) float sIn[2 * C::PLANE];
     __shared__ __attribute__((aligned(16))) float sX[2 * C::XPLANE];
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -448,7 +457,10 @@ void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __re
 template <int RXY, int RZ>
 hipError_t launch_r(const Fused3dArgs& a, int mode, hipStream_t st) {
     const int nt = a.tiles_x * a.tiles_y * a.tiles_z;
-    const dim3 grid(((nt + 7) / 8) * 8), block(Cfg<RXY, RZ>::NTH);
+    const dim3 grid(((nt + 7) / 8) * 8);
+    const dim3 block(mode == FUSED_FWD_EXACT  ? Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH
+                     : mode == FUSED_FWD_FAST ? Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH
+                                              : Cfg<RXY, RZ, FUSED_ADJ>::NTH);
     switch (mode) {
         case FUSED_FWD_EXACT:
             hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT>), grid, block, 0, st, a);
@@ -468,7 +480,10 @@ hipError_t launch_r(const Fused3dArgs& a, int mode, hipStream_t st) {
 template <int RXY, int RZ>
 hipError_t launch_batch_r(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                           int mode, hipStream_t st) {
-    const dim3 grid(grid_blocks), block(Cfg<RXY, RZ>::NTH);
+    const dim3 grid(grid_blocks);
+    const dim3 block(mode == FUSED_FWD_EXACT  ? Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH
+                     : mode == FUSED_FWD_FAST ? Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH
+                                              : Cfg<RXY, RZ, FUSED_ADJ>::NTH);
     switch (mode) {
         case FUSED_FWD_EXACT:
             hipLaunchKernelGGL((fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>), grid, block, 0, st,
@@ -494,31 +509,38 @@ template <int RXY, int RZ, bool PERSIST>
 int occupancy_r(int mode) {
     int blocks = 0;
     hipError_t e = hipErrorInvalidValue;
-    const int nth = Cfg<RXY, RZ>::NTH;
-    auto query = [&](auto kern) {
+    auto query = [&](auto kern, int nth) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, nth, 0);
     };
+    constexpr int n0 = Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH, n1 = Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH,
+                  n2 = Cfg<RXY, RZ, FUSED_ADJ>::NTH;
     if (mode == FUSED_FWD_EXACT)
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT>);
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>, n0)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT>, n0);
     else if (mode == FUSED_FWD_FAST)
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST>)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_FAST>);
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST>, n1)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_FAST>, n1);
     else
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_ADJ>);
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>, n2)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_ADJ>, n2);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
 // Registry entry of one (rxy, rz) instantiation.
 template <int RXY, int RZ>
 constexpr FusedEntry make_entry() {
-    return FusedEntry{RXY, RZ, Cfg<RXY, RZ>::TY, &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>,
+    return FusedEntry{RXY, RZ,
+                      {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
+                       Cfg<RXY, RZ, FUSED_ADJ>::TY},
+                      &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>,
                       &occupancy_r<RXY, RZ, true>, &occupancy_r<RXY, RZ, false>};
 }
 template <int RXY, int RZ>
 constexpr FusedEntry make_entry_single() {
-    return FusedEntry{RXY, RZ, Cfg<RXY, RZ>::TY, &launch_r<RXY, RZ>, nullptr, nullptr,
+    return FusedEntry{RXY, RZ,
+                      {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
+                       Cfg<RXY, RZ, FUSED_ADJ>::TY},
+                      &launch_r<RXY, RZ>, nullptr, nullptr,
                       &occupancy_r<RXY, RZ, false>};
 }
 

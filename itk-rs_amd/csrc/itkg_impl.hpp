@@ -73,7 +73,8 @@ struct Fused3dArgs {
 // rz (0 for 2-D images), its y tile height, and its launchers.  `batch` and
 // `occupancy` are null for instantiations without a persistent form.
 struct FusedEntry {
-    int rxy, rz, ty;
+    int rxy, rz;
+    int ty[3];  // tile rows per mode (FusedMode index)
     hipError_t (*launch)(const Fused3dArgs& a, int mode, hipStream_t s);
     hipError_t (*batch)(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                         int mode, hipStream_t s);

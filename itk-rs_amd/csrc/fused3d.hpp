@@ -56,21 +56,18 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Waves and output columns per lane, by z radius and mode: the z ring holds
 // CPT * (2rz+1) accumulators per lane; the y tile is TY = NW * CPT rows.
-//   rz <= 3: 4 waves x 8 columns;
-//   rz 4..7: FAST 16 waves x 4 columns (64-row tiles: less y halo), EXACT
-//            and the adjoint 8 x 4 (their larger register need would spill
-//            under the 128-VGPR cap of a 16-wave workgroup);
-//   rz >= 8: 16 waves x 2 columns, except EXACT at rz 9..14, which spills
-//            there and runs 8 waves x 4 (rz <= 10) or x 2 columns.
+//   FAST and adjoint: 16 waves; 4 columns (64-row tiles) up to rz 7, then 2;
+//   EXACT (a larger register need: the product and the sum round
+//   separately): 8 waves x 4 columns up to rz 7; 16 x 2 at rz 8 and 15;
+//   8 x 4 (rz 9, 10) or 8 x 2 (rz 11..14), where 16 waves would spill.
 template <int RXY, int RZ, int MODE>
 struct Cfg {
     static constexpr bool EXACT_MID = MODE == FUSED_FWD_EXACT && RZ >= 9 && RZ <= 14;
     static constexpr int NW =
-        RZ <= 3 ? 4 : (RZ <= 7 ? (MODE == FUSED_FWD_FAST ? 16 : 8) : (EXACT_MID ? 8 : 16));
+        MODE != FUSED_FWD_EXACT ? 16 : (RZ <= 7 ? 8 : (EXACT_MID ? 8 : 16));
     static constexpr int NTH = 64 * NW;
     static constexpr int TX = 64;
-    static constexpr int CPT =
-        RZ <= 3 ? 8 : (RZ <= 7 ? 4 : (EXACT_MID && RZ <= 10 ? 4 : 2));
+    static constexpr int CPT = RZ <= 7 ? 4 : (EXACT_MID && RZ <= 10 ? 4 : 2);
     static constexpr int TY = NW * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows
@@ -245,20 +242,23 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                 for (int j = 0; j < Q; ++j) o[j] = tap_sum<RXY, MODE>(v + j, wx);
                 if constexpr (ADJOINT) {
                     const int xg = x0 + Q * xq;  // destination x of o[0]
-                    if (x_edge) {
+                    const int jlo = -xg, jhi = ndx - 1 - xg;  // output index of x = 0 / ndx - 1
+                    // one test per lane group; only the (at most two) lanes
+                    // holding an edge output enter
+                    if (x_edge && ((unsigned)jlo < (unsigned)Q || (unsigned)jhi < (unsigned)Q)) {
 #pragma unroll
-                    for (int j = 0; j < Q; ++j) {
-                        if (xg + j == 0) {
+                        for (int j = 0; j < Q; ++j) {
+                            if (j == jlo) {
 #pragma unroll
-                            for (int m = 1; m <= RXY; ++m)
-                                o[j] = __builtin_fmaf(a.px[m], v[j + 2 * RXY - m], o[j]);
+                                for (int m = 1; m <= RXY; ++m)
+                                    o[j] = __builtin_fmaf(a.px[m], v[j + 2 * RXY - m], o[j]);
+                            }
+                            if (j == jhi) {
+#pragma unroll
+                                for (int m = 1; m <= RXY; ++m)
+                                    o[j] = __builtin_fmaf(a.px[m], v[j + m], o[j]);
+                            }
                         }
-                        if (xg + j == ndx - 1) {
-#pragma unroll
-                            for (int m = 1; m <= RXY; ++m)
-                                o[j] = __builtin_fmaf(a.px[m], v[j + m], o[j]);
-                        }
-                    }
                     }
                 }
                 vecq* dq = reinterpret_cast<vecq*>(&dstp[row * C::SX + Q * xq]);
@@ -306,14 +306,14 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
 #pragma unroll
     for (int p = 0; p < P; ++p)
         r0[p] = r1[p] = r2[p] = r3[p] = r4[p] = r5[p] = r6[p] = r7[p] = 0.0f;
-    float tlo[CPT], thi[CPT];  // adjoint: z-edge sums of the first / last plane
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) tlo[c] = thi[c] = 0.0f;
-
     const int tz_act = min(a.tz, ndz - z0);
     const int U = tz_act + 2 * RZ;
     const int t_hi = ndz - 1 - z0;  // step-relative index of the last destination plane
     const bool z_edge = z0 == 0 || z0 + tz_act >= ndz;
+    // Adjoint z edges: the folded tail terms go straight into the ring slot
+    // of the edge output (output t lives in slot t mod P from step t to
+    // step t + 2r): slot 0 for the first plane, slot t_hi mod P for the last.
+    const int s_hi = ((t_hi % P) + P) % P;
 
     // Step u, entered with plane u x-filtered in sX half u % 2, plane u + 1
     // in sIn half (u + 1) % 2 and plane u + 2 in register set `r`:
@@ -355,15 +355,13 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                     acc[j] = __builtin_fmaf(val[c], w[-j], acc[j]);
                 }
             }
-            o[c] = acc[sn];  // tap 2r has just been added
             if constexpr (ADJOINT) {
-                if (z_edge) {
-                    tlo[c] = __builtin_fmaf(val[c], plo, tlo[c]);
-                    thi[c] = __builtin_fmaf(val[c], phi, thi[c]);
-                    if (z0 + t == 0) o[c] = o[c] + tlo[c];
-                    if (t == t_hi) o[c] = o[c] + thi[c];
+                if (z_edge) {  // plo / phi are uniform: only the steps that carry a term
+                    if (plo != 0.0f) acc[0] = __builtin_fmaf(val[c], plo, acc[0]);
+                    if (phi != 0.0f) acc[s_hi] = __builtin_fmaf(val[c], phi, acc[s_hi]);
                 }
             }
+            o[c] = acc[sn];  // tap 2r has just been added
         });
         s = sn;
         if (t >= 0 && t < tz_act) {
@@ -371,12 +369,9 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
                 if (x_ok && ybase + c < ndy) {
-                    float q = o[c];
-                    if constexpr (ADJOINT) {
-                        if (a.accumulate) q = dst[c * dys] + q;
-                    }
-                    if constexpr (ADJOINT)
-                        dst[c * dys] = q;
+                    const float q = o[c];
+                    if (ADJOINT && a.accumulate)
+                        dst[c * dys] = dst[c * dys] + q;
                     else  // streamed output: keep L2 for the input halos
                         __builtin_nontemporal_store(q, &dst[c * dys]);
                 }

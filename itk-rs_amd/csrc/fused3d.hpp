@@ -271,6 +271,8 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
 
     // ---- y pass: CPT outputs down column `lane`, rows wave*CPT ...
     const int ybase = y0 + wave * CPT;
+    // wave-uniform: this wave's rows hold the first or the last output row
+    [[maybe_unused]] const bool y_edge = ybase <= 0 || ybase + CPT > ndy - 1;
     float val[CPT];
     auto y_pass = [&](const float* xp) {
         float v[CPT + 2 * RXY];
@@ -280,12 +282,12 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         for (int c = 0; c < CPT; ++c) {
             val[c] = tap_sum<RXY, MODE>(v + c, wy);
             if constexpr (ADJOINT) {
-                if (ybase + c == 0) {
+                if (y_edge && ybase + c == 0) {
 #pragma unroll
                     for (int m = 1; m <= RXY; ++m)
                         val[c] = __builtin_fmaf(a.py[m], v[c + 2 * RXY - m], val[c]);
                 }
-                if (ybase + c == ndy - 1) {
+                if (y_edge && ybase + c == ndy - 1) {
 #pragma unroll
                     for (int m = 1; m <= RXY; ++m)
                         val[c] = __builtin_fmaf(a.py[m], v[c + m], val[c]);
@@ -329,17 +331,13 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         y_pass(sX + h * C::XPLANE);
         const int sn = (s + 1 == P) ? 0 : s + 1;
         const float* w = a.wz2 + s + P;  // w[-j] = wz[(s - j) mod P]
-        // Adjoint z-edge weights of this plane (uniform; chosen by compare so
-        // that the argument block is never indexed with a runtime value).
+        // Adjoint z-edge weights of this plane (uniform scalar loads; zero
+        // outside the first plane's and the last plane's windows).
         float plo = 0.0f, phi = 0.0f;
         if (ADJOINT && z_edge) {
-            const int mlo = (z0 == 0) ? 2 * RZ - u : -1;
-            const int mhi = u - t_hi;
-#pragma unroll
-            for (int m = 1; m <= RZ; ++m) {
-                plo = (mlo == m) ? a.pz[m] : plo;
-                phi = (mhi == m) ? a.pz[m] : phi;
-            }
+            const int mlo = 2 * RZ - u, mhi = u - t_hi;
+            if (z0 == 0 && mlo >= 1 && mlo <= RZ) plo = a.pz[mlo];
+            if (mhi >= 1 && mhi <= RZ) phi = a.pz[mhi];
         }
         float o[CPT];
         static_for<CPT>([&](auto ci) {

@@ -13,8 +13,9 @@ reference, the accuracy the target is stated at).  Also reported (under
   - the max error of both modes against an f64 evaluation of the same
     operator (and of FAST against EXACT), relative to the max |output|,
   - forward + adjoint at 256^3 (sigma 1.8) vs forward only,
-  - the batched form (64 uneven volumes, one captured launch) vs a loop of
-    plain single-volume launches,
+  - the batched form (64 uneven volumes, persistent launches per radius
+    group in one captured graph, z runs picked by the autotuner) vs a loop
+    of plain single-volume launches,
   - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration.
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
@@ -199,6 +200,7 @@ def main():
         plan = itk.BatchPlan(vols, outs, [p[1] for p in params], [p[2] for p in params],
                              direction=itk.BATCH_FORWARD, arith=arith)
         nb = plan.voxels
+        plan.autotune(stream)  # z run per radius group, measured on these volumes
         wb, _ = timed_loop(lambda: plan.run(stream), 5, 2, stream)
         imgs = [itk.Image(itk.Region([0, 0, 0], list(reversed(v.shape))), p[2], (0.0,) * 3, v)
                 for v, p in zip(vols, params)]

@@ -48,6 +48,7 @@ struct BatchClass {
     int2* d_work = nullptr;          // (index into d_args, tile), XCD-interleaved
     int nwork = 0;
     int grid = 0;
+    float measured_ms = -1.0f;  // autotuner's time of the chosen z run (< 0: not timed)
 };
 
 // Branches (side streams) a plan forks its launches into.  A process gets
@@ -98,6 +99,7 @@ struct BatchImpl {
     hipEvent_t join_ev[kBatchBranches] = {};
     // per branch: a group (index >= 0) or a per-axis volume v (encoded -1 - v)
     std::vector<std::vector<int>> branch_units;
+    std::vector<float> rest_ms;  // autotuner's time per per-axis volume (empty: not timed)
 
     ~BatchImpl() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
@@ -151,14 +153,46 @@ struct BatchImpl {
         return best;
     }
 
-    // Deals the groups (largest modelled time first) and the per-axis
-    // remainder out to the branches, each to the least loaded one.
+    // Relative GPU time of one output voxel, fitted to the measured plain
+    // launches on one 512^3 volume: the fused kernel costs about the same up
+    // to radius 6 (memory and synchronisation bound) and grows with the taps
+    // beyond; the per-axis path makes three passes through HBM.
+    static double voxel_cost(int r, bool fused) {
+        if (!fused) return 3.0;
+        return std::max(1.0, r / 6.0);
+    }
+
+    // Estimated time of a group: its voxels at the radius's cost, times the
+    // z-ring fill of each volume's tiles (tz + 2r + 3 steps for tz planes).
+    double class_work(const BatchClass& c) const {
+        double w = 0.0;
+        for (int i : c.vol) {
+            const itkg_volume& v = vols[i];
+            const double nz = (double)v.size[2];
+            const double tzv = std::min<double>(c.tz, nz);
+            w += (double)v.size[0] * v.size[1] * nz * voxel_cost(c.r, true) *
+                 (tzv + 2 * c.r + 3) / tzv;
+        }
+        return w;
+    }
+
+    // Deals the groups and the per-axis volumes (largest estimated time
+    // first) out to the branches, each to the least loaded one.
+    // After itkg_batch_autotune the measured times are used instead.
     void assign_branches() {
         std::vector<std::pair<double, int>> units;
+        bool timed = rest_ms.size() == rest.size();
+        for (const BatchClass& c : classes) timed = timed && c.measured_ms >= 0.0f;
         for (size_t i = 0; i < classes.size(); ++i)
-            units.push_back({model_cost(classes[i], classes[i].tz), (int)i});
-        for (int i : rest)  // three passes over the volume, ~64 voxels per CU and step
-            units.push_back({(double)region_len(3, plans[i].req) * 3.0 / (64.0 * num_cu), -1 - i});
+            units.push_back({timed ? (double)classes[i].measured_ms : class_work(classes[i]), (int)i});
+        for (size_t k = 0; k < rest.size(); ++k) {
+            const Plan& p = plans[rest[k]];
+            int64_t r = 0;
+            for (int d = 0; d < 3; ++d) r = std::max<int64_t>(r, p.radii[d]);
+            units.push_back({timed ? (double)rest_ms[k]
+                                   : (double)region_len(3, p.req) * voxel_cost((int)r, false),
+                             -1 - rest[k]});
+        }
         std::sort(units.begin(), units.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
         const int nb = std::min<int>(kBatchBranches, (int)units.size());
         branch_units.assign(nb, {});
@@ -382,7 +416,20 @@ int itkg_batch_autotune(itkg_batch* plan, void* stream) {
         }
         if (rc) break;
         if ((rc = B.build_class(c, best))) break;
+        c.measured_ms = best_ms / 3.0f;
         append_cache({(int)B.dir, (int)B.arith, c.r}, best);
+    }
+    // the per-axis volumes, for the branch balance
+    B.rest_ms.clear();
+    for (size_t k = 0; k < B.rest.size() && !rc; ++k) {
+        rc = B.enqueue_volume(B.rest[k], s, B.scratch[0]);  // warm-up
+        (void)hipEventRecord(e0, s);
+        if (!rc) rc = B.enqueue_volume(B.rest[k], s, B.scratch[0]);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        B.rest_ms.push_back(ms);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -87,6 +87,17 @@ struct Cfg {
                                                   : (xcost(2) <= xcost(1) ? 2 : 1);
     static constexpr int RG = (HY + Q - 1) / Q;   // x-pass items
     static constexpr int XQ = (Q + 2 * RXY + Q - 1) / Q;  // Q-wide reads per x item
+    // Split x pass: full rounds of Q-row items (every wave one item per
+    // round), then the rows left over one row per item (one output per
+    // lane), dealt to the waves; used when its slowest wave has less work.
+    static constexpr int NA = (HY / Q) / NW * NW;  // items of the full rounds
+    static constexpr int RA = Q * NA;              // rows they cover
+    static constexpr int RL = HY - RA;             // rows left over
+    static constexpr int xcost_split() {
+        return (NA / NW) * (Q * (2 * RXY + 1) + 2 * XQ) +
+               ((RL + NW - 1) / NW) * 3 * (2 * RXY + 1);
+    }
+    static constexpr bool XSPLIT = Q > 1 && NA > 0 && xcost_split() < xcost(Q);
     // One sIn plane: the window (row stride WX), room for the unconditional
     // stores of the last loads (LJ * NTH) and for the x pass's reads of the
     // rows past HY in its last item (their outputs are never used).
@@ -215,10 +226,11 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     using vecq = std::conditional_t<Q == 4, float4, std::conditional_t<Q == 2, float2, float>>;
     const int xq = lane % LPR, xr = lane / LPR;
     auto x_pass = [&](const float* srcp, float* dstp) {
+        constexpr int NG = C::XSPLIT ? C::NA : C::RG;  // Q-row items
 #pragma unroll
-        for (int g0 = 0; g0 < C::RG; g0 += C::NW) {
+        for (int g0 = 0; g0 < NG; g0 += C::NW) {
             const int g = g0 + wave;
-            if (g < C::RG) {
+            if (g < NG) {
                 const int row = Q * g + xr;
                 const vecq* srcq = reinterpret_cast<const vecq*>(&srcp[row * C::WX + Q * xq]);
                 float v[Q * C::XQ];
@@ -265,6 +277,31 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                 if constexpr (Q == 4) *dq = make_float4(o[0], o[1], o[2], o[3]);
                 else if constexpr (Q == 2) *dq = make_float2(o[0], o[1]);
                 else *dq = o[0];
+            }
+        }
+        if constexpr (C::XSPLIT) {  // the rows left over: one output per lane
+#pragma unroll
+            for (int k0 = 0; k0 < C::RL; k0 += C::NW) {
+                const int k = k0 + wave;
+                if (k < C::RL) {
+                    const int row = C::RA + k;
+                    const float* rp = srcp + row * C::WX + lane;
+                    float v[2 * RXY + 1];
+#pragma unroll
+                    for (int i = 0; i <= 2 * RXY; ++i) v[i] = rp[i];
+                    float o = tap_sum<RXY, MODE>(v, wx);
+                    if constexpr (ADJOINT) {
+                        if (x_edge && x0 + lane == 0) {
+#pragma unroll
+                            for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[2 * RXY - m], o);
+                        }
+                        if (x_edge && x0 + lane == ndx - 1) {
+#pragma unroll
+                            for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[m], o);
+                        }
+                    }
+                    dstp[row * C::SX + lane] = o;
+                }
             }
         }
     };

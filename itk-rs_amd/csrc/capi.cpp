@@ -335,19 +335,20 @@ void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& 
     const int TY = fe ? fe->ty[mode] : 32;
     A.tiles_x = (A.n_dst[0] + kFusedTileX - 1) / kFusedTileX;
     A.tiles_y = (A.n_dst[1] + TY - 1) / TY;
-    A.tz = tz > 0 ? tz : fused_tz(fe, mode, A.tiles_x * A.tiles_y, A.n_dst[2], rz);
+    A.tz = tz > 0 ? tz : fused_tz(fe, mode, A.tiles_x * A.tiles_y * ncomp, A.n_dst[2], rz);
     A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
 }
 
 int run_fused(const Plan& p, int ncomp, const itkg_view& src_view, const itkg_view& dst_view,
               int mode, int accumulate, hipStream_t s) {
     const FusedEntry* fe = fused_for(p);
-    for (int c = 0; c < ncomp; ++c) {
-        Fused3dArgs A;
-        fill_fused_args(p, src_view, dst_view, mode, accumulate, 0, c, ncomp, &A);
-        hipError_t e = fe->launch(A, mode, s);
-        if (e != hipSuccess) return hip_fail(e, "fused launch");
-    }
+    FusedMulti m;
+    std::memset(&m, 0, sizeof m);
+    m.nc = ncomp;  // one launch over (tile, component) items
+    for (int c = 0; c < ncomp; ++c)
+        fill_fused_args(p, src_view, dst_view, mode, accumulate, 0, c, ncomp, &m.a[c]);
+    hipError_t e = fe->launch(m, mode, s);
+    if (e != hipSuccess) return hip_fail(e, "fused launch");
     return ITKG_OK;
 }
 

@@ -337,6 +337,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     const int ox = x0 + lane;
     const bool x_ok = ox < ndx;
     const int64_t dxs = a.dst_stride[0], dys = a.dst_stride[1], dzs = a.dst_stride[2];
+    const bool dense_x = dxs == 1;
     float* dcol = a.dst + (int64_t)ox * dxs + (int64_t)ybase * dys;
 
     // ---- z ring: fixed slots, rotating weights (wz2 = wz repeated twice)
@@ -407,8 +408,10 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                     const float q = o[c];
                     if (ADJOINT && a.accumulate)
                         dst[c * dys] = dst[c * dys] + q;
-                    else  // streamed output: keep L2 for the input halos
+                    else if (dense_x)  // streamed output: keep L2 for the input halos
                         __builtin_nontemporal_store(q, &dst[c * dys]);
+                    else  // one component of interleaved pixels: L2 merges the lines
+                        dst[c * dys] = q;
                 }
             }
         }
@@ -450,21 +453,26 @@ struct WavesPerEu {
 
 template <int RXY, int RZ, int MODE>
 __global__ __launch_bounds__((Cfg<RXY, RZ, MODE>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
-void fused3d_kernel([[maybe_unused]] const Fused3dArgs args) {
+void fused3d_kernel([[maybe_unused]] const FusedMulti args) {
     using C = Cfg<RXY, RZ, MODE>;
-    // The argument block is read in place from the kernel-argument segment
+    // The argument blocks are read in place from the kernel-argument segment
     // (where `args` lies, at offset 0), so that runtime-indexed weights are
-    // scalar loads and the block is never copied to private memory.
-    const Fused3dArgs& a = *(const Fused3dArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    // scalar loads and no block is copied to private memory.
+    const FusedMulti& m = *(const FusedMulti*)__builtin_amdgcn_kernarg_segment_ptr();
     __shared__ __attribute__((aligned(16))) float sIn[2 * C::PLANE];
     __shared__ __attribute__((aligned(16))) float sX[2 * C::XPLANE];
-    // XCD-aware tile order: hardware deals workgroups to the 8 XCDs round
-    // robin, so consecutive tiles (which share halos) go to one XCD's L2.
-    const int nt = a.tiles_x * a.tiles_y * a.tiles_z;
+    // XCD-aware order of the (tile, component) items: hardware deals
+    // workgroups to the 8 XCDs round robin, so consecutive items (the
+    // components of one tile, then neighbouring tiles, which share halos and
+    // cache lines) go to one XCD's L2.
+    const int nc = m.nc;
+    const Fused3dArgs& a0 = m.a[0];
+    const int nt = a0.tiles_x * a0.tiles_y * a0.tiles_z * nc;
     const int per = (nt + 7) / 8;
-    const int tile = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (tile >= nt) return;
-    fused3d_tile<RXY, RZ, MODE>(a, tile, sIn, sX);
+    const int item = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (item >= nt) return;
+    const int comp = item % nc;
+    fused3d_tile<RXY, RZ, MODE>(m.a[comp], item / nc, sIn, sX);
 }
 
 // Persistent form over a packed work list of (volume, tile) items for many
@@ -485,8 +493,8 @@ void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __re
 }
 
 template <int RXY, int RZ>
-hipError_t launch_r(const Fused3dArgs& a, int mode, hipStream_t st) {
-    const int nt = a.tiles_x * a.tiles_y * a.tiles_z;
+hipError_t launch_r(const FusedMulti& a, int mode, hipStream_t st) {
+    const int nt = a.a[0].tiles_x * a.a[0].tiles_y * a.a[0].tiles_z * a.nc;
     const dim3 grid(((nt + 7) / 8) * 8);
     const dim3 block(mode == FUSED_FWD_EXACT  ? Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH
                      : mode == FUSED_FWD_FAST ? Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH

@@ -72,10 +72,18 @@ struct Fused3dArgs {
 // One compiled instantiation of the fused kernel: x/y radius rxy, z radius
 // rz (0 for 2-D images), its y tile height, and its launchers.  `batch` and
 // `occupancy` are null for instantiations without a persistent form.
+// One plain fused launch over the components of an interleaved pixel: each
+// component is a scalar image with strides scaled by the component count.
+constexpr int kFusedMaxComp = 4;
+struct FusedMulti {
+    Fused3dArgs a[kFusedMaxComp];
+    int32_t nc;
+};
+
 struct FusedEntry {
     int rxy, rz;
     int ty[3];  // tile rows per mode (FusedMode index)
-    hipError_t (*launch)(const Fused3dArgs& a, int mode, hipStream_t s);
+    hipError_t (*launch)(const FusedMulti& m, int mode, hipStream_t s);
     hipError_t (*batch)(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                         int mode, hipStream_t s);
     int (*occupancy)(int mode);        // resident workgroups per CU of the persistent form

@@ -63,11 +63,12 @@ __device__ __forceinline__ void lds_barrier() {
 template <int RXY, int RZ, int MODE>
 struct Cfg {
     static constexpr bool EXACT_MID = MODE == FUSED_FWD_EXACT && RZ >= 9 && RZ <= 14;
-    static constexpr int NW =
-        MODE != FUSED_FWD_EXACT ? 16 : (RZ <= 7 ? 8 : (EXACT_MID ? 8 : 16));
+    static constexpr int NW = MODE != FUSED_FWD_EXACT ? 16 : (EXACT_MID ? 8 : 16);
     static constexpr int NTH = 64 * NW;
     static constexpr int TX = 64;
-    static constexpr int CPT = RZ <= 7 ? 4 : (EXACT_MID && RZ <= 10 ? 4 : 2);
+    static constexpr int CPT = MODE == FUSED_FWD_EXACT
+                                   ? (EXACT_MID && RZ <= 10 ? 4 : 2)
+                                   : (RZ <= 7 ? 4 : 2);
     static constexpr int TY = NW * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows

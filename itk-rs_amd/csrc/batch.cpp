@@ -42,6 +42,7 @@ namespace {
 struct BatchClass {
     int r = 0;
     const FusedEntry* entry = nullptr;  // its persistent launcher
+    int mode = FUSED_FWD_EXACT;         // kernel mode, which sets the tile shape
     int tz = 0;
     std::vector<int> vol;            // volume indices in this group
     Fused3dArgs* d_args = nullptr;   // one per volume of the group
@@ -58,25 +59,27 @@ constexpr int kBatchBranches = 3;
 
 using TuneKey = std::tuple<int, int, int>;  // direction, arith, radius
 
-std::map<TuneKey, int> read_cache() {
-    std::map<TuneKey, int> m;
+// Cache lines: "direction arith radius tz tile_rows".
+std::map<TuneKey, std::pair<int, int>> read_cache() {
+    std::map<TuneKey, std::pair<int, int>> m;
     const char* path = std::getenv("ITKG_TUNE_CACHE");
     if (!path) return m;
     std::ifstream f(path);
     std::string line;
     while (std::getline(f, line)) {
         std::istringstream ss(line);
-        int d, a, r, tz;
-        if (ss >> d >> a >> r >> tz && tz > 0) m[{d, a, r}] = tz;
+        int d, a, r, tz, ty;
+        if (ss >> d >> a >> r >> tz >> ty && tz > 0) m[{d, a, r}] = {tz, ty};
     }
     return m;
 }
 
-void append_cache(const TuneKey& k, int tz) {
+void append_cache(const TuneKey& k, int tz, int ty) {
     const char* path = std::getenv("ITKG_TUNE_CACHE");
     if (!path) return;
     std::ofstream f(path, std::ios::app);
-    f << std::get<0>(k) << ' ' << std::get<1>(k) << ' ' << std::get<2>(k) << ' ' << tz << '\n';
+    f << std::get<0>(k) << ' ' << std::get<1>(k) << ' ' << std::get<2>(k) << ' ' << tz << ' ' << ty
+      << '\n';
 }
 
 }  // namespace
@@ -116,6 +119,11 @@ struct BatchImpl {
             if (p) (void)hipFree(p);
     }
 
+    // The 32-row-tile variant of the plan's mode, or -1 (EXACT has none).
+    int small_mode() const {
+        return mode == FUSED_FWD_FAST ? FUSED_FWD_FAST_S : (mode == FUSED_ADJ ? FUSED_ADJ_S : -1);
+    }
+
     void drop_graphs() {
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         graphs.clear();
@@ -124,13 +132,13 @@ struct BatchImpl {
     // Modelled time of one group at z run tz, in plane steps per slot: the
     // persistent grid deals the tiles out evenly, and the last tile dealt can
     // finish up to one tile's walk after the others.
-    double model_cost(const BatchClass& c, int tz) const {
-        const int64_t slots = (int64_t)num_cu * std::max(1, c.entry->occupancy(mode));
+    double model_cost(const BatchClass& c, int tz, int m) const {
+        const int64_t slots = (int64_t)num_cu * std::max(1, c.entry->occupancy(m));
         double steps = 0.0, longest = 0.0;
         for (int i : c.vol) {
             const itkg_volume& v = vols[i];
             const int64_t txy = ((v.size[0] + kFusedTileX - 1) / kFusedTileX) *
-                                ((v.size[1] + c.entry->ty[mode] - 1) / c.entry->ty[mode]);
+                                ((v.size[1] + c.entry->ty[m] - 1) / c.entry->ty[m]);
             const int64_t nz = v.size[2];
             const int64_t tzv = std::min<int64_t>(tz, nz);
             const double walk = (double)(tzv + 2 * c.r + 3);
@@ -140,11 +148,15 @@ struct BatchImpl {
         return steps / (double)slots + longest;
     }
 
-    int model_tz(const BatchClass& c) const {
+    // Work of one plane step grows with the tile's rows plus their x halo,
+    // plus a fixed part (loads, barrier).
+    static double step_work(const BatchClass& c, int m) { return 3.0 * c.entry->ty[m] + 2.0 * c.r + 30.0; }
+
+    int model_tz(const BatchClass& c, int m) const {
         int best = 32;
         double best_cost = 1e300;
         for (int tz : {16, 24, 32, 48, 64, 96, 128, 192, 256}) {
-            const double cost = model_cost(c, tz);
+            const double cost = model_cost(c, tz, m);
             if (cost < best_cost) {
                 best_cost = cost;
                 best = tz;
@@ -213,7 +225,7 @@ struct BatchImpl {
             const Plan& p = plans[c.vol[i]];
             const itkg_view vin = dense_view(3, const_cast<float*>(v.input), p.req);
             const itkg_view vout = dense_view(3, v.output, p.req);
-            fill_fused_args(p, vin, vout, mode, 0, tz, 0, 1, &args[i]);
+            fill_fused_args(p, vin, vout, c.mode, 0, tz, 0, 1, &args[i]);
             const int nt = args[i].tiles_x * args[i].tiles_y * args[i].tiles_z;
             for (int t = 0; t < nt; ++t) items.push_back(make_int2((int)i, t));
         }
@@ -238,14 +250,14 @@ struct BatchImpl {
         if (e != hipSuccess) return hip_fail(e, "batch tables");
         c.tz = tz;
         c.nwork = (int)work.size();
-        const int per_cu = std::max(1, c.entry->occupancy(mode));
+        const int per_cu = std::max(1, c.entry->occupancy(c.mode));
         c.grid = std::min(c.nwork, ((num_cu * per_cu + 7) / 8) * 8);
         return ITKG_OK;
     }
 
     int enqueue_class(const BatchClass& c, hipStream_t s) const {
         if (c.nwork == 0) return ITKG_OK;
-        hipError_t e = c.entry->batch(c.d_args, c.d_work, c.nwork, c.grid, mode, s);
+        hipError_t e = c.entry->batch(c.d_args, c.d_work, c.nwork, c.grid, c.mode, s);
         if (e != hipSuccess) return hip_fail(e, "batched fused launch");
         return ITKG_OK;
     }
@@ -368,7 +380,27 @@ int itkg_batch_create(int32_t n, const itkg_volume* volumes, itkg_batch_directio
     const auto cache = read_cache();
     for (BatchClass& c : B.classes) {
         auto it = cache.find({(int)direction, (int)arith, c.r});
-        const int rc = B.build_class(c, it != cache.end() ? it->second : B.model_tz(c));
+        int tz = 0;
+        c.mode = B.mode;
+        const int small = B.small_mode();
+        if (it != cache.end()) {
+            tz = it->second.first;
+            if (small >= 0 && it->second.second == c.entry->ty[small]) c.mode = small;
+        } else {
+            // the tile shape and z run the model prefers
+            double best = 1e300;
+            for (int m : {B.mode, small}) {
+                if (m < 0) continue;
+                const int t = B.model_tz(c, m);
+                const double cost = B.model_cost(c, t, m) * BatchImpl::step_work(c, m);
+                if (cost < best) {
+                    best = cost;
+                    c.mode = m;
+                    tz = t;
+                }
+            }
+        }
+        const int rc = B.build_class(c, tz);
         if (rc) {
             delete b;
             return rc;
@@ -393,7 +425,10 @@ int itkg_batch_autotune(itkg_batch* plan, void* stream) {
         int maxz = 1;
         for (int i : c.vol) maxz = std::max<int>(maxz, (int)B.vols[i].size[2]);
         float best_ms = 1e30f;
-        int best = c.tz;
+        int best = c.tz, best_mode = c.mode;
+        for (int m : {B.mode, B.small_mode()}) {
+        if (m < 0) continue;
+        c.mode = m;
         for (int tz : {16, 24, 32, 48, 64, 96, 128, 192, 256}) {
             if (tz < 2 * c.r || (tz > maxz && tz != 16)) continue;
             if ((rc = B.build_class(c, tz))) break;
@@ -412,12 +447,16 @@ int itkg_batch_autotune(itkg_batch* plan, void* stream) {
             if (tot < best_ms) {
                 best_ms = tot;
                 best = tz;
+                best_mode = m;
             }
         }
         if (rc) break;
+        }
+        if (rc) break;
+        c.mode = best_mode;
         if ((rc = B.build_class(c, best))) break;
         c.measured_ms = best_ms / 3.0f;
-        append_cache({(int)B.dir, (int)B.arith, c.r}, best);
+        append_cache({(int)B.dir, (int)B.arith, c.r}, best, c.entry->ty[c.mode]);
     }
     // the per-axis volumes, for the branch balance
     B.rest_ms.clear();

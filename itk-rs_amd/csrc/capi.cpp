@@ -290,7 +290,7 @@ const FusedEntry* fused_for(const Plan& p) {
 // grad_input at the input region.  tz <= 0 picks the z run with fused_tz.
 void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
                      int mode, int accumulate, int tz, int comp, int ncomp, Fused3dArgs* out) {
-    const bool adj = mode == FUSED_ADJ;
+    const bool adj = fused_is_adjoint(mode);
     const itkg_region& sreg = adj ? p.req : p.in_reg;
     const itkg_region& dreg = adj ? p.in_reg : p.req;
     const int D = p.D;
@@ -339,9 +339,38 @@ void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& 
     A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
 }
 
+// Tile shape of a FAST or adjoint launch: 64-row tiles (fewer halo rows per
+// output) or 32-row tiles (twice the tiles, so longer z runs fill the GPU
+// on smaller volumes).  Modelled cost: rounds of the grid x plane steps per
+// tile x work per step, which grows with the tile's rows plus their x halo
+// plus a fixed per-step part (loads, barrier).  ITKG_FUSED_SHAPE=64|32
+// overrides.
+int pick_shape(const Plan& p, const FusedEntry* fe, int mode, int ncomp) {
+    if (mode != FUSED_FWD_FAST && mode != FUSED_ADJ) return mode;
+    const int small = mode == FUSED_FWD_FAST ? FUSED_FWD_FAST_S : FUSED_ADJ_S;
+    if (const char* e = std::getenv("ITKG_FUSED_SHAPE")) return std::atoi(e) == 32 ? small : mode;
+    const itkg_region& d = fused_is_adjoint(mode) ? p.in_reg : p.req;
+    const int64_t nx = d.size[0], ny = p.D >= 2 ? d.size[1] : 1, nz = p.D == 3 ? d.size[2] : 1;
+    const int rxy = (int)p.radii[0], rz = p.D == 3 ? (int)p.radii[2] : 0;
+    auto cost = [&](int m) {
+        const int ty = fe->ty[m];
+        const int64_t txy = ((nx + kFusedTileX - 1) / kFusedTileX) * ((ny + ty - 1) / ty) * ncomp;
+        const int tz = fused_tz(fe, m, (int)std::min<int64_t>(txy, INT32_MAX), (int)nz, rz);
+        const int64_t tiles = txy * ((nz + tz - 1) / tz);
+        const int64_t slots = (int64_t)device_cus() * std::max(1, fe->plain_occupancy(m));
+        const double rounds = (double)((tiles + slots - 1) / slots);
+        return rounds * (double)(std::min<int64_t>(tz, nz) + 2 * rz + 3) * (3.0 * ty + 2.0 * rxy + 30.0);
+    };
+    const int pick = cost(small) < cost(mode) ? small : mode;
+    if (std::getenv("ITKG_DEBUG"))
+        std::fprintf(stderr, "itkg: fused shape %d-row tiles (mode %d)\n", fe->ty[pick], pick);
+    return pick;
+}
+
 int run_fused(const Plan& p, int ncomp, const itkg_view& src_view, const itkg_view& dst_view,
               int mode, int accumulate, hipStream_t s) {
     const FusedEntry* fe = fused_for(p);
+    mode = pick_shape(p, fe, mode, ncomp);
     FusedMulti m;
     std::memset(&m, 0, sizeof m);
     m.nc = ncomp;  // one launch over (tile, component) items

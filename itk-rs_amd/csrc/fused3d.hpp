@@ -57,6 +57,7 @@ __device__ __forceinline__ void lds_barrier() {
 // Waves and output columns per lane, by z radius and mode: the z ring holds
 // CPT * (2rz+1) accumulators per lane; the y tile is TY = NW * CPT rows.
 //   FAST and adjoint: 16 waves; 4 columns (64-row tiles) up to rz 7, then 2;
+//   their _S variants: 16 waves x 2 columns (32-row tiles) at every radius;
 //   EXACT (a larger register need: the product and the sum round
 //   separately): 8 waves x 4 columns up to rz 7; 16 x 2 at rz 8 and 15;
 //   8 x 4 (rz 9, 10) or 8 x 2 (rz 11..14), where 16 waves would spill.
@@ -66,9 +67,10 @@ struct Cfg {
     static constexpr int NW = MODE != FUSED_FWD_EXACT ? 16 : (EXACT_MID ? 8 : 16);
     static constexpr int NTH = 64 * NW;
     static constexpr int TX = 64;
+    static constexpr bool SMALL = MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S;
     static constexpr int CPT = MODE == FUSED_FWD_EXACT
                                    ? (EXACT_MID && RZ <= 10 ? 4 : 2)
-                                   : (RZ <= 7 ? 4 : 2);
+                                   : (RZ <= 7 && !SMALL ? 4 : 2);
     static constexpr int TY = NW * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows
@@ -162,7 +164,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                                              float* sX) {
     using C = Cfg<RXY, RZ, MODE>;
     constexpr int CPT = C::CPT, P = C::P, LJ = C::LJ;
-    constexpr bool ADJOINT = MODE == FUSED_ADJ;
+    constexpr bool ADJOINT = fused_is_adjoint(MODE);
     const int tx_i = tile % a.tiles_x;
     const int ty_i = (tile / a.tiles_x) % a.tiles_y;
     const int tz_i = tile / (a.tiles_x * a.tiles_y);
@@ -493,13 +495,22 @@ void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __re
     }
 }
 
+// Workgroup size of mode `mode` for (RXY, RZ).
+template <int RXY, int RZ>
+int block_of(int mode) {
+    switch (mode) {
+        case FUSED_FWD_EXACT: return Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH;
+        case FUSED_FWD_FAST: return Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH;
+        case FUSED_ADJ: return Cfg<RXY, RZ, FUSED_ADJ>::NTH;
+        case FUSED_FWD_FAST_S: return Cfg<RXY, RZ, FUSED_FWD_FAST_S>::NTH;
+        default: return Cfg<RXY, RZ, FUSED_ADJ_S>::NTH;
+    }
+}
+
 template <int RXY, int RZ>
 hipError_t launch_r(const FusedMulti& a, int mode, hipStream_t st) {
     const int nt = a.a[0].tiles_x * a.a[0].tiles_y * a.a[0].tiles_z * a.nc;
-    const dim3 grid(((nt + 7) / 8) * 8);
-    const dim3 block(mode == FUSED_FWD_EXACT  ? Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH
-                     : mode == FUSED_FWD_FAST ? Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH
-                                              : Cfg<RXY, RZ, FUSED_ADJ>::NTH);
+    const dim3 grid(((nt + 7) / 8) * 8), block(block_of<RXY, RZ>(mode));
     switch (mode) {
         case FUSED_FWD_EXACT:
             hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT>), grid, block, 0, st, a);
@@ -510,6 +521,12 @@ hipError_t launch_r(const FusedMulti& a, int mode, hipStream_t st) {
         case FUSED_ADJ:
             hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_ADJ>), grid, block, 0, st, a);
             break;
+        case FUSED_FWD_FAST_S:
+            hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_FAST_S>), grid, block, 0, st, a);
+            break;
+        case FUSED_ADJ_S:
+            hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_ADJ_S>), grid, block, 0, st, a);
+            break;
         default:
             return hipErrorInvalidValue;
     }
@@ -519,10 +536,7 @@ hipError_t launch_r(const FusedMulti& a, int mode, hipStream_t st) {
 template <int RXY, int RZ>
 hipError_t launch_batch_r(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                           int mode, hipStream_t st) {
-    const dim3 grid(grid_blocks);
-    const dim3 block(mode == FUSED_FWD_EXACT  ? Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH
-                     : mode == FUSED_FWD_FAST ? Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH
-                                              : Cfg<RXY, RZ, FUSED_ADJ>::NTH);
+    const dim3 grid(grid_blocks), block(block_of<RXY, RZ>(mode));
     switch (mode) {
         case FUSED_FWD_EXACT:
             hipLaunchKernelGGL((fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>), grid, block, 0, st,
@@ -534,6 +548,14 @@ hipError_t launch_batch_r(const Fused3dArgs* vols, const int2* work, int nwork, 
             break;
         case FUSED_ADJ:
             hipLaunchKernelGGL((fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>), grid, block, 0, st, vols,
+                               work, nwork);
+            break;
+        case FUSED_FWD_FAST_S:
+            hipLaunchKernelGGL((fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST_S>), grid, block, 0, st,
+                               vols, work, nwork);
+            break;
+        case FUSED_ADJ_S:
+            hipLaunchKernelGGL((fused3d_batch_kernel<RXY, RZ, FUSED_ADJ_S>), grid, block, 0, st, vols,
                                work, nwork);
             break;
         default:
@@ -552,16 +574,23 @@ int occupancy_r(int mode) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, nth, 0);
     };
     constexpr int n0 = Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH, n1 = Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH,
-                  n2 = Cfg<RXY, RZ, FUSED_ADJ>::NTH;
+                  n2 = Cfg<RXY, RZ, FUSED_ADJ>::NTH, n3 = Cfg<RXY, RZ, FUSED_FWD_FAST_S>::NTH,
+                  n4 = Cfg<RXY, RZ, FUSED_ADJ_S>::NTH;
     if (mode == FUSED_FWD_EXACT)
         e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>, n0)
                     : query(fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT>, n0);
     else if (mode == FUSED_FWD_FAST)
         e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST>, n1)
                     : query(fused3d_kernel<RXY, RZ, FUSED_FWD_FAST>, n1);
-    else
+    else if (mode == FUSED_ADJ)
         e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>, n2)
                     : query(fused3d_kernel<RXY, RZ, FUSED_ADJ>, n2);
+    else if (mode == FUSED_FWD_FAST_S)
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST_S>, n3)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_FAST_S>, n3);
+    else if (mode == FUSED_ADJ_S)
+        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ_S>, n4)
+                    : query(fused3d_kernel<RXY, RZ, FUSED_ADJ_S>, n4);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
@@ -570,7 +599,8 @@ template <int RXY, int RZ>
 constexpr FusedEntry make_entry() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ>::TY},
+                       Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY},
                       &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>,
                       &occupancy_r<RXY, RZ, true>, &occupancy_r<RXY, RZ, false>};
 }
@@ -578,7 +608,8 @@ template <int RXY, int RZ>
 constexpr FusedEntry make_entry_single() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ>::TY},
+                       Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY},
                       &launch_r<RXY, RZ>, nullptr, nullptr,
                       &occupancy_r<RXY, RZ, false>};
 }

@@ -50,7 +50,17 @@ hipError_t launch_axis_adjoint_range(const AxisPassArgs<T>& a, int ncomp, int64_
 // Fused one-round-trip 3-D kernel for f32 scalar pixels (forward or adjoint).
 // Destination pixel j reads source pixels j + shift + k - r, k = 0..2r
 // (forward: clamped into the source; adjoint: zero outside it).
-enum FusedMode { FUSED_FWD_EXACT = 0, FUSED_FWD_FAST = 1, FUSED_ADJ = 2 };
+// The _S modes are FAST and adjoint on 32-row tiles (the others use 64-row
+// tiles), for volumes small enough that shorter tiles buy longer z runs.
+enum FusedMode {
+    FUSED_FWD_EXACT = 0,
+    FUSED_FWD_FAST = 1,
+    FUSED_ADJ = 2,
+    FUSED_FWD_FAST_S = 3,
+    FUSED_ADJ_S = 4,
+    FUSED_NMODES = 5
+};
+constexpr bool fused_is_adjoint(int mode) { return mode == FUSED_ADJ || mode == FUSED_ADJ_S; }
 constexpr int kFusedMaxR = 16;
 
 struct Fused3dArgs {
@@ -82,7 +92,7 @@ struct FusedMulti {
 
 struct FusedEntry {
     int rxy, rz;
-    int ty[3];  // tile rows per mode (FusedMode index)
+    int ty[FUSED_NMODES];  // tile rows per mode (FusedMode index)
     hipError_t (*launch)(const FusedMulti& m, int mode, hipStream_t s);
     hipError_t (*batch)(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                         int mode, hipStream_t s);

@@ -66,7 +66,7 @@ def test_batch_autotune_keeps_results(itk, dev, tmp_path, monkeypatch):
     for a, b in zip(first, outs):
         assert torch.equal(a, b)
     lines = (tmp_path / "tune.txt").read_text().split("\n")
-    assert any(len(ln.split()) == 4 for ln in lines)
+    assert any(len(ln.split()) == 5 for ln in lines)   # direction arith radius tz tile_rows
     # a new plan picks the cached choice up and gives the same results
     outs2 = [torch.empty_like(v) for v in vols]
     plan2 = itk.BatchPlan(vols, outs2, [p[1] for p in params], [p[2] for p in params])

@@ -247,3 +247,23 @@ def test_per_axis_strided_output_view(itk, orc, dev):
     rest = big.clone()
     rest[2:11, 1:16, 5:275] = 0.0
     assert torch.isnan(rest).sum().item() == rest.numel() - win.numel()  # nothing else written
+
+
+@pytest.mark.parametrize("sigma", [0.9, 2.0, 3.4])
+def test_fused_tile_shapes_agree(itk, orc, dev, monkeypatch, sigma):
+    """FAST forward and the adjoint on 64-row and on 32-row tiles: every voxel
+    follows the same FMA chain, so the two shapes agree bit for bit; both
+    within tolerance of the oracle."""
+    rng = np.random.default_rng(31)
+    arr = rng.random((37, 90, 130)).astype(np.float32)
+    x = torch.from_numpy(arr).to(dev)
+    outs, grads = {}, {}
+    for shape in ("64", "32"):
+        monkeypatch.setenv("ITKG_FUSED_SHAPE", shape)
+        outs[shape] = itk.gaussian(x, sigma, (1.0, 1.0, 1.0), arith=itk.FAST)
+        grads[shape] = itk.gaussian_backward(x, sigma, (1.0, 1.0, 1.0))
+    torch.cuda.synchronize()
+    assert torch.equal(outs["64"], outs["32"])
+    assert torch.equal(grads["64"], grads["32"])
+    ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, [0, 0, 0], [130, 90, 37])
+    assert _rel_err(outs["64"].cpu().numpy(), ref) <= FAST_TOL

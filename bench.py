@@ -14,8 +14,9 @@ reference, the accuracy the target is stated at).  Also reported (under
     operator (and of FAST against EXACT), relative to the max |output|,
   - forward + adjoint at 256^3 (sigma 1.8) vs forward only,
   - the batched form (64 uneven volumes, persistent launches per radius
-    group in one captured graph, z runs picked by the autotuner) vs a loop
-    of plain single-volume launches,
+    group in one captured graph, tile shapes and z runs picked by the
+    autotuner) vs a loop of plain single-volume launches, and the batched
+    adjoint over 16 uneven gradients vs a loop of plain adjoint launches,
   - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration.
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
@@ -218,6 +219,27 @@ def main():
                                    "speedup": round(wl_ / wb, 3)}
         plan.close()
         del vols, outs, imgs, oims, fl
+        torch.cuda.empty_cache()
+
+        # batched adjoint: 16 uneven gradients (extents 48..192), sigma 1.8
+        grads, gparams = wl.c5_batch(dev)
+        gouts = [torch.empty_like(q) for q in grads]
+        gplan = itk.BatchPlan(grads, gouts, [p[1] for p in gparams], [p[2] for p in gparams],
+                              direction=itk.BATCH_BACKWARD)
+        ng = gplan.voxels
+        gplan.autotune(stream)
+        wgb, _ = timed_loop(lambda: gplan.run(stream), 10, 2, stream)
+
+        def gloop():
+            for q, o, p in zip(grads, gouts, gparams):
+                itk.gaussian_backward(q, p[1], p[2], out=o)
+        wgl, _ = timed_loop(gloop, 10, 2, stream)
+        extras["c5_batched_adjoint_16"] = {"voxels": ng,
+                                           "batched_gvox_s": round(ng * 10 / wgb / 1e9, 3),
+                                           "loop_gvox_s": round(ng * 10 / wgl / 1e9, 3),
+                                           "speedup": round(wgl / wgb, 3)}
+        gplan.close()
+        del grads, gouts
         torch.cuda.empty_cache()
 
         # 512^2 2-D and the VecPixel configuration

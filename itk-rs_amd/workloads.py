@@ -163,3 +163,17 @@ def c5(device, shape=(256, 256, 256)):
     x = uniform(4, 0, shape, device)
     g = normal(4, 1, shape, device)
     return x, g, 1.8, (1.0, 1.0, 1.0)
+
+
+def c5_batch(device, nvol: int = 16, lo: int = 48, hi: int = 192, seed: int = 4):
+    """Batched adjoint: nvol upstream gradients N(0,1) with extents drawn per
+    axis uniformly in [lo, hi], sigma 1.8, spacing 1.  Returns (grads, params)
+    with params[v] = (shape_zyx, sigma, spacing)."""
+    p = _host_uniform(seed, 300, nvol * 3).reshape(nvol, 3)
+    grads, params = [], []
+    for v in range(nvol):
+        ext = [lo + min(int(p[v, d] * (hi - lo + 1)), hi - lo) for d in range(3)]
+        shape = (ext[2], ext[1], ext[0])
+        grads.append(normal(seed, 2000 + v, shape, device))
+        params.append((shape, 1.8, (1.0, 1.0, 1.0)))
+    return grads, params

@@ -76,3 +76,24 @@ def test_batch_autotune_keeps_results(itk, dev, tmp_path, monkeypatch):
         assert torch.equal(a, b)
     plan.close()
     plan2.close()
+
+
+def test_batched_adjoint_uneven_gradients(itk, dev):
+    """The batched adjoint over uneven gradient volumes (sigma 1.8) equals the
+    plain adjoint of each, before and after autotuning."""
+    from itk_rs_amd import workloads as wl
+    grads, params = wl.c5_batch(dev, nvol=6)
+    outs = [torch.full_like(g, float("nan")) for g in grads]
+    plan = itk.BatchPlan(grads, outs, [p[1] for p in params], [p[2] for p in params],
+                         direction=itk.BATCH_BACKWARD)
+    s = torch.cuda.Stream()
+    for tune in (False, True):
+        if tune:
+            plan.autotune(s)
+        plan.run(s)
+        s.synchronize()
+        for g, o, (shape, sigma, sp) in zip(grads, outs, params):
+            ref = itk.gaussian_backward(g, sigma, sp)
+            torch.cuda.synchronize()
+            assert torch.equal(o, ref), (shape, tune)
+    plan.close()

@@ -32,6 +32,7 @@ struct AxisPassArgs {
     int64_t dst_lo[ITKG_MAX_DIM];
     int64_t dst_stride[ITKG_MAX_DIM];
     int64_t clamp_lo, clamp_hi;      // pass input extent on `axis`
+    int64_t g_lo, g_hi;              // adjoint only: incoming-gradient extent on `axis`
     int64_t total;                   // product of n
     int32_t axis, radius;
     int32_t accumulate;              // adjoint only: dst += result

@@ -89,8 +89,43 @@ __device__ __forceinline__ void conv_run(const T* w, const int r, Load&& load, T
     if (jb < taps) block(jb, taps - jb);
 }
 
-// Pass along a strided axis (a.axis >= 1).
-template <typename T, int NC, bool FAST, int CPT>
+// Adjoint edge terms of one output at pass-axis position j, added to o:
+// at j = clamp_lo the outputs i < clamp_lo + r fold their clamped taps
+// onto it with weight tail[clamp_lo - i + r], at j = clamp_hi the outputs
+// i > clamp_hi - r with weight tail[i + r - clamp_hi] (the weights are
+// symmetric, so the top-m sum equals the first-m sum).  `at(i)` reads the
+// incoming gradient at pass-axis position i.
+template <typename T, int NC, class At>
+__device__ __forceinline__ void adjoint_edges(const AxisPassArgs<T>& a, int64_t j, At&& at,
+                                              T (&o)[NC]) {
+    const int r = a.radius;
+    if (j == a.clamp_lo) {
+        const int64_t ie = a.clamp_lo + r - 1 < a.g_hi ? a.clamp_lo + r - 1 : a.g_hi;
+        for (int64_t i = a.g_lo; i <= ie; ++i) {
+            T v[NC];
+            at(i, v);
+            const T t = a.tail[a.clamp_lo - i + r];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) o[q] = tap<T, true>(o[q], v[q], t);
+        }
+    }
+    if (j == a.clamp_hi) {
+        const int64_t ib = a.clamp_hi - r + 1 > a.g_lo ? a.clamp_hi - r + 1 : a.g_lo;
+        for (int64_t i = ib; i <= a.g_hi; ++i) {
+            T v[NC];
+            at(i, v);
+            const T t = a.tail[i + r - a.clamp_hi];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) o[q] = tap<T, true>(o[q], v[q], t);
+        }
+    }
+}
+
+// Pass along a strided axis (a.axis >= 1).  Forward: inputs clamped to
+// [clamp_lo, clamp_hi] on the pass axis.  Adjoint (ADJ): the incoming
+// gradient is zero outside [g_lo, g_hi] (a zero-padded correlation, which
+// equals the convolution for the symmetric weights), plus the edge terms.
+template <typename T, int NC, bool FAST, bool ADJ, int CPT>
 __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] const AxisPassArgs<T> args) {
     // Read in place from the kernel-argument segment: the weights are indexed
     // with runtime values (scalar loads, no private copy of the block).
@@ -118,24 +153,46 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
     const int64_t p0 = a.out_lo[0] + colc, p1 = a.out_lo[b1] + o1, p2 = a.out_lo[b2] + o2;
     const int r = a.radius;
     const int64_t first = a.out_lo[ax] + ia * CPT - r;  // pass-axis position of input 0
-    // clamp bounds relative to input 0 (the extent along one axis fits int)
-    const int lo_k = (int)(a.clamp_lo - first), hi_k = (int)(a.clamp_hi - first);
+    // input bounds relative to input 0 (the extent along one axis fits int):
+    // forward clamps into [clamp_lo, clamp_hi], the adjoint reads zeros
+    // outside [g_lo, g_hi]
+    const int lo_k = (int)((ADJ ? a.g_lo : a.clamp_lo) - first);
+    const int hi_k = (int)((ADJ ? a.g_hi : a.clamp_hi) - first);
     const int64_t sa = a.src_stride[ax] * NC;
     const T* src = a.src + ((p0 - a.src_lo[0]) * a.src_stride[0] + (p1 - a.src_lo[b1]) * a.src_stride[b1] +
                             (p2 - a.src_lo[b2]) * a.src_stride[b2] + (first - a.src_lo[ax]) * a.src_stride[ax]) * NC;
     auto load = [&](int jb, auto off, T (&v)[NC]) {
-        int i = jb + decltype(off)::value;
-        i = i < lo_k ? lo_k : (i > hi_k ? hi_k : i);
-        const T* p = src + (int64_t)i * sa;
+        const int i = jb + decltype(off)::value;
+        if constexpr (ADJ) {
+            if (i < lo_k || i > hi_k) {
+#pragma unroll
+                for (int q = 0; q < NC; ++q) v[q] = T(0);
+                return;
+            }
+        }
+        const int ic = i < lo_k ? lo_k : (i > hi_k ? hi_k : i);
+        const T* p = src + (int64_t)ic * sa;
 #pragma unroll
         for (int q = 0; q < NC; ++q) v[q] = p[q];
     };
     T acc[CPT][NC];
     conv_run<T, NC, FAST, CPT>(a.w, r, load, acc);
+    const int64_t j0 = a.out_lo[ax] + ia * CPT;  // pass-axis position of output 0
+    if constexpr (ADJ) {
+        if (j0 <= a.clamp_lo + CPT - 1 || j0 + CPT - 1 >= a.clamp_hi) {  // uniform
+            auto at = [&](int64_t i, T (&v)[NC]) {
+                const T* p = src + (i - first) * sa;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) v[q] = p[q];
+            };
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) adjoint_edges<T, NC>(a, j0 + c, at, acc[c]);
+        }
+    }
     if (!col_ok) return;
     T* dst = a.dst + ((a.out_lo[0] + col - a.dst_lo[0]) * a.dst_stride[0] +
                       (p1 - a.dst_lo[b1]) * a.dst_stride[b1] + (p2 - a.dst_lo[b2]) * a.dst_stride[b2]) * NC;
-    const int64_t oa0 = a.out_lo[ax] + ia * CPT - a.dst_lo[ax];
+    const int64_t oa0 = j0 - a.dst_lo[ax];
     const int64_t da = a.dst_stride[ax] * NC;
     const int nval = (int)(a.n[ax] - ia * CPT);  // outputs of this run inside the region
 #pragma unroll
@@ -143,7 +200,7 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
         if (c < nval) {
             T* o = dst + (oa0 + c) * da;
 #pragma unroll
-            for (int q = 0; q < NC; ++q) o[q] = acc[c][q];
+            for (int q = 0; q < NC; ++q) o[q] = (ADJ && a.accumulate) ? o[q] + acc[c][q] : acc[c][q];
         }
     }
 }
@@ -153,8 +210,9 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
 template <int CPT>
 __device__ __forceinline__ int staged(int i) { return (i % CPT) + (i / CPT) * (CPT + 1); }
 
-// Pass along axis 0.  Dynamic LDS: 4 waves x row_lds_bytes() / 4.
-template <typename T, int NC, bool FAST, int CPT>
+// Pass along axis 0 (forward, or the adjoint with ADJ).  Dynamic LDS: 4
+// waves x row_lds_bytes() / 4.
+template <typename T, int NC, bool FAST, bool ADJ, int CPT>
 __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const AxisPassArgs<T> args) {
     const AxisPassArgs<T>& a = *(const AxisPassArgs<T>*)__builtin_amdgcn_kernarg_segment_ptr();
     extern __shared__ __attribute__((aligned(16))) unsigned char row_lds_raw[];
@@ -177,17 +235,20 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     const bool live = o3 < a.n[3];  // wave-uniform; dead waves only skip memory
     const int64_t q1 = a.out_lo[1] + o1, q2 = a.out_lo[2] + o2, q3 = a.out_lo[3] + o3;
     const int64_t first = a.out_lo[0] + is * 64 * CPT - r;  // axis-0 position of input 0
-    const int lo_k = (int)(a.clamp_lo - first), hi_k = (int)(a.clamp_hi - first);
+    const int lo_k = (int)((ADJ ? a.g_lo : a.clamp_lo) - first);
+    const int hi_k = (int)((ADJ ? a.g_hi : a.clamp_hi) - first);
     const int64_t s0 = a.src_stride[0] * NC;
     const T* src = a.src + ((q1 - a.src_lo[1]) * a.src_stride[1] + (q2 - a.src_lo[2]) * a.src_stride[2] +
                             (q3 - a.src_lo[3]) * a.src_stride[3] + (first - a.src_lo[0]) * a.src_stride[0]) * NC;
     if (live) {
+#pragma unroll 4
         for (int i = lane; i < span; i += 64) {
+            const bool in = i >= lo_k && i <= hi_k;
             const int ic = i < lo_k ? lo_k : (i > hi_k ? hi_k : i);
             const T* p = src + (int64_t)ic * s0;
             T* d = buf + staged<CPT>(i) * NC;
 #pragma unroll
-            for (int q = 0; q < NC; ++q) d[q] = p[q];
+            for (int q = 0; q < NC; ++q) d[q] = (ADJ && !in) ? T(0) : p[q];
         }
     }
     // The staged segment is private to this wave: order its LDS writes
@@ -208,6 +269,18 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     T acc[CPT][NC];
     conv_run<T, NC, FAST, CPT>(a.w, r, load, acc);
     const int64_t x0 = is * 64 * CPT + base;  // output index of acc[0] along axis 0
+    if constexpr (ADJ) {
+        const int64_t j0 = a.out_lo[0] + x0;
+        if (j0 <= a.clamp_lo + CPT - 1 || j0 + CPT - 1 >= a.clamp_hi) {  // at most two lanes
+            auto at = [&](int64_t i, T (&v)[NC]) {
+                const T* p = src + (i - first) * s0;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) v[q] = p[q];
+            };
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) adjoint_edges<T, NC>(a, j0 + c, at, acc[c]);
+        }
+    }
     T* dst = a.dst + ((a.out_lo[0] + x0 - a.dst_lo[0]) * a.dst_stride[0] +
                       (q1 - a.dst_lo[1]) * a.dst_stride[1] + (q2 - a.dst_lo[2]) * a.dst_stride[2] +
                       (q3 - a.dst_lo[3]) * a.dst_stride[3]) * NC;
@@ -215,84 +288,14 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     for (int c = 0; c < CPT; ++c) {
         if (x0 + c < a.n[0]) {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) dst[c * a.dst_stride[0] * NC + q] = acc[c][q];
+            for (int q = 0; q < NC; ++q) {
+                T* o = dst + c * a.dst_stride[0] * NC + q;
+                *o = (ADJ && a.accumulate) ? *o + acc[c][q] : acc[c][q];
+            }
         }
     }
 }
 
-// Transposed pass: the destination is the forward pass's source region.
-// g_in[j] = sum_k g_out[j - k + r] * w_k over outputs that exist, plus, at
-// the two clamp edges, every output's clamped taps folded onto the edge:
-// at j = lo the outputs i < lo + r contribute g_out[i] * sum_{k < lo-i+r} w_k,
-// at j = hi the outputs i > hi - r contribute g_out[i] * sum_{k < i+r-hi} w_k
-// (the weights are symmetric, so the top-m sum equals the first-m sum).
-template <typename T, int NC>
-__global__ __launch_bounds__(256) void axis_adjoint_kernel(const AxisPassArgs<T> a,
-                                                           int64_t g_lo, int64_t g_hi) {
-    const int ax = a.axis;
-    const int64_t step = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < a.total;
-         lin += step) {
-        int64_t rem = lin;
-        int64_t c[ITKG_MAX_DIM];
-#pragma unroll
-        for (int d = 0; d < ITKG_MAX_DIM - 1; ++d) {
-            c[d] = rem % a.n[d];
-            rem /= a.n[d];
-        }
-        c[ITKG_MAX_DIM - 1] = rem;
-        int64_t soff = 0, doff = 0, j = 0;
-#pragma unroll
-        for (int d = 0; d < ITKG_MAX_DIM; ++d) {
-            const int64_t o = a.out_lo[d] + c[d];
-            doff += (o - a.dst_lo[d]) * a.dst_stride[d];
-            if (d == ax) {
-                j = o;
-            } else {
-                soff += (o - a.src_lo[d]) * a.src_stride[d];
-            }
-        }
-        const int64_t gs = a.src_stride[ax];
-        T acc[NC];
-#pragma unroll
-        for (int q = 0; q < NC; ++q) acc[q] = T(0);
-        const int r = a.radius;
-        for (int k = 0; k <= 2 * r; ++k) {
-            const int64_t i = j - k + r;
-            if (i < g_lo || i > g_hi) continue;
-            const T* p = a.src + (soff + (i - a.src_lo[ax]) * gs) * NC;
-#pragma unroll
-            for (int q = 0; q < NC; ++q) acc[q] = acc[q] + p[q] * a.w[k];
-        }
-        if (j == a.clamp_lo) {
-            const int64_t ie = a.clamp_lo + r - 1 < g_hi ? a.clamp_lo + r - 1 : g_hi;
-            for (int64_t i = g_lo; i <= ie; ++i) {
-                const T t = a.tail[a.clamp_lo - i + r];
-                const T* p = a.src + (soff + (i - a.src_lo[ax]) * gs) * NC;
-#pragma unroll
-                for (int q = 0; q < NC; ++q) acc[q] = acc[q] + p[q] * t;
-            }
-        }
-        if (j == a.clamp_hi) {
-            const int64_t ib = a.clamp_hi - r + 1 > g_lo ? a.clamp_hi - r + 1 : g_lo;
-            for (int64_t i = ib; i <= g_hi; ++i) {
-                const T t = a.tail[i + r - a.clamp_hi];
-                const T* p = a.src + (soff + (i - a.src_lo[ax]) * gs) * NC;
-#pragma unroll
-                for (int q = 0; q < NC; ++q) acc[q] = acc[q] + p[q] * t;
-            }
-        }
-        T* out = a.dst + doff * NC;
-#pragma unroll
-        for (int q = 0; q < NC; ++q) out[q] = a.accumulate ? out[q] + acc[q] : acc[q];
-    }
-}
-
-inline unsigned grid_for(int64_t total) {
-    int64_t b = (total + 255) / 256;
-    const int64_t cap = int64_t(1) << 20;
-    return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
-}
 
 // Dynamic LDS of the row kernel (4 waves).
 inline size_t row_lds_bytes(int cpt, int r, int nc, size_t elem) {
@@ -300,8 +303,8 @@ inline size_t row_lds_bytes(int cpt, int r, int nc, size_t elem) {
     return (size_t)4 * ((span - 1) / cpt + 1) * (cpt + 1) * nc * elem;
 }
 
-template <typename T, int NC, bool FAST, int CPT>
-hipError_t fwd_launch(const AxisPassArgs<T>& a, hipStream_t s) {
+template <typename T, int NC, bool FAST, bool ADJ, int CPT>
+hipError_t pass_launch(const AxisPassArgs<T>& a, hipStream_t s) {
     const int64_t other = a.total / a.n[a.axis];
     int64_t items;
     if (a.axis == 0) {
@@ -309,35 +312,35 @@ hipError_t fwd_launch(const AxisPassArgs<T>& a, hipStream_t s) {
         items = ((a.n[0] + seg - 1) / seg) * other;
         const size_t lds = row_lds_bytes(CPT, a.radius, NC, sizeof(T));
         const dim3 grid((unsigned)((items + 3) / 4)), block(256);
-        hipLaunchKernelGGL((axis_row_kernel<T, NC, FAST, CPT>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((axis_row_kernel<T, NC, FAST, ADJ, CPT>), grid, block, lds, s, a);
     } else {
         items = ((a.n[0] + 63) / 64) * ((a.n[a.axis] + CPT - 1) / CPT) * (other / a.n[0]);
         const dim3 grid((unsigned)((items + 3) / 4)), block(256);
-        hipLaunchKernelGGL((axis_strided_kernel<T, NC, FAST, CPT>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((axis_strided_kernel<T, NC, FAST, ADJ, CPT>), grid, block, 0, s, a);
     }
     return hipGetLastError();
 }
 
-template <typename T, int NC, bool FAST>
-hipError_t fwd_cpt(const AxisPassArgs<T>& a, hipStream_t s) {
+template <typename T, int NC, bool FAST, bool ADJ>
+hipError_t pass_cpt(const AxisPassArgs<T>& a, hipStream_t s) {
     int cpt = 8;  // outputs per lane of a run
     // the row kernel's staging must fit the 64 KiB a workgroup gets by default
     while (a.axis == 0 && cpt > 1 && row_lds_bytes(cpt, a.radius, NC, sizeof(T)) > 65536) cpt /= 2;
     switch (cpt) {
-        case 8: return fwd_launch<T, NC, FAST, 8>(a, s);
-        case 4: return fwd_launch<T, NC, FAST, 4>(a, s);
-        case 2: return fwd_launch<T, NC, FAST, 2>(a, s);
-        default: return fwd_launch<T, NC, FAST, 1>(a, s);
+        case 8: return pass_launch<T, NC, FAST, ADJ, 8>(a, s);
+        case 4: return pass_launch<T, NC, FAST, ADJ, 4>(a, s);
+        case 2: return pass_launch<T, NC, FAST, ADJ, 2>(a, s);
+        default: return pass_launch<T, NC, FAST, ADJ, 1>(a, s);
     }
 }
 
-template <typename T, bool FAST>
-hipError_t fwd_nc(const AxisPassArgs<T>& a, int ncomp, hipStream_t s) {
+template <typename T, bool FAST, bool ADJ>
+hipError_t pass_nc(const AxisPassArgs<T>& a, int ncomp, hipStream_t s) {
     switch (ncomp) {
-        case 1: return fwd_cpt<T, 1, FAST>(a, s);
-        case 2: return fwd_cpt<T, 2, FAST>(a, s);
-        case 3: return fwd_cpt<T, 3, FAST>(a, s);
-        case 4: return fwd_cpt<T, 4, FAST>(a, s);
+        case 1: return pass_cpt<T, 1, FAST, ADJ>(a, s);
+        case 2: return pass_cpt<T, 2, FAST, ADJ>(a, s);
+        case 3: return pass_cpt<T, 3, FAST, ADJ>(a, s);
+        case 4: return pass_cpt<T, 4, FAST, ADJ>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -348,22 +351,19 @@ template <typename T>
 hipError_t launch_axis_forward(const AxisPassArgs<T>& a, int ncomp, itkg_arith arith,
                                hipStream_t s) {
     if (a.total == 0) return hipSuccess;
-    return arith == ITKG_FAST ? fwd_nc<T, true>(a, ncomp, s) : fwd_nc<T, false>(a, ncomp, s);
+    return arith == ITKG_FAST ? pass_nc<T, true, false>(a, ncomp, s)
+                              : pass_nc<T, false, false>(a, ncomp, s);
 }
 
+// The adjoint pass runs one fused multiply-add per tap.
 template <typename T>
 hipError_t launch_axis_adjoint_range(const AxisPassArgs<T>& a, int ncomp, int64_t g_lo,
                                      int64_t g_hi, hipStream_t s) {
     if (a.total == 0) return hipSuccess;
-    const dim3 grid(grid_for(a.total)), block(256);
-    switch (ncomp) {
-        case 1: hipLaunchKernelGGL((axis_adjoint_kernel<T, 1>), grid, block, 0, s, a, g_lo, g_hi); break;
-        case 2: hipLaunchKernelGGL((axis_adjoint_kernel<T, 2>), grid, block, 0, s, a, g_lo, g_hi); break;
-        case 3: hipLaunchKernelGGL((axis_adjoint_kernel<T, 3>), grid, block, 0, s, a, g_lo, g_hi); break;
-        case 4: hipLaunchKernelGGL((axis_adjoint_kernel<T, 4>), grid, block, 0, s, a, g_lo, g_hi); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    AxisPassArgs<T> b = a;
+    b.g_lo = g_lo;
+    b.g_hi = g_hi;
+    return pass_nc<T, true, true>(b, ncomp, s);
 }
 
 template hipError_t launch_axis_forward<float>(const AxisPassArgs<float>&, int, itkg_arith,

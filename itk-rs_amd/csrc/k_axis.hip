@@ -89,6 +89,14 @@ __device__ __forceinline__ void conv_run(const T* w, const int r, Load&& load, T
     if (jb < taps) block(jb, taps - jb);
 }
 
+// XCD-aware block order: hardware deals workgroups to the 8 XCDs round
+// robin, so block b works on item block (b mod 8) * ceil(G / 8) + b / 8 and
+// neighbouring runs (which share 2r inputs) read through one XCD's L2.
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t g = gridDim.x, b = blockIdx.x;
+    return (b % 8) * ((g + 7) / 8) + b / 8;
+}
+
 // Adjoint edge terms of one output at pass-axis position j, added to o:
 // at j = clamp_lo the outputs i < clamp_lo + r fold their clamped taps
 // onto it with weight tail[clamp_lo - i + r], at j = clamp_hi the outputs
@@ -133,7 +141,7 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
     const int ax = a.axis;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t item = (int64_t)blockIdx.x * 4 + wave;  // wave-uniform from here on
+    const int64_t item = xcd_block() * 4 + wave;  // wave-uniform from here on
     // the two axes that are neither 0 nor the pass axis
     const int b1 = ax == 1 ? 2 : 1;
     const int b2 = ax == 3 ? 2 : 3;
@@ -223,7 +231,7 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     const int span = 64 * CPT + 2 * r + CPT;
     const int per_wave = ((span - 1) / CPT + 1) * (CPT + 1);
     T* buf = reinterpret_cast<T*>(row_lds_raw) + (int64_t)wave * per_wave * NC;
-    const int64_t item = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t item = xcd_block() * 4 + wave;
     const int64_t nseg = (a.n[0] + 64 * CPT - 1) / (64 * CPT);
     int64_t it = item;
     const int64_t is = it % nseg;
@@ -311,11 +319,11 @@ hipError_t pass_launch(const AxisPassArgs<T>& a, hipStream_t s) {
         const int64_t seg = 64 * CPT;
         items = ((a.n[0] + seg - 1) / seg) * other;
         const size_t lds = row_lds_bytes(CPT, a.radius, NC, sizeof(T));
-        const dim3 grid((unsigned)((items + 3) / 4)), block(256);
+        const dim3 grid((unsigned)(((items + 3) / 4 + 7) / 8 * 8)), block(256);
         hipLaunchKernelGGL((axis_row_kernel<T, NC, FAST, ADJ, CPT>), grid, block, lds, s, a);
     } else {
         items = ((a.n[0] + 63) / 64) * ((a.n[a.axis] + CPT - 1) / CPT) * (other / a.n[0]);
-        const dim3 grid((unsigned)((items + 3) / 4)), block(256);
+        const dim3 grid((unsigned)(((items + 3) / 4 + 7) / 8 * 8)), block(256);
         hipLaunchKernelGGL((axis_strided_kernel<T, NC, FAST, ADJ, CPT>), grid, block, 0, s, a);
     }
     return hipGetLastError();

@@ -15,19 +15,17 @@
 // (terms outside g_out are zero).  The transposed passes commute in exact
 // arithmetic, so they run in the same x, y, z order as the forward.
 //
-// Workgroup: NW waves.  It owns a 64 x TY destination tile in x/y and a run
-// of TZ destination planes, and walks the source planes the run needs in z
-// order, one step per plane (planes clamped at the image edge are filtered
-// again; adjoint planes outside the source are zeros, so every step is the
-// same).  Per step:
-//   - the x pass of the step's plane from one half of a double-buffered LDS
-//     window ((TY+2r) x (64+2r), filled by coalesced row loads; forward:
-//     clamped at load time, adjoint: zero outside);
-//   - the next plane's window, loaded into registers two steps earlier, is
-//     stored into the other half, and the loads of the plane two steps ahead
-//     are issued (two register sets alternate, the step loop is unrolled by
-//     two, so the loads have two steps to arrive);
-//   - the y pass (CPT+2r LDS values per CPT outputs down a column);
+// Workgroup: NW waves.  It owns a 64 x TY destination tile in x/y (one
+// component of it, for interleaved pixels) and a run of TZ destination
+// planes, and walks the source planes the run needs in z order, one step
+// per plane (planes clamped at the image edge are filtered again; adjoint
+// planes outside the source are zeros, so every step is the same).  The
+// (TY+2r) x (64+2r) source window of a plane is loaded as a flattened array
+// with per-lane offsets computed once per tile, into one of two register
+// sets two steps ahead of its use.  Per step, with both LDS buffers (the
+// window and the x-filtered tile) double-buffered:
+//   - the y pass of plane u (CPT+2r LDS values per CPT outputs down a
+//     column);
 //   - one z step: every in-flight output of the lane's columns receives its
 //     next tap (scatter form), so each output sums taps k = 0..2r in order --
 //     EXACT: fl(v * w_k) added to an accumulator that starts at +0;
@@ -35,14 +33,17 @@
 //     slots: the slot that opens (zeroed) and the one that completes are
 //     selected by a uniform index (one indexed register move each), and the
 //     weights rotate instead of the data;
-//   - the completed output plane is stored.
+//   - the completed output plane is stored;
+//   - plane u+2 goes from its register set into the window buffer plane u
+//     has left, and the set is reloaded with plane u+4;
+//   - the x pass of plane u+1;
+//   - one LDS-only barrier.
 #pragma once
 
 #include <type_traits>
 #include <utility>
 
 #include "itkg_impl.hpp"
-
 
 namespace itkg {
 

@@ -142,6 +142,13 @@ def main():
 
     extras = {}
     if not args.no_extras:
+        # reference point: one device copy of the same 512 MiB (read + write),
+        # the rate a bare stream of these bytes reaches on this device
+        _, msc = timed_loop(lambda: out.copy_(x), args.steps, 2, stream)
+        copy_gbs = 8.0 * n / (float(np.mean(msc)) * 1e-3) / 1e9
+        extras["device_copy_512MiB"] = {"gb_s": round(copy_gbs, 1),
+                                        "hbm_frac": round(copy_gbs / HBM_PEAK_GBS, 4),
+                                        "headline_vs_copy": round(achieved / copy_gbs, 4)}
         # the other arithmetic mode on the same volume
         other = itk.FAST if arith == itk.EXACT else itk.EXACT
         f2 = itk.GaussianFilter(img, sigma)

@@ -619,20 +619,8 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     }
 }
 
-// Minimum resident waves per SIMD the register allocator must keep.  For
-// the FMA modes (forward FAST, adjoint) of the radii listed, capping the
-// kernel at 128 VGPRs (two 8-wave workgroups per CU) costs at most a few
-// dwords of spill and pays for itself in latency hiding; the other radii
-// and EXACT (whose ring needs more registers) run unconstrained.
-template <int RZ, int MODE>
-struct WavesPerEu {
-    static constexpr bool cap =
-        MODE != FUSED_FWD_EXACT && RZ >= 4 && (RZ <= 7 || (RZ >= 11 && RZ <= 13));
-    static constexpr int value = cap ? 4 : 1;
-};
-
 template <int RXY, int RZ, int MODE>
-__global__ __launch_bounds__((Cfg<RXY, RZ, MODE>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
+__global__ __launch_bounds__((Cfg<RXY, RZ, MODE>::NTH))
 void fused3d_kernel([[maybe_unused]] const FusedMulti args) {
     using C = Cfg<RXY, RZ, MODE>;
     // The argument blocks are read in place from the kernel-argument segment
@@ -659,7 +647,7 @@ void fused3d_kernel([[maybe_unused]] const FusedMulti args) {
 // volumes of one radius: workgroup b takes items b, b + G, b + 2G, ...
 // The host interleaves the list so that items b and b + G share an XCD.
 template <int RXY, int RZ, int MODE>
-__global__ __launch_bounds__((Cfg<RXY, RZ, MODE>::NTH)) __attribute__((amdgpu_waves_per_eu(WavesPerEu<RZ, MODE>::value)))
+__global__ __launch_bounds__((Cfg<RXY, RZ, MODE>::NTH))
 void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __restrict__ work,
                           const int nwork) {
     using C = Cfg<RXY, RZ, MODE>;

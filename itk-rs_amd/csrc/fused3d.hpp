@@ -21,9 +21,10 @@
 // per plane (planes clamped at the image edge are filtered again; adjoint
 // planes outside the source are zeros, so every step is the same).  The
 // (TY+2r) x (64+2r) source window of a plane is loaded as a flattened array
-// with per-lane offsets computed once per tile, into one of two register
-// sets two steps ahead of its use.  Per step, with both LDS buffers (the
-// window and the x-filtered tile) double-buffered:
+// with per-lane offsets computed once per tile, into one of NSET register
+// sets (2, or 4 for forward FAST at small radii), NSET steps before it is
+// stored to LDS.  Per step, with both LDS buffers (the window and the
+// x-filtered tile) double-buffered:
 //   - the y pass of plane u (CPT+2r LDS values per CPT outputs down a
 //     column);
 //   - one z step: every in-flight output of the lane's columns receives its
@@ -35,7 +36,8 @@
 //     weights rotate instead of the data;
 //   - the completed output plane is stored;
 //   - plane u+2 goes from its register set into the window buffer plane u
-//     has left, and the set is reloaded with plane u+4;
+//     has left, and the set is reloaded with plane u+2+NSET (with 4 sets,
+//     after the x pass, which then has the set's registers);
 //   - the x pass of plane u+1;
 //   - one LDS-only barrier.
 // The adjoint at small radii (Cfg::EARLY) issues the LDS reads of its first
@@ -128,6 +130,10 @@ struct Cfg {
     // are issued before the z step.  Measured 4-6% faster for the adjoint;
     // no gain for forward FAST, and larger radii would spill.
     static constexpr bool EARLY = MODE == FUSED_ADJ && RXY <= 7 && RZ <= 7;
+    // Register sets of window loads: a plane is loaded NSET steps before it
+    // is stored to LDS.  4 for forward FAST up to radius 6 (4% faster at
+    // r = 6; the registers fit, 126 VGPRs at r = 6), else 2.
+    static constexpr int NSET = MODE == FUSED_FWD_FAST && RXY <= 6 && RZ <= 6 ? 4 : 2;
     static_assert(WX - 64 <= 64, "halo wider than one wave");
     static_assert((2 * PLANE + 2 * XPLANE) * 4 <= 160 * 1024, "LDS budget of one workgroup");
 };
@@ -505,7 +511,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     // in sIn half (u + 1) % 2 and plane u + 2 in register set `r`:
     //   y pass of plane u, one z step, store of the completed output plane;
     //   plane u + 2 into sIn half u % 2 (its plane u was read before the last
-    //   barrier), `r` reloaded with plane u + 4;
+    //   barrier), `r` reloaded with plane u + 2 + NSET;
     //   x pass of plane u + 1 into sX half (u + 1) % 2 (read by step u - 1,
     //   before the last barrier);
     //   one barrier.
@@ -584,13 +590,21 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
             lds_barrier_tracked();
         } else {
             store_plane(r, h, u + 2);
-            load_plane(r, u + 4);
-            x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
+            if constexpr (C::NSET == 4) {
+                // the x pass while `r` is free (the register peak), then the
+                // reload of `r`
+                x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
+                __builtin_amdgcn_sched_barrier(0);
+                load_plane(r, u + 2 + C::NSET);
+            } else {
+                load_plane(r, u + 4);
+                x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
+            }
             lds_barrier();
         }
     };
 
-    float ra[LJ], rb[LJ];
+    float ra[LJ], rb[LJ], rc[LJ], rd[LJ];
     if constexpr (C::EARLY) {
         pin_sgprs(wxr.w);
         pin_sgprs(wyr.w);
@@ -601,7 +615,19 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     store_plane(ra, 0, 0);
     store_plane(rb, 1, 1);
     load_plane(ra, 2);
-    load_plane(rb, 3);
+    if constexpr (C::NSET == 4) {
+        // each set's loads kept together, in plane order, as the loop issues
+        // them (the compiler's load waits in the loop assume that order)
+        __builtin_amdgcn_sched_barrier(0);
+        load_plane(rb, 3);
+        __builtin_amdgcn_sched_barrier(0);
+        load_plane(rc, 4);
+        __builtin_amdgcn_sched_barrier(0);
+        load_plane(rd, 5);
+        __builtin_amdgcn_sched_barrier(0);
+    } else {
+        load_plane(rb, 3);
+    }
     lds_barrier();
     if constexpr (C::EARLY) {
         x_read(sIn);
@@ -612,10 +638,19 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         x_pass(sIn, sX);
         lds_barrier();
     }
-    // An even number of steps: the extra last step (if any) stores nothing.
-    for (int u = 0; u < U; u += 2) {
-        step(u, ra, 0);
-        step(u + 1, rb, 1);
+    // Whole rounds of NSET steps: the extra last steps (if any) store nothing.
+    if constexpr (C::NSET == 4) {
+        for (int u = 0; u < U; u += 4) {
+            step(u, ra, 0);
+            step(u + 1, rb, 1);
+            step(u + 2, rc, 0);
+            step(u + 3, rd, 1);
+        }
+    } else {
+        for (int u = 0; u < U; u += 2) {
+            step(u, ra, 0);
+            step(u + 1, rb, 1);
+        }
     }
 }
 

@@ -22,8 +22,8 @@
 // planes outside the source are zeros, so every step is the same).  The
 // (TY+2r) x (64+2r) source window of a plane is loaded as a flattened array
 // with per-lane offsets computed once per tile, into one of NSET register
-// sets (2, or 4 for forward FAST at small radii), NSET steps before it is
-// stored to LDS.  Per step, with both LDS buffers (the window and the
+// sets (2, or 4 where the registers allow), NSET steps before it is stored
+// to LDS.  Per step, with both LDS buffers (the window and the
 // x-filtered tile) double-buffered:
 //   - the y pass of plane u (CPT+2r LDS values per CPT outputs down a
 //     column);
@@ -131,9 +131,15 @@ struct Cfg {
     // no gain for forward FAST, and larger radii would spill.
     static constexpr bool EARLY = MODE == FUSED_ADJ && RXY <= 7 && RZ <= 7;
     // Register sets of window loads: a plane is loaded NSET steps before it
-    // is stored to LDS.  4 for forward FAST up to radius 6 (4% faster at
-    // r = 6; the registers fit, 126 VGPRs at r = 6), else 2.
-    static constexpr int NSET = MODE == FUSED_FWD_FAST && RXY <= 6 && RZ <= 6 ? 4 : 2;
+    // is stored to LDS.  4 where the registers fit without spilling and it
+    // measured faster: forward FAST with 64-row tiles up to radius 6 (126
+    // VGPRs at r = 6; 4% at r = 6) and the 32-row shapes, forward and
+    // adjoint, up to radius 10 (2-3%); 2 elsewhere.
+    static constexpr int NSET =
+        (MODE == FUSED_FWD_FAST && RXY <= 6 && RZ <= 6) ||
+                ((MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S) && RXY <= 10 && RZ <= 10)
+            ? 4
+            : 2;
     static_assert(WX - 64 <= 64, "halo wider than one wave");
     static_assert((2 * PLANE + 2 * XPLANE) * 4 <= 160 * 1024, "LDS budget of one workgroup");
 };
@@ -585,7 +591,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         if constexpr (C::EARLY) {
             x_finish(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
             store_plane(r, h, u + 2);
-            load_plane(r, u + 4);
+            load_plane(r, u + 2 + C::NSET);
             load_wz();
             lds_barrier_tracked();
         } else {

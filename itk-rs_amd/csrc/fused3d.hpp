@@ -133,11 +133,12 @@ struct Cfg {
     // Register sets of window loads: a plane is loaded NSET steps before it
     // is stored to LDS.  4 where the registers fit without spilling and it
     // measured faster: forward FAST with 64-row tiles up to radius 6 (126
-    // VGPRs at r = 6; 4% at r = 6) and the 32-row shapes, forward and
-    // adjoint, up to radius 10 (2-3%); 2 elsewhere.
+    // VGPRs at r = 6; 4% at r = 6) and the FMA modes with 32-row tiles (2
+    // columns per lane), forward and adjoint, up to radius 10 (2-3%); 2
+    // elsewhere.
     static constexpr int NSET =
-        (MODE == FUSED_FWD_FAST && RXY <= 6 && RZ <= 6) ||
-                ((MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S) && RXY <= 10 && RZ <= 10)
+        MODE != FUSED_FWD_EXACT && RXY <= 10 && RZ <= 10 &&
+                (CPT == 2 || (MODE == FUSED_FWD_FAST && RXY <= 6 && RZ <= 6))
             ? 4
             : 2;
     static_assert(WX - 64 <= 64, "halo wider than one wave");

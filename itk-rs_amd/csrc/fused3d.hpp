@@ -216,12 +216,25 @@ __device__ __forceinline__ float tap_sum(const float* v, const Weights<N>& w) {
 
 // One destination tile (64 x TY x TZ) of one volume.  The caller's previous
 // tile, if any, may still be using the LDS buffers (the tile starts with a
-// barrier).
-template <int RXY, int RZ, int MODE>
+// barrier).  PERSIST: called from the persistent batch kernel, whose work
+// loop holds more registers: there 64-row tiles keep two load sets, the
+// adjoint keeps the plain step order and all outputs take plain stores.
+template <int RXY, int RZ, int MODE, bool PERSIST>
 __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int tile, float* sIn,
                                              float* sX) {
     using C = Cfg<RXY, RZ, MODE>;
     constexpr int CPT = C::CPT, P = C::P, LJ = C::LJ;
+    // The persistent kernel uses the register-hungry options only where its
+    // registers allow them (no spills): the non-temporal dense stores and the
+    // opaque load offsets in the FMA modes up to radius 6 and with 32-row
+    // tiles up to radius 10; two load sets and the plain step order
+    // elsewhere in 64-row tiles.
+    constexpr bool FMA = MODE != FUSED_FWD_EXACT;
+    constexpr bool PFIT = FMA && ((RXY <= 6 && RZ <= 6) || (CPT == 2 && RXY <= 10 && RZ <= 10));
+    constexpr int NSET = PERSIST && CPT == 4 ? 2 : C::NSET;
+    constexpr bool EARLY = C::EARLY && !PERSIST;
+    constexpr bool PNT = !PERSIST || PFIT;   // non-temporal dense stores
+    constexpr bool PASM = !PERSIST || PFIT;  // opaque load offsets (scalar-base loads)
     constexpr bool ADJOINT = fused_is_adjoint(MODE);
     const int tx_i = tile % a.tiles_x;
     const int ty_i = (tile / a.tiles_x) % a.tiles_y;
@@ -259,7 +272,13 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         const int zr = clampi(zv0 + u, 0, nsz - 1);
         const char* plane = src_bytes + (int64_t)zr * a.src_stride[2] * (int64_t)sizeof(float);
 #pragma unroll
-        for (int j = 0; j < LJ; ++j) r[j] = *reinterpret_cast<const float*>(plane + gofs[j]);
+        for (int j = 0; j < LJ; ++j) {
+            // the offset made opaque at every call, so that its widening is
+            // not hoisted out of the loop: the load keeps the scalar-base +
+            // 32-bit-offset form (no 64-bit address per lane and load)
+            if constexpr (PASM) asm volatile("" : "+v"(gofs[j]));
+            r[j] = *reinterpret_cast<const float*>(plane + gofs[j]);
+        }
     };
     float* const sIn_lane = sIn + wave * 64 + lane;
     auto store_plane = [&](const float* r, int half, int u) {
@@ -475,7 +494,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         for (int i = 0; i < CPT + 2 * RXY; ++i) v[i] = xp[(wave * CPT + i) * C::SX + lane];
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-            if constexpr (C::EARLY) val[c] = tap_sum<RXY, MODE>(v + c, wyr);
+            if constexpr (EARLY) val[c] = tap_sum<RXY, MODE>(v + c, wyr);
             else val[c] = tap_sum<RXY, MODE>(v + c, wy);
             if constexpr (ADJOINT) {
                 if (y_edge && ybase + c == 0) {
@@ -492,12 +511,15 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         }
     };
 
-    // ---- output column of this lane
+    // ---- output column of this lane.  Dense rows (x stride 1) are written
+    // non-temporally at a wave-uniform base (scalar registers) plus the
+    // lane's byte offset.
     const int ox = x0 + lane;
     const bool x_ok = ox < ndx;
     const int64_t dxs = a.dst_stride[0], dys = a.dst_stride[1], dzs = a.dst_stride[2];
-    const bool dense_x = dxs == 1;
-    float* dcol = a.dst + (int64_t)ox * dxs + (int64_t)ybase * dys;
+    const bool store_nt = PNT && dxs == 1 && !(ADJOINT && a.accumulate);  // tile-uniform
+    float* const dwave = a.dst + x0 + (int64_t)ybase * dys;  // used when dxs == 1
+    uint32_t dlane = (uint32_t)lane * (uint32_t)sizeof(float);
 
     // ---- z ring: fixed slots, rotating weights (wz2 = wz repeated twice)
     static_assert(CPT <= 8, "at most 8 ring columns per lane");
@@ -538,7 +560,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         y_pass(sX + h * C::XPLANE);
         // EARLY: the first x-pass round's LDS reads go out now, so that the
         // z step and the stores cover their latency.
-        if constexpr (C::EARLY) x_read(sIn + (1 - h) * C::PLANE);
+        if constexpr (EARLY) x_read(sIn + (1 - h) * C::PLANE);
         const int sn = (s + 1 == P) ? 0 : s + 1;
         const float* w = a.wz2 + s + P;  // w[-j] = wz[(s - j) mod P]
         // Adjoint z-edge weights of this plane (uniform scalar loads; zero
@@ -559,7 +581,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                 if constexpr (MODE == FUSED_FWD_EXACT) {
                     const float prod = val[c] * w[-j];
                     acc[j] = acc[j] + prod;
-                } else if constexpr (C::EARLY) {
+                } else if constexpr (EARLY) {
                     acc[j] = __builtin_fmaf(val[c], wzs[j], acc[j]);
                 } else {
                     acc[j] = __builtin_fmaf(val[c], w[-j], acc[j]);
@@ -575,34 +597,48 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         });
         s = sn;
         if (t >= 0 && t < tz_act) {
-            float* dst = dcol + (int64_t)(z0 + t) * dzs;
+            if (store_nt) {
+                // dense output, non-temporal (keeps L2 for the input halos);
+                // the lane offset is opaque here so that its widening stays
+                // at the stores, which take the scalar-base + 32-bit-offset
+                // form
+                asm volatile("" : "+v"(dlane));
+                char* const dplane = reinterpret_cast<char*>(dwave + (int64_t)(z0 + t) * dzs);
 #pragma unroll
-            for (int c = 0; c < CPT; ++c) {
-                if (x_ok && ybase + c < ndy) {
-                    const float q = o[c];
-                    if (ADJOINT && a.accumulate)
-                        dst[c * dys] = dst[c * dys] + q;
-                    else if (dense_x)  // streamed output: keep L2 for the input halos
-                        __builtin_nontemporal_store(q, &dst[c * dys]);
-                    else  // one component of interleaved pixels: L2 merges the lines
-                        dst[c * dys] = q;
+                for (int c = 0; c < CPT; ++c)
+                    if (x_ok && ybase + c < ndy)
+                        __builtin_nontemporal_store(o[c], reinterpret_cast<float*>(dplane + (int64_t)c * dys * 4 + dlane));
+            } else {
+                // one component of interleaved pixels (L2 merges the partial
+                // lines), or the adjoint's accumulation; its own address
+                // arithmetic, so that the compiler cannot merge these stores
+                // with the non-temporal ones (a merge drops the hint)
+                float* const dcol = a.dst + (int64_t)ox * dxs + (int64_t)ybase * dys + (int64_t)(z0 + t) * dzs;
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    if (x_ok && ybase + c < ndy) {
+                        if (ADJOINT && a.accumulate)
+                            dcol[c * dys] = dcol[c * dys] + o[c];
+                        else
+                            dcol[c * dys] = o[c];
+                    }
                 }
             }
         }
-        if constexpr (C::EARLY) {
+        if constexpr (EARLY) {
             x_finish(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
             store_plane(r, h, u + 2);
-            load_plane(r, u + 2 + C::NSET);
+            load_plane(r, u + 2 + NSET);
             load_wz();
             lds_barrier_tracked();
         } else {
             store_plane(r, h, u + 2);
-            if constexpr (C::NSET == 4) {
+            if constexpr (NSET == 4) {
                 // the x pass while `r` is free (the register peak), then the
                 // reload of `r`
                 x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
                 __builtin_amdgcn_sched_barrier(0);
-                load_plane(r, u + 2 + C::NSET);
+                load_plane(r, u + 2 + NSET);
             } else {
                 load_plane(r, u + 4);
                 x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
@@ -612,7 +648,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     };
 
     float ra[LJ], rb[LJ], rc[LJ], rd[LJ];
-    if constexpr (C::EARLY) {
+    if constexpr (EARLY) {
         pin_sgprs(wxr.w);
         pin_sgprs(wyr.w);
     }
@@ -622,7 +658,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     store_plane(ra, 0, 0);
     store_plane(rb, 1, 1);
     load_plane(ra, 2);
-    if constexpr (C::NSET == 4) {
+    if constexpr (NSET == 4) {
         // each set's loads kept together, in plane order, as the loop issues
         // them (the compiler's load waits in the loop assume that order)
         __builtin_amdgcn_sched_barrier(0);
@@ -636,7 +672,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         load_plane(rb, 3);
     }
     lds_barrier();
-    if constexpr (C::EARLY) {
+    if constexpr (EARLY) {
         x_read(sIn);
         x_finish(sIn, sX);
         load_wz();
@@ -646,7 +682,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         lds_barrier();
     }
     // Whole rounds of NSET steps: the extra last steps (if any) store nothing.
-    if constexpr (C::NSET == 4) {
+    if constexpr (NSET == 4) {
         for (int u = 0; u < U; u += 4) {
             step(u, ra, 0);
             step(u + 1, rb, 1);
@@ -682,7 +718,7 @@ void fused3d_kernel([[maybe_unused]] const FusedMulti args) {
     const int item = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (item >= nt) return;
     const int comp = item % nc;
-    fused3d_tile<RXY, RZ, MODE>(m.a[comp], item / nc, sIn, sX);
+    fused3d_tile<RXY, RZ, MODE, false>(m.a[comp], item / nc, sIn, sX);
 }
 
 // Persistent form over a packed work list of (volume, tile) items for many
@@ -698,7 +734,7 @@ void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __re
     for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int2 it = work[w];
         if (it.x < 0) continue;  // padding of the interleaved list
-        fused3d_tile<RXY, RZ, MODE>(vols[it.x], it.y, sIn, sX);
+        fused3d_tile<RXY, RZ, MODE, true>(vols[it.x], it.y, sIn, sX);
     }
 }
 

@@ -300,6 +300,18 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     // barrier (a scalar load still in flight at the loop head would make
     // every step's first LDS wait a full drain).
     [[maybe_unused]] const Weights<2 * RXY + 1> wxr(a.wx), wyr(a.wy);
+    // The adjoint with 32-row tiles is short of scalar registers and, up to
+    // radius 8, has vector registers to spare: its y weights live in vector
+    // registers there (made lane-varying to the compiler by an empty asm).
+    constexpr bool WYV = ADJOINT && CPT == 2 && !EARLY && RXY <= 8 && RZ <= 8;
+    [[maybe_unused]] float wyv[2 * RXY + 1];
+    if constexpr (WYV) {
+#pragma unroll
+        for (int k = 0; k <= 2 * RXY; ++k) {
+            wyv[k] = a.wy[k];
+            asm volatile("" : "+v"(wyv[k]));
+        }
+    }
 
     // Adjoint edge terms exist only in tiles that hold an image edge (tile-
     // uniform tests, so interior tiles skip them).
@@ -495,6 +507,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
             if constexpr (EARLY) val[c] = tap_sum<RXY, MODE>(v + c, wyr);
+            else if constexpr (WYV) val[c] = tap_sum<RXY, MODE>(v + c, wyv);
             else val[c] = tap_sum<RXY, MODE>(v + c, wy);
             if constexpr (ADJOINT) {
                 if (y_edge && ybase + c == 0) {

@@ -387,13 +387,16 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                     for (int i = 0; i <= 2 * RXY; ++i) v[i] = rp[i];
                     float o = tap_sum<RXY, MODE>(v, wx);
                     if constexpr (ADJOINT) {
-                        if (x_edge && x0 + lane == 0) {
+                        // one test; only the (at most two) edge lanes enter
+                        if (x_edge && (x0 + lane == 0 || x0 + lane == ndx - 1)) {
+                            if (x0 + lane == 0) {
 #pragma unroll
-                            for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[2 * RXY - m], o);
-                        }
-                        if (x_edge && x0 + lane == ndx - 1) {
+                                for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[2 * RXY - m], o);
+                            }
+                            if (x0 + lane == ndx - 1) {
 #pragma unroll
-                            for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[m], o);
+                                for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[m], o);
+                            }
                         }
                     }
                     dstp[row * C::SX + lane] = o;
@@ -480,13 +483,16 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                     for (int i = 0; i <= 2 * RXY; ++i) v[i] = rp[i];
                     float o = tap_sum<RXY, MODE>(v, wxr);
                     if constexpr (ADJOINT) {
-                        if (x_edge && x0 + lane == 0) {
+                        // one test; only the (at most two) edge lanes enter
+                        if (x_edge && (x0 + lane == 0 || x0 + lane == ndx - 1)) {
+                            if (x0 + lane == 0) {
 #pragma unroll
-                            for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[2 * RXY - m], o);
-                        }
-                        if (x_edge && x0 + lane == ndx - 1) {
+                                for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[2 * RXY - m], o);
+                            }
+                            if (x0 + lane == ndx - 1) {
 #pragma unroll
-                            for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[m], o);
+                                for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[m], o);
+                            }
                         }
                     }
                     dstp[row * C::SX + lane] = o;
@@ -509,16 +515,23 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
             if constexpr (EARLY) val[c] = tap_sum<RXY, MODE>(v + c, wyr);
             else if constexpr (WYV) val[c] = tap_sum<RXY, MODE>(v + c, wyv);
             else val[c] = tap_sum<RXY, MODE>(v + c, wy);
-            if constexpr (ADJOINT) {
-                if (y_edge && ybase + c == 0) {
+        }
+        if constexpr (ADJOINT) {
+            // y-edge terms: one wave-uniform test per step for the waves
+            // away from the edges
+            if (y_edge) {
 #pragma unroll
-                    for (int m = 1; m <= RXY; ++m)
-                        val[c] = __builtin_fmaf(a.py[m], v[c + 2 * RXY - m], val[c]);
-                }
-                if (y_edge && ybase + c == ndy - 1) {
+                for (int c = 0; c < CPT; ++c) {
+                    if (ybase + c == 0) {
 #pragma unroll
-                    for (int m = 1; m <= RXY; ++m)
-                        val[c] = __builtin_fmaf(a.py[m], v[c + m], val[c]);
+                        for (int m = 1; m <= RXY; ++m)
+                            val[c] = __builtin_fmaf(a.py[m], v[c + 2 * RXY - m], val[c]);
+                    }
+                    if (ybase + c == ndy - 1) {
+#pragma unroll
+                        for (int m = 1; m <= RXY; ++m)
+                            val[c] = __builtin_fmaf(a.py[m], v[c + m], val[c]);
+                    }
                 }
             }
         }
@@ -600,13 +613,23 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                     acc[j] = __builtin_fmaf(val[c], w[-j], acc[j]);
                 }
             }
-            if constexpr (ADJOINT) {
-                if (z_edge) {  // plo / phi are uniform: only the steps that carry a term
-                    if (plo != 0.0f) acc[0] = __builtin_fmaf(val[c], plo, acc[0]);
-                    if (phi != 0.0f) acc[s_hi] = __builtin_fmaf(val[c], phi, acc[s_hi]);
-                }
+        });
+        if constexpr (ADJOINT) {
+            // z-edge terms, all columns behind one uniform test per step (a
+            // test per column splits the step into blocks the scheduler
+            // cannot interleave); plo / phi are zero outside the windows
+            if (z_edge && (plo != 0.0f || phi != 0.0f)) {
+                static_for<CPT>([&](auto ci) {
+                    constexpr int c = decltype(ci)::value;
+                    float* acc = ring_of<c>(r0, r1, r2, r3, r4, r5, r6, r7);
+                    acc[0] = __builtin_fmaf(val[c], plo, acc[0]);
+                    acc[s_hi] = __builtin_fmaf(val[c], phi, acc[s_hi]);
+                });
             }
-            o[c] = acc[sn];  // tap 2r has just been added
+        }
+        static_for<CPT>([&](auto ci) {
+            constexpr int c = decltype(ci)::value;
+            o[c] = ring_of<c>(r0, r1, r2, r3, r4, r5, r6, r7)[sn];  // tap 2r has just been added
         });
         s = sn;
         if (t >= 0 && t < tz_act) {

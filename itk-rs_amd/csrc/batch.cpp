@@ -444,6 +444,8 @@ int itkg_batch_autotune(itkg_batch* plan, void* stream) {
                 tot += ms;
             }
             if (rc) break;
+            if (std::getenv("ITKG_DEBUG"))
+                std::fprintf(stderr, "itkg: autotune r=%d mode=%d tz=%d: %.4f ms\n", c.r, m, tz, tot / 3.0f);
             if (tot < best_ms) {
                 best_ms = tot;
                 best = tz;

@@ -247,31 +247,32 @@ int device_cus() {
 }
 
 // z run of a fused tile.  A workgroup walks tz + 2r source planes, plus a
-// short prologue; the grid runs in ceil(tiles / slots) rounds where slots =
-// CUs x resident workgroups per CU.  The z runs are the equal splits
-// ceil(nz / k); the one with the fewest steps on the critical path wins (the
-// smaller tz on ties: shorter tails).  ITKG_FUSED_TZ overrides.
-int fused_tz(const FusedEntry* fe, int mode, int tiles_xy, int nz, int r) {
+// short prologue.  A CU works through ceil(tiles / CUs) tiles: workgroups
+// resident on one CU together share it and finish no sooner than one after
+// the other (measured: two 32-row workgroups per CU at 512^3, r = 3, take
+// 24% longer than one 64-row workgroup covering the same outputs).  The z
+// runs are the equal splits ceil(nz / k); the one with the fewest plane
+// steps per CU wins (the smaller tz on ties: shorter tails).
+// ITKG_FUSED_TZ overrides.
+int fused_tz(int tiles_xy, int nz, int r) {
     if (const char* e = std::getenv("ITKG_FUSED_TZ")) return std::max(1, std::atoi(e));
     nz = std::max(nz, 1);
-    const int occ = fe && fe->plain_occupancy ? fe->plain_occupancy(mode) : 1;
-    const int64_t slots = (int64_t)device_cus() * std::max(occ, 1);
+    const int64_t cus = device_cus();
     int best_tz = nz;
     int64_t best = INT64_MAX;
     for (int k = 1; k <= std::min(nz, 1024); ++k) {
         const int tz = (nz + k - 1) / k;
         if (k > 1 && tz == (nz + k - 2) / (k - 1)) continue;  // same split as k - 1
         const int64_t tiles = (int64_t)tiles_xy * ((nz + tz - 1) / tz);
-        const int64_t rounds = (tiles + slots - 1) / slots;
-        const int64_t cost = rounds * (tz + 2 * r + 3);
+        const int64_t per_cu = (tiles + cus - 1) / cus;
+        const int64_t cost = per_cu * (tz + 2 * r + 3);
         if (cost <= best) {
             best = cost;
             best_tz = tz;
         }
     }
     if (std::getenv("ITKG_DEBUG"))
-        std::fprintf(stderr, "itkg: fused r=%d mode=%d occupancy=%d slots=%lld tiles_xy=%d nz=%d -> tz=%d\n",
-                     r, mode, occ, (long long)slots, tiles_xy, nz, best_tz);
+        std::fprintf(stderr, "itkg: fused r=%d tiles_xy=%d nz=%d -> tz=%d\n", r, tiles_xy, nz, best_tz);
     return best_tz;
 }
 
@@ -335,16 +336,16 @@ void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& 
     const int TY = fe ? fe->ty[mode] : 32;
     A.tiles_x = (A.n_dst[0] + kFusedTileX - 1) / kFusedTileX;
     A.tiles_y = (A.n_dst[1] + TY - 1) / TY;
-    A.tz = tz > 0 ? tz : fused_tz(fe, mode, A.tiles_x * A.tiles_y * ncomp, A.n_dst[2], rz);
+    A.tz = tz > 0 ? tz : fused_tz(A.tiles_x * A.tiles_y * ncomp, A.n_dst[2], rz);
     A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
 }
 
 // Tile shape of a FAST or adjoint launch: 64-row tiles (fewer halo rows per
 // output) or 32-row tiles (twice the tiles, so longer z runs fill the GPU
-// on smaller volumes).  Modelled cost: rounds of the grid x plane steps per
-// tile x work per step, which grows with the tile's rows plus their x halo
-// plus a fixed per-step part (loads, barrier).  ITKG_FUSED_SHAPE=64|32
-// overrides.
+// on smaller volumes).  Modelled cost: tiles per CU x plane steps per tile x
+// work per step, which grows with the tile's rows plus their x halo plus a
+// fixed per-step part (loads, barrier); the z run of each shape as
+// fused_tz picks it.  ITKG_FUSED_SHAPE=64|32 overrides.
 int pick_shape(const Plan& p, const FusedEntry* fe, int mode, int ncomp) {
     if (mode != FUSED_FWD_FAST && mode != FUSED_ADJ) return mode;
     const int small = mode == FUSED_FWD_FAST ? FUSED_FWD_FAST_S : FUSED_ADJ_S;
@@ -355,11 +356,11 @@ int pick_shape(const Plan& p, const FusedEntry* fe, int mode, int ncomp) {
     auto cost = [&](int m) {
         const int ty = fe->ty[m];
         const int64_t txy = ((nx + kFusedTileX - 1) / kFusedTileX) * ((ny + ty - 1) / ty) * ncomp;
-        const int tz = fused_tz(fe, m, (int)std::min<int64_t>(txy, INT32_MAX), (int)nz, rz);
+        const int tz = fused_tz((int)std::min<int64_t>(txy, INT32_MAX), (int)nz, rz);
         const int64_t tiles = txy * ((nz + tz - 1) / tz);
-        const int64_t slots = (int64_t)device_cus() * std::max(1, fe->plain_occupancy(m));
-        const double rounds = (double)((tiles + slots - 1) / slots);
-        return rounds * (double)(std::min<int64_t>(tz, nz) + 2 * rz + 3) * (3.0 * ty + 2.0 * rxy + 30.0);
+        const int64_t cus = device_cus();
+        const double per_cu = (double)((tiles + cus - 1) / cus);
+        return per_cu * (double)(std::min<int64_t>(tz, nz) + 2 * rz + 3) * (3.0 * ty + 2.0 * rxy + 30.0);
     };
     const int pick = cost(small) < cost(mode) ? small : mode;
     if (std::getenv("ITKG_DEBUG"))

@@ -843,33 +843,24 @@ hipError_t launch_batch_r(const Fused3dArgs* vols, const int2* work, int nwork, 
     return hipGetLastError();
 }
 
-// Resident workgroups per CU of the plain (persist = false) or persistent
-// batch (persist = true) kernel.
-template <int RXY, int RZ, bool PERSIST>
+// Resident workgroups per CU of the persistent batch kernel.
+template <int RXY, int RZ>
 int occupancy_r(int mode) {
     int blocks = 0;
     hipError_t e = hipErrorInvalidValue;
     auto query = [&](auto kern, int nth) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, nth, 0);
     };
-    constexpr int n0 = Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH, n1 = Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH,
-                  n2 = Cfg<RXY, RZ, FUSED_ADJ>::NTH, n3 = Cfg<RXY, RZ, FUSED_FWD_FAST_S>::NTH,
-                  n4 = Cfg<RXY, RZ, FUSED_ADJ_S>::NTH;
     if (mode == FUSED_FWD_EXACT)
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>, n0)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT>, n0);
+        e = query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_EXACT>, Cfg<RXY, RZ, FUSED_FWD_EXACT>::NTH);
     else if (mode == FUSED_FWD_FAST)
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST>, n1)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_FAST>, n1);
+        e = query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST>, Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH);
     else if (mode == FUSED_ADJ)
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>, n2)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_ADJ>, n2);
+        e = query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ>, Cfg<RXY, RZ, FUSED_ADJ>::NTH);
     else if (mode == FUSED_FWD_FAST_S)
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST_S>, n3)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_FWD_FAST_S>, n3);
+        e = query(fused3d_batch_kernel<RXY, RZ, FUSED_FWD_FAST_S>, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::NTH);
     else if (mode == FUSED_ADJ_S)
-        e = PERSIST ? query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ_S>, n4)
-                    : query(fused3d_kernel<RXY, RZ, FUSED_ADJ_S>, n4);
+        e = query(fused3d_batch_kernel<RXY, RZ, FUSED_ADJ_S>, Cfg<RXY, RZ, FUSED_ADJ_S>::NTH);
     return e == hipSuccess && blocks > 0 ? blocks : 1;
 }
 
@@ -880,8 +871,7 @@ constexpr FusedEntry make_entry() {
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ_S>::TY},
-                      &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>,
-                      &occupancy_r<RXY, RZ, true>, &occupancy_r<RXY, RZ, false>};
+                      &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>, &occupancy_r<RXY, RZ>};
 }
 template <int RXY, int RZ>
 constexpr FusedEntry make_entry_single() {
@@ -889,8 +879,7 @@ constexpr FusedEntry make_entry_single() {
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ_S>::TY},
-                      &launch_r<RXY, RZ>, nullptr, nullptr,
-                      &occupancy_r<RXY, RZ, false>};
+                      &launch_r<RXY, RZ>, nullptr, nullptr};
 }
 
 }  // namespace

@@ -97,8 +97,7 @@ struct FusedEntry {
     hipError_t (*launch)(const FusedMulti& m, int mode, hipStream_t s);
     hipError_t (*batch)(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                         int mode, hipStream_t s);
-    int (*occupancy)(int mode);        // resident workgroups per CU of the persistent form
-    int (*plain_occupancy)(int mode);  // resident workgroups per CU of the plain form
+    int (*occupancy)(int mode);  // resident workgroups per CU of the persistent form
 };
 constexpr int kFusedTileX = 64;
 // The instantiation for (rxy, rz), or null.
@@ -124,7 +123,7 @@ size_t scratch_bytes(const Plan& p, itkg_dtype dt, int ncomp);
 bool use_fused(const Plan& p, itkg_dtype dt, int ncomp);
 const FusedEntry* fused_for(const Plan& p);
 // z run of a fused tile for a grid of tiles_xy tiles per z run over nz planes.
-int fused_tz(const FusedEntry* fe, int mode, int tiles_xy, int nz, int r);
+int fused_tz(int tiles_xy, int nz, int r);
 // True when the fused kernel can address the source view of this call (in-
 // plane offsets are 32-bit).
 bool fused_src_ok(const Plan& p, const itkg_view& src, int ncomp, bool adjoint);

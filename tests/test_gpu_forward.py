@@ -267,3 +267,27 @@ def test_fused_tile_shapes_agree(itk, orc, dev, monkeypatch, sigma):
     assert torch.equal(grads["64"], grads["32"])
     ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, [0, 0, 0], [130, 90, 37])
     assert _rel_err(outs["64"].cpu().numpy(), ref) <= FAST_TOL
+
+
+@pytest.mark.parametrize("sigma", [1.2, 2.0])
+def test_fused_tile_shapes_agree_interleaved(itk, dev, monkeypatch, sigma):
+    """Three interleaved components (VecPixel<f32,3>): the strided output
+    path of both tile shapes and of the per-axis kernels agrees bit for bit
+    (x = 200: a partial last tile)."""
+    rng = np.random.default_rng(32)
+    arr = rng.random((21, 40, 200, 3)).astype(np.float32)
+    x = torch.from_numpy(arr).to(dev)
+    outs = {}
+    for shape in ("64", "32"):
+        monkeypatch.setenv("ITKG_FUSED_SHAPE", shape)
+        outs[shape] = itk.gaussian(x, sigma, (1.0, 1.0, 1.0), ndim=3, arith=itk.FAST)
+    torch.cuda.synchronize()
+    assert torch.equal(outs["64"], outs["32"])
+    monkeypatch.delenv("ITKG_FUSED_SHAPE")
+    per = {}
+    itk.set_path(itk.PATH_PER_AXIS)
+    try:
+        per = itk.gaussian(x, sigma, (1.0, 1.0, 1.0), ndim=3, arith=itk.FAST)
+    finally:
+        itk.set_path(itk.PATH_AUTO)
+    assert torch.equal(outs["64"], per)  # the per-axis FAST kernels follow the same FMA chain

@@ -129,11 +129,12 @@ struct BatchImpl {
         graphs.clear();
     }
 
-    // Modelled time of one group at z run tz, in plane steps per slot: the
+    // Modelled time of one group at z run tz, in plane steps per CU: the
     // persistent grid deals the tiles out evenly, and the last tile dealt can
-    // finish up to one tile's walk after the others.
+    // finish up to one tile's walk after the others.  Workgroups resident
+    // together on one CU share it (they add no throughput).
     double model_cost(const BatchClass& c, int tz, int m) const {
-        const int64_t slots = (int64_t)num_cu * std::max(1, c.entry->occupancy(m));
+        const int64_t slots = num_cu;
         double steps = 0.0, longest = 0.0;
         for (int i : c.vol) {
             const itkg_volume& v = vols[i];

@@ -45,6 +45,15 @@
 // latency, and keeps all weights in scalar registers, with the next step's z
 // weights loaded before the barrier: a scalar load is waited for with a full
 // lgkmcnt drain, which would also wait for the LDS reads in flight.
+//
+// Addressing: window loads and dense output stores take a scalar base plus a
+// 32-bit lane offset (the offsets are made opaque to the compiler at each
+// use, so that it does not hoist their widening into 64-bit per-lane
+// addresses).  Dense output is written non-temporally; interleaved
+// components and the adjoint's accumulation take plain stores on a separate
+// address path (a store merged from both kinds would lose the hint).  The
+// persistent batch kernel holds more registers and uses these options, the
+// 4 load sets and EARLY only where they fit without spilling.
 #pragma once
 
 #include <type_traits>
@@ -75,8 +84,8 @@ __device__ __forceinline__ void lds_barrier_tracked() {
 //   FAST and adjoint: 16 waves; 4 columns (64-row tiles) up to rz 7, then 2;
 //   their _S variants: 16 waves x 2 columns (32-row tiles) at every radius;
 //   EXACT (a larger register need: the product and the sum round
-//   separately): 8 waves x 4 columns up to rz 7; 16 x 2 at rz 8 and 15;
-//   8 x 4 (rz 9, 10) or 8 x 2 (rz 11..14), where 16 waves would spill.
+//   separately): 16 waves x 2 columns up to rz 8 and at rz 15; 8 x 4
+//   (rz 9, 10) or 8 x 2 (rz 11..14), where 16 waves would spill.
 template <int RXY, int RZ, int MODE>
 struct Cfg {
     static constexpr bool EXACT_MID = MODE == FUSED_FWD_EXACT && RZ >= 9 && RZ <= 14;

@@ -109,6 +109,7 @@ __device__ __forceinline__ void adjoint_edges(const AxisPassArgs<T>& a, int64_t 
     const int r = a.radius;
     if (j == a.clamp_lo) {
         const int64_t ie = a.clamp_lo + r - 1 < a.g_hi ? a.clamp_lo + r - 1 : a.g_hi;
+#pragma unroll 4
         for (int64_t i = a.g_lo; i <= ie; ++i) {
             T v[NC];
             at(i, v);
@@ -119,6 +120,7 @@ __device__ __forceinline__ void adjoint_edges(const AxisPassArgs<T>& a, int64_t 
     }
     if (j == a.clamp_hi) {
         const int64_t ib = a.clamp_hi - r + 1 > a.g_lo ? a.clamp_hi - r + 1 : a.g_lo;
+#pragma unroll 4
         for (int64_t i = ib; i <= a.g_hi; ++i) {
             T v[NC];
             at(i, v);
@@ -184,7 +186,23 @@ __global__ __launch_bounds__(256) void axis_strided_kernel([[maybe_unused]] cons
         for (int q = 0; q < NC; ++q) v[q] = p[q];
     };
     T acc[CPT][NC];
-    conv_run<T, NC, FAST, CPT>(a.w, r, load, acc);
+    if constexpr (ADJ) {
+        // Runs whose inputs 0 .. 2r + CPT all lie inside the gradient's
+        // extent (all but the runs at the two ends of the axis) read without
+        // the per-tap zero test, whose branches would otherwise hold each
+        // block's loads to the block that issues them.
+        auto load_in = [&](int jb, auto off, T (&v)[NC]) {
+            const T* p = src + (int64_t)(jb + decltype(off)::value) * sa;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) v[q] = p[q];
+        };
+        if (lo_k <= 0 && hi_k >= 2 * r + CPT)  // wave-uniform
+            conv_run<T, NC, FAST, CPT>(a.w, r, load_in, acc);
+        else
+            conv_run<T, NC, FAST, CPT>(a.w, r, load, acc);
+    } else {
+        conv_run<T, NC, FAST, CPT>(a.w, r, load, acc);
+    }
     const int64_t j0 = a.out_lo[ax] + ia * CPT;  // pass-axis position of output 0
     if constexpr (ADJ) {
         if (j0 <= a.clamp_lo + CPT - 1 || j0 + CPT - 1 >= a.clamp_hi) {  // uniform
@@ -279,14 +297,48 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     const int64_t x0 = is * 64 * CPT + base;  // output index of acc[0] along axis 0
     if constexpr (ADJ) {
         const int64_t j0 = a.out_lo[0] + x0;
-        if (j0 <= a.clamp_lo + CPT - 1 || j0 + CPT - 1 >= a.clamp_hi) {  // at most two lanes
-            auto at = [&](int64_t i, T (&v)[NC]) {
-                const T* p = src + (i - first) * s0;
+        // Edge terms (the two lanes holding the outputs at clamp_lo and
+        // clamp_hi), in one loop for both lanes: each adds its terms to its
+        // edge output in the order of adjoint_edges.  The gradient positions
+        // they read lie within r of the edge output, inside this segment's
+        // staged span [first, first + span): they are read from LDS.
+        const int64_t lo_end = a.clamp_lo + r - 1 < a.g_hi ? a.clamp_lo + r - 1 : a.g_hi;
+        const int64_t hi_beg = a.clamp_hi - r + 1 > a.g_lo ? a.clamp_hi - r + 1 : a.g_lo;
+        const int64_t clo = a.clamp_lo - j0, chi = a.clamp_hi - j0;  // edge output index in the run
+        const bool has_lo = clo >= 0 && clo < CPT, has_hi = chi >= 0 && chi < CPT;
+        if (has_lo || has_hi) {  // at most two lanes
+            // a lane holding both edges (an axis of at most CPT pixels) does
+            // them one after the other, as adjoint_edges does
+            auto edge = [&](const int64_t c, const int64_t i0, const int n, const int t0, const int dt) {
+                T o[NC];
 #pragma unroll
-                for (int q = 0; q < NC; ++q) v[q] = p[q];
+                for (int q = 0; q < NC; ++q) o[q] = acc[0][q];
+#pragma unroll
+                for (int cc = 1; cc < CPT; ++cc)
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) o[q] = c == cc ? acc[cc][q] : o[q];
+                const T* p0 = buf;
+#pragma unroll 4
+                for (int k = 0; k < n; ++k) {
+                    const T* p = p0 + staged<CPT>((int)(i0 + k - first)) * NC;
+                    const T t = a.tail[t0 + dt * k];
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) o[q] = tap<T, true>(o[q], p[q], t);
+                }
+#pragma unroll
+                for (int cc = 0; cc < CPT; ++cc)
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) acc[cc][q] = c == cc ? o[q] : acc[cc][q];
             };
-#pragma unroll
-            for (int c = 0; c < CPT; ++c) adjoint_edges<T, NC>(a, j0 + c, at, acc[c]);
+            // lo edge: i = g_lo .. lo_end, weight tail[clamp_lo - i + r];
+            // hi edge: i = hi_beg .. g_hi, weight tail[i + r - clamp_hi].
+            // The lane's first edge is chosen by value, so that the two edge
+            // lanes run the same loop.
+            const int n_lo = (int)(lo_end - a.g_lo + 1), n_hi = (int)(a.g_hi - hi_beg + 1);
+            const int t_lo = (int)(a.clamp_lo - a.g_lo + r), t_hi = (int)(hi_beg + r - a.clamp_hi);
+            edge(has_lo ? clo : chi, has_lo ? a.g_lo : hi_beg, has_lo ? n_lo : n_hi,
+                 has_lo ? t_lo : t_hi, has_lo ? -1 : 1);
+            if (has_lo && has_hi) edge(chi, hi_beg, n_hi, t_hi, 1);
         }
     }
     T* dst = a.dst + ((a.out_lo[0] + x0 - a.dst_lo[0]) * a.dst_stride[0] +
@@ -315,22 +367,30 @@ template <typename T, int NC, bool FAST, bool ADJ, int CPT>
 hipError_t pass_launch(const AxisPassArgs<T>& a, hipStream_t s) {
     const int64_t other = a.total / a.n[a.axis];
     int64_t items;
-    if (a.axis == 0) {
-        const int64_t seg = 64 * CPT;
-        items = ((a.n[0] + seg - 1) / seg) * other;
-        const size_t lds = row_lds_bytes(CPT, a.radius, NC, sizeof(T));
-        const dim3 grid((unsigned)(((items + 3) / 4 + 7) / 8 * 8)), block(256);
-        hipLaunchKernelGGL((axis_row_kernel<T, NC, FAST, ADJ, CPT>), grid, block, lds, s, a);
-    } else {
-        items = ((a.n[0] + 63) / 64) * ((a.n[a.axis] + CPT - 1) / CPT) * (other / a.n[0]);
-        const dim3 grid((unsigned)(((items + 3) / 4 + 7) / 8 * 8)), block(256);
-        hipLaunchKernelGGL((axis_strided_kernel<T, NC, FAST, ADJ, CPT>), grid, block, 0, s, a);
+    if constexpr (CPT <= 8) {
+        if (a.axis == 0) {
+            const int64_t seg = 64 * CPT;
+            items = ((a.n[0] + seg - 1) / seg) * other;
+            const size_t lds = row_lds_bytes(CPT, a.radius, NC, sizeof(T));
+            const dim3 grid((unsigned)(((items + 3) / 4 + 7) / 8 * 8)), block(256);
+            hipLaunchKernelGGL((axis_row_kernel<T, NC, FAST, ADJ, CPT>), grid, block, lds, s, a);
+            return hipGetLastError();
+        }
     }
+    items = ((a.n[0] + 63) / 64) * ((a.n[a.axis] + CPT - 1) / CPT) * (other / a.n[0]);
+    const dim3 grid((unsigned)(((items + 3) / 4 + 7) / 8 * 8)), block(256);
+    hipLaunchKernelGGL((axis_strided_kernel<T, NC, FAST, ADJ, CPT>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
 template <typename T, int NC, bool FAST, bool ADJ>
 hipError_t pass_cpt(const AxisPassArgs<T>& a, hipStream_t s) {
+#ifndef ITKG_S16
+#define ITKG_S16 8
+#endif
+    if constexpr (std::is_same_v<T, float> && NC == 1) {
+        if (a.axis != 0 && a.radius >= ITKG_S16) return pass_launch<T, NC, FAST, ADJ, 16>(a, s);
+    }
     int cpt = 8;  // outputs per lane of a run
     // the row kernel's staging must fit the 64 KiB a workgroup gets by default
     while (a.axis == 0 && cpt > 1 && row_lds_bytes(cpt, a.radius, NC, sizeof(T)) > 65536) cpt /= 2;

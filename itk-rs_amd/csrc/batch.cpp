@@ -69,7 +69,7 @@ std::map<TuneKey, std::pair<int, int>> read_cache() {
     while (std::getline(f, line)) {
         std::istringstream ss(line);
         int d, a, r, tz, ty;
-        if (ss >> d >> a >> r >> tz >> ty && tz > 0) m[{d, a, r}] = {tz, ty};
+        if (ss >> d >> a >> r >> tz >> ty && tz >= 0) m[{d, a, r}] = {tz, ty};
     }
     return m;
 }
@@ -182,7 +182,12 @@ struct BatchImpl {
         for (int i : c.vol) {
             const itkg_volume& v = vols[i];
             const double nz = (double)v.size[2];
-            const double tzv = std::min<double>(c.tz, nz);
+            const int tz = c.tz > 0 ? c.tz
+                                    : fused_tz((int)(((v.size[0] + kFusedTileX - 1) / kFusedTileX) *
+                                                     ((v.size[1] + c.entry->ty[c.mode] - 1) /
+                                                      c.entry->ty[c.mode])),
+                                               (int)v.size[2], c.r);
+            const double tzv = std::min<double>(tz, nz);
             w += (double)v.size[0] * v.size[1] * nz * voxel_cost(c.r, true) *
                  (tzv + 2 * c.r + 3) / tzv;
         }
@@ -430,8 +435,9 @@ int itkg_batch_autotune(itkg_batch* plan, void* stream) {
         for (int m : {B.mode, B.small_mode()}) {
         if (m < 0) continue;
         c.mode = m;
-        for (int tz : {16, 24, 32, 48, 64, 96, 128, 192, 256}) {
-            if (tz < 2 * c.r || (tz > maxz && tz != 16)) continue;
+        // tz 0: each volume takes the z run of its own plain launch
+        for (int tz : {0, 16, 24, 32, 48, 64, 96, 128, 192, 256}) {
+            if (tz > maxz && tz != 16) continue;
             if ((rc = B.build_class(c, tz))) break;
             if ((rc = B.enqueue_class(c, s))) break;  // warm-up
             float tot = 0.0f;

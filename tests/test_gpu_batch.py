@@ -78,6 +78,32 @@ def test_batch_autotune_keeps_results(itk, dev, tmp_path, monkeypatch):
     plan2.close()
 
 
+@pytest.mark.parametrize("direction", [0, 1])
+def test_batch_per_volume_z_run(itk, dev, tmp_path, monkeypatch, direction):
+    """A cached z run of 0 (each volume takes its own plain launch's z run)
+    gives every volume's plain result, for both tile shapes."""
+    from itk_rs_amd import workloads as wl
+    vols, params = wl.c4(dev, nvol=8, lo=20, hi=120, seed=11)
+    arith = itk.FAST
+    for rows in (64, 32):
+        cache = tmp_path / ("tune%d_%d.txt" % (direction, rows))
+        cache.write_text("".join("%d %d %d 0 %d\n" % (direction, arith, r, rows)
+                                 for r in range(1, 16)))
+        monkeypatch.setenv("ITKG_TUNE_CACHE", str(cache))
+        outs = [torch.full_like(v, float("nan")) for v in vols]
+        plan = itk.BatchPlan(vols, outs, [p[1] for p in params], [p[2] for p in params],
+                             direction=direction, arith=arith)
+        s = torch.cuda.Stream()
+        for rep in range(2):
+            plan.run(s)
+            s.synchronize()
+            for v, o, (shape, sigma, sp) in zip(vols, outs, params):
+                ref = _plain(itk, v, sigma, sp, direction, arith)
+                torch.cuda.synchronize()
+                assert torch.equal(o, ref), (rows, rep, shape, sigma)
+        plan.close()
+
+
 def test_batched_adjoint_uneven_gradients(itk, dev):
     """The batched adjoint over uneven gradient volumes (sigma 1.8) equals the
     plain adjoint of each, before and after autotuning."""

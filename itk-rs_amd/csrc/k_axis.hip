@@ -267,8 +267,34 @@ __global__ __launch_bounds__(256) void axis_row_kernel([[maybe_unused]] const Ax
     const T* src = a.src + ((q1 - a.src_lo[1]) * a.src_stride[1] + (q2 - a.src_lo[2]) * a.src_stride[2] +
                             (q3 - a.src_lo[3]) * a.src_stride[3] + (first - a.src_lo[0]) * a.src_stride[0]) * NC;
     if (live) {
-#pragma unroll 4
-        for (int i = lane; i < span; i += 64) {
+        // All of a lane's loads are issued before its first LDS write (one
+        // memory latency per wave): KB rounds of 64 cover spans up to
+        // 64 * KB, i.e. radii up to 32 * (KB - CPT) - CPT / 2; longer spans
+        // finish in the loop after.  The loads need no guard (the clamp
+        // keeps every address inside the input extent); the writes past the
+        // span are skipped.
+        constexpr int KB = CPT + 2;
+        T v[KB][NC];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const int i = lane + 64 * k;
+            const int ic = i < lo_k ? lo_k : (i > hi_k ? hi_k : i);
+            const T* p = src + (int64_t)ic * s0;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) v[k][q] = p[q];
+        }
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+            const int i = lane + 64 * k;
+            if (i < span) {
+                const bool in = i >= lo_k && i <= hi_k;
+                T* d = buf + staged<CPT>(i) * NC;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) d[q] = (ADJ && !in) ? T(0) : v[k][q];
+            }
+        }
+#pragma unroll 2
+        for (int i = lane + 64 * KB; i < span; i += 64) {
             const bool in = i >= lo_k && i <= hi_k;
             const int ic = i < lo_k ? lo_k : (i > hi_k ? hi_k : i);
             const T* p = src + (int64_t)ic * s0;

@@ -133,6 +133,47 @@ int itkg_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t n
                                        const itkg_region* requested, void* output,
                                        itkg_arith arith);
 
+/* ---- the other convolve_axis filters ----------------------------------- */
+
+/* box_kernel, src/filters/mean.rs:15-18: 2*radius+1 weights of 1/(2r+1)
+ * (f64) written to `out` (capacity `cap`). */
+int itkg_box_kernel(uint64_t radius, double* out, uint64_t cap);
+
+/* MeanFilter / BoxMeanFilterN::input_region_for_output,
+ * src/filters/mean.rs:49-57 and 122-124 (and BinomialBlurFilter's,
+ * src/filters/binomial_blur.rs:46-55, with radii = repetitions):
+ * output.padded_per_axis(radii).clipped_to(largest). */
+int itkg_separable_input_region_for_output(int32_t ndim, const uint64_t* radii,
+                                           const itkg_region* largest, const itkg_region* output,
+                                           itkg_region* input);
+
+/* generate_region of a separable clamp-to-edge filter with caller-given
+ * weights: MeanFilter::generate_region (src/filters/mean.rs:59-83) with
+ * radii[d] = radius and box_kernel weights on every axis,
+ * BoxMeanFilterN::generate_region (mean.rs:126-144) with per-axis radii, and
+ * one repetition of BinomialBlurFilter::generate_region
+ * (src/filters/binomial_blur.rs:57-89: radius 1, weights 1/4, 1/2, 1/4, and
+ * the requested region of that repetition).  Each pass is the convolve_axis
+ * of src/filters/conv.rs:11-41 and the pass regions are those of
+ * itkg_gaussian_generate_region.
+ *
+ *   radii   : radius per axis (at most ITKG_MAX_RADIUS)
+ *   weights : axis 0's 2*radii[0]+1 f64 weights, then axis 1's, ...; each
+ *             axis symmetric (w_k == w_{2r-k}), else ITKG_ERR_INVALID_ARG;
+ *             cast to the pixel type at each use, as the reference does
+ *   input, output, largest, arith, stream: as itkg_gaussian_generate_region */
+int itkg_separable_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                   const uint64_t* radii, const double* weights,
+                                   const itkg_region* largest, const itkg_view* input,
+                                   const itkg_view* output, itkg_arith arith, void* stream);
+
+/* The same on dense host buffers, as itkg_gaussian_generate_region_host. */
+int itkg_separable_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                        const uint64_t* radii, const double* weights,
+                                        const itkg_region* largest, const void* input,
+                                        const itkg_region* requested, void* output,
+                                        itkg_arith arith);
+
 /* ---- adjoint (no counterpart in the reference) ------------------------- */
 
 /* Transpose of itkg_gaussian_generate_region for the same (largest, sigma,

@@ -38,6 +38,8 @@ EXPORTS = (
     "itkg_gaussian_backward", "itkg_batch_create", "itkg_batch_run", "itkg_batch_autotune",
     "itkg_batch_voxels",
     "itkg_batch_destroy", "itkg_last_error", "itkg_set_path", "itkg_version",
+    "itkg_box_kernel", "itkg_separable_input_region_for_output",
+    "itkg_separable_generate_region", "itkg_separable_generate_region_host",
 )
 
 
@@ -81,6 +83,15 @@ def _load():
         P(c_region), P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
     L.itkg_gaussian_generate_region_host.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
+        P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int]
+    L.itkg_box_kernel.argtypes = [ctypes.c_uint64, P(ctypes.c_double), ctypes.c_uint64]
+    L.itkg_separable_input_region_for_output.argtypes = [
+        ctypes.c_int32, P(ctypes.c_uint64), P(c_region), P(c_region), P(c_region)]
+    L.itkg_separable_generate_region.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_int32, P(ctypes.c_uint64), P(ctypes.c_double),
+        P(c_region), P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
+    L.itkg_separable_generate_region_host.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_int32, P(ctypes.c_uint64), P(ctypes.c_double),
         P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int]
     L.itkg_gaussian_backward.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
@@ -412,6 +423,169 @@ def stream_full(source, num_pieces: int) -> Image:
 
 
 # ---------------------------------------------------------------------------
+# the other convolve_axis filters (src/filters/mean.rs, binomial_blur.rs)
+# ---------------------------------------------------------------------------
+
+def box_kernel(radius: int) -> List[float]:
+    """box_kernel (src/filters/mean.rs:15-18): 2r+1 weights of 1/(2r+1)."""
+    n = 2 * int(radius) + 1
+    out = (ctypes.c_double * n)()
+    _check(lib.itkg_box_kernel(int(radius), out, n))
+    return list(out)
+
+
+BINOMIAL_KERNEL = (0.25, 0.5, 0.25)  # src/filters/binomial_blur.rs:35
+
+
+def _separable_input_region(radii: Sequence[int], largest: Region, output_region: Region) -> Region:
+    D = output_region.ndim
+    out = c_region()
+    _check(lib.itkg_separable_input_region_for_output(
+        D, (ctypes.c_uint64 * D)(*[int(r) for r in radii]), ctypes.byref(largest.to_c()),
+        ctypes.byref(output_region.to_c()), ctypes.byref(out)))
+    return Region(list(out.index)[:D], list(out.size)[:D])
+
+
+def _separable_into(radii: Sequence[int], weights: Sequence[Sequence[float]], largest: Region,
+                    inp: Image, out: Image, arith: int) -> None:
+    """One generate_region of a separable clamp-to-edge filter (per-axis
+    radii and weights) from `inp` into `out` (itkg_separable_generate_region)."""
+    D = out.ndim
+    dt, nc = _pixel_info(inp.data, D)
+    flat = [float(w) for ws in weights for w in ws]
+    with torch.cuda.device(inp.data.device):
+        _check(lib.itkg_separable_generate_region(
+            D, dt, nc, (ctypes.c_uint64 * D)(*[int(r) for r in radii]),
+            (ctypes.c_double * len(flat))(*flat), ctypes.byref(largest.to_c()),
+            ctypes.byref(inp.view()), ctypes.byref(out.view()), arith,
+            _stream_ptr(inp.data.device)))
+
+
+class _SeparableFilter(ImageSource):
+    """A separable filter with fixed per-axis radii and weights: the shape of
+    MeanFilter and BoxMeanFilterN (src/filters/mean.rs:42-83, 115-144)."""
+
+    arith = EXACT
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def _radii(self, D: int) -> List[int]:
+        raise NotImplementedError
+
+    def input_region_for_output(self, output_region: Region) -> Region:
+        return _separable_input_region(self._radii(output_region.ndim), self.largest_region(),
+                                       output_region)
+
+    def generate_region(self, requested: Region) -> Image:
+        radii = self._radii(requested.ndim)
+        input_region = requested.padded_per_axis(radii).clipped_to(self.largest_region())
+        inp = self.source.generate_region(input_region)
+        _, nc = _pixel_info(inp.data, requested.ndim)
+        out = Image.allocate(requested, self.spacing, self.origin, 0.0, nc, inp.data.dtype,
+                             inp.data.device)
+        self.generate_into(inp, out)
+        return out
+
+    def generate_into(self, inp: Image, out: Image) -> None:
+        """Write generate_region(out.region) into `out` from an upstream image
+        covering input_region_for_output(out.region)."""
+        radii = self._radii(out.ndim)
+        _separable_into(radii, [box_kernel(r) for r in radii], self.largest_region(), inp, out,
+                        self.arith)
+
+
+class MeanFilter(_SeparableFilter):
+    """MeanFilter<S> (src/filters/mean.rs:28-84): box mean of half-width
+    `radius` voxels on every axis, in index space; the same two public fields."""
+
+    def __init__(self, source, radius: int):
+        self.source = source
+        self.radius = int(radius)
+
+    def _radii(self, D: int) -> List[int]:
+        return [self.radius] * D
+
+
+class BoxMeanFilterN(_SeparableFilter):
+    """BoxMeanFilterN<S, D> (src/filters/mean.rs:102-145): box mean with a
+    half-width per axis."""
+
+    def __init__(self, source, radii: Sequence[int]):
+        self.source = source
+        self.radii = [int(r) for r in radii]
+
+    def _radii(self, D: int) -> List[int]:
+        if len(self.radii) != D:
+            raise ValueError(f"{len(self.radii)} radii for a {D}-D image")
+        return list(self.radii)
+
+
+class BinomialBlurFilter(ImageSource):
+    """BinomialBlurFilter<S> (src/filters/binomial_blur.rs:17-90): `repetitions`
+    rounds of the [1, 2, 1] / 4 kernel on every axis.  Each round is one
+    radius-1 separable generate_region whose requested region is the final
+    request padded by the rounds still to come (the per-pass regions of
+    binomial_blur.rs:70-85); rounds ping-pong between device images."""
+
+    arith = EXACT
+
+    def __init__(self, source, repetitions: int):
+        self.source = source
+        self.repetitions = int(repetitions)
+
+    @staticmethod
+    def effective_sigma_pixels(repetitions: int) -> float:
+        """binomial_blur.rs:29-31: sqrt(n / 4)."""
+        return (repetitions / 4.0) ** 0.5
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def input_region_for_output(self, output_region: Region) -> Region:
+        return _separable_input_region([self.repetitions] * output_region.ndim,
+                                       self.largest_region(), output_region)
+
+    def generate_region(self, requested: Region) -> Image:
+        D = requested.ndim
+        bounds = self.largest_region()
+        n = self.repetitions
+        cur = self.source.generate_region(requested.padded(n).clipped_to(bounds))
+        _, nc = _pixel_info(cur.data, D)
+        for rep in range(n):
+            remaining = n - rep - 1
+            reg = requested if remaining == 0 else requested.padded(remaining).clipped_to(bounds)
+            out = Image.allocate(reg, self.spacing, self.origin, 0.0, nc, cur.data.dtype,
+                                 cur.data.device)
+            _separable_into([1] * D, [BINOMIAL_KERNEL] * D, bounds, cur, out, self.arith)
+            cur = out
+        if n == 0:
+            # no rounds: the input region is the request itself (a copy, as
+            # the reference returns the upstream image)
+            out = Image.allocate(requested, self.spacing, self.origin, 0.0, nc, cur.data.dtype,
+                                 cur.data.device)
+            out.data.copy_(cur.data)
+            cur = out
+        return cur
+
+
+# ---------------------------------------------------------------------------
 # functional / autograd entry points
 # ---------------------------------------------------------------------------
 
@@ -456,6 +630,30 @@ def gaussian_backward(g: torch.Tensor, sigma: float, spacing=None, ndim: Optiona
             ctypes.byref(gimg.view()), ctypes.byref(oimg.view()), int(accumulate),
             _stream_ptr(g.device)))
     return out
+
+
+def mean_filter(x: torch.Tensor, radius, ndim: Optional[int] = None,
+                arith: int = EXACT) -> torch.Tensor:
+    """Whole-image MeanFilter{source: x, radius} (an int) or BoxMeanFilterN
+    {source: x, radii} (a sequence) .generate_region(largest)."""
+    nd = ndim if ndim is not None else x.dim()
+    src = _full_image(x.contiguous(), nd, (1.0,) * nd)
+    f = BoxMeanFilterN(src, radius) if isinstance(radius, (list, tuple)) else MeanFilter(src, radius)
+    f.arith = arith
+    out = Image.allocate(src.region, src.spacing, src.origin, 0.0,
+                         x.shape[-1] if x.dim() > nd else 1, x.dtype, x.device)
+    f.generate_into(src, out)
+    return out.data
+
+
+def binomial_blur(x: torch.Tensor, repetitions: int, ndim: Optional[int] = None,
+                  arith: int = EXACT) -> torch.Tensor:
+    """Whole-image BinomialBlurFilter{source: x, repetitions}.generate_region(largest)."""
+    nd = ndim if ndim is not None else x.dim()
+    src = _full_image(x.contiguous(), nd, (1.0,) * nd)
+    f = BinomialBlurFilter(src, repetitions)
+    f.arith = arith
+    return f.generate_region(src.region).data
 
 
 class GaussianFunction(torch.autograd.Function):

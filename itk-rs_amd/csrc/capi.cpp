@@ -108,23 +108,11 @@ void weights_of(double sigma_px, uint64_t radius, double* k) {
 
 // ---- per-call plan (src/filters/gaussian.rs:59-90) ----------------------
 
-int make_plan(int D, double sigma, const double* spacing, const itkg_region* largest,
-              const itkg_region& req, Plan* p) {
-    if (D < 1 || D > ITKG_MAX_DIM) return fail(ITKG_ERR_INVALID_ARG, "ndim %d not in [1, %d]", D, ITKG_MAX_DIM);
-    if (!spacing || !largest) return fail(ITKG_ERR_INVALID_ARG, "null spacing or largest region");
-    if (!region_ok(D, *largest) || !region_ok(D, req))
-        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
-    p->D = D;
-    p->bounds = *largest;
-    p->req = req;
-    for (int d = 0; d < D; ++d) {
-        const uint64_t r = radius_of(sigma, spacing[d]);
-        if (r > ITKG_MAX_RADIUS)
-            return fail(ITKG_ERR_UNSUPPORTED, "axis %d radius %llu exceeds %d", d,
-                        (unsigned long long)r, ITKG_MAX_RADIUS);
-        p->radii[d] = (int64_t)r;
-        weights_of(sigma / spacing[d], r, p->w[d]);
-    }
+// The regions of a plan whose radii are set: the input region (the request
+// padded by the radii, clipped to the largest region: gaussian.rs:52-57,
+// mean.rs:49-57) and each pass's output region (gaussian.rs:79-85,
+// mean.rs:71-80).
+int plan_regions(int D, const itkg_region* largest, const itkg_region& req, Plan* p) {
     if (region_len(D, req) > 0 && !contains(D, *largest, req))
         return fail(ITKG_ERR_INVALID_ARG, "requested region is not inside the largest region");
     p->in_reg = clipped(D, padded(D, req, p->radii), *largest);
@@ -139,6 +127,57 @@ int make_plan(int D, double sigma, const double* spacing, const itkg_region* lar
         }
     }
     return ITKG_OK;
+}
+
+int plan_common(int D, const itkg_region* largest, const itkg_region& req, Plan* p) {
+    if (D < 1 || D > ITKG_MAX_DIM) return fail(ITKG_ERR_INVALID_ARG, "ndim %d not in [1, %d]", D, ITKG_MAX_DIM);
+    if (!largest) return fail(ITKG_ERR_INVALID_ARG, "null largest region");
+    if (!region_ok(D, *largest) || !region_ok(D, req))
+        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
+    p->D = D;
+    p->bounds = *largest;
+    p->req = req;
+    return ITKG_OK;
+}
+
+int make_plan(int D, double sigma, const double* spacing, const itkg_region* largest,
+              const itkg_region& req, Plan* p) {
+    if (!spacing) return fail(ITKG_ERR_INVALID_ARG, "null spacing");
+    if (int rc = plan_common(D, largest, req, p)) return rc;
+    for (int d = 0; d < D; ++d) {
+        const uint64_t r = radius_of(sigma, spacing[d]);
+        if (r > ITKG_MAX_RADIUS)
+            return fail(ITKG_ERR_UNSUPPORTED, "axis %d radius %llu exceeds %d", d,
+                        (unsigned long long)r, ITKG_MAX_RADIUS);
+        p->radii[d] = (int64_t)r;
+        weights_of(sigma / spacing[d], r, p->w[d]);
+    }
+    return plan_regions(D, largest, req, p);
+}
+
+// A plan from caller-given radii and weights (axis 0's 2 radii[0] + 1
+// weights, then axis 1's, ...), as MeanFilter, BoxMeanFilterN and
+// BinomialBlurFilter hand convolve_axis their own kernels.  The weights must
+// be symmetric (w_k = w_{2r-k}): the adjoint kernels rely on it.
+int make_plan_weights(int D, const uint64_t* radii, const double* weights,
+                      const itkg_region* largest, const itkg_region& req, Plan* p) {
+    if (!radii || !weights) return fail(ITKG_ERR_INVALID_ARG, "null radii or weights");
+    if (int rc = plan_common(D, largest, req, p)) return rc;
+    const double* w = weights;
+    for (int d = 0; d < D; ++d) {
+        const uint64_t r = radii[d];
+        if (r > ITKG_MAX_RADIUS)
+            return fail(ITKG_ERR_UNSUPPORTED, "axis %d radius %llu exceeds %d", d,
+                        (unsigned long long)r, ITKG_MAX_RADIUS);
+        p->radii[d] = (int64_t)r;
+        for (uint64_t k = 0; k <= 2 * r; ++k) {
+            if (!(w[k] == w[2 * r - k]))
+                return fail(ITKG_ERR_INVALID_ARG, "axis %d weights are not symmetric", d);
+            p->w[d][k] = w[k];
+        }
+        w += 2 * r + 1;
+    }
+    return plan_regions(D, largest, req, p);
 }
 
 size_t dtype_size(itkg_dtype t) { return t == ITKG_F64 ? 8 : 4; }
@@ -464,6 +503,45 @@ int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout
                           : backward_per_axis<float>(p, ncomp, gout, gin, accumulate, s, scratch);
 }
 
+// Host buffers: stage the input region in, run the plan, stage the
+// requested region out, on a private stream (synchronous for the caller).
+int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, void* output,
+                 itkg_arith arith) {
+    const int ndim = p.D;
+    const int64_t nin = region_len(ndim, p.in_reg), nout = region_len(ndim, p.req);
+    if (nout == 0) return ITKG_OK;
+    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
+    int rc = ITKG_OK;
+    const size_t px = dtype_size(dtype) * ncomp;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    void *din = nullptr, *dout = nullptr;
+    e = hipMallocAsync(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "hipMallocAsync(host staging)");
+    } else {
+        e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy in");
+        if (!rc) {
+            const itkg_view vin = dense_view(ndim, din, p.in_reg);
+            const itkg_view vout = dense_view(ndim, dout, p.req);
+            rc = forward_impl(p, dtype, ncomp, vin, vout, arith, s, nullptr);
+        }
+        if (!rc) {
+            e = hipMemcpyAsync(output, dout, (size_t)nout * px, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) rc = hip_fail(e, "copy out");
+        }
+    }
+    if (din) (void)hipFreeAsync(din, s);
+    if (dout) (void)hipFreeAsync(dout, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
 bool args_ok(int ndim, itkg_dtype dt, int ncomp) {
     return ndim >= 1 && ndim <= ITKG_MAX_DIM && (dt == ITKG_F32 || dt == ITKG_F64) && ncomp >= 1 &&
            ncomp <= 4;
@@ -562,37 +640,66 @@ int itkg_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t n
     Plan p;
     int rc = make_plan(ndim, sigma, spacing, largest, *requested, &p);
     if (rc) return rc;
-    const int64_t nin = region_len(ndim, p.in_reg), nout = region_len(ndim, p.req);
-    if (nout == 0) return ITKG_OK;
-    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
-    const size_t px = dtype_size(dtype) * ncomp;
-    hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
-    void *din = nullptr, *dout = nullptr;
-    e = hipMallocAsync(&din, (size_t)nin * px, s);
-    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
-    if (e != hipSuccess) {
-        rc = hip_fail(e, "hipMallocAsync(host staging)");
-    } else {
-        e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) rc = hip_fail(e, "copy in");
-        if (!rc) {
-            const itkg_view vin = dense_view(ndim, din, p.in_reg);
-            const itkg_view vout = dense_view(ndim, dout, p.req);
-            rc = forward_impl(p, dtype, ncomp, vin, vout, arith, s, nullptr);
-        }
-        if (!rc) {
-            e = hipMemcpyAsync(output, dout, (size_t)nout * px, hipMemcpyDeviceToHost, s);
-            if (e != hipSuccess) rc = hip_fail(e, "copy out");
-        }
+    return host_forward(p, dtype, ncomp, input, output, arith);
+}
+
+int itkg_box_kernel(uint64_t radius, double* out, uint64_t cap) {
+    clear_error();
+    if (!out) return fail(ITKG_ERR_INVALID_ARG, "null output");
+    if (radius > (uint64_t)ITKG_MAX_RADIUS)
+        return fail(ITKG_ERR_UNSUPPORTED, "radius exceeds %d", ITKG_MAX_RADIUS);
+    if (cap < 2 * radius + 1) return fail(ITKG_ERR_INVALID_ARG, "capacity below 2r+1");
+    const uint64_t len = 2 * radius + 1;
+    for (uint64_t k = 0; k < len; ++k) out[k] = 1.0 / (double)len;
+    return ITKG_OK;
+}
+
+int itkg_separable_input_region_for_output(int32_t ndim, const uint64_t* radii,
+                                           const itkg_region* largest, const itkg_region* output,
+                                           itkg_region* input) {
+    clear_error();
+    if (ndim < 1 || ndim > ITKG_MAX_DIM || !radii || !largest || !output || !input)
+        return fail(ITKG_ERR_INVALID_ARG, "bad arguments");
+    int64_t r[ITKG_MAX_DIM] = {0, 0, 0, 0};
+    for (int d = 0; d < ndim; ++d)
+        r[d] = radii[d] > (uint64_t)INT64_MAX / 4 ? INT64_MAX / 4 : (int64_t)radii[d];
+    itkg_region out = clipped(ndim, padded(ndim, *output, r), *largest);
+    for (int d = ndim; d < ITKG_MAX_DIM; ++d) {
+        out.index[d] = 0;
+        out.size[d] = 0;
     }
-    if (din) (void)hipFreeAsync(din, s);
-    if (dout) (void)hipFreeAsync(dout, s);
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
-    return rc;
+    *input = out;
+    return ITKG_OK;
+}
+
+int itkg_separable_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                   const uint64_t* radii, const double* weights,
+                                   const itkg_region* largest, const itkg_view* input,
+                                   const itkg_view* output, itkg_arith arith, void* stream) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (!output) return fail(ITKG_ERR_INVALID_ARG, "null output view");
+    Plan p;
+    int rc = make_plan_weights(ndim, radii, weights, largest, output->region, &p);
+    if (rc) return rc;
+    if ((rc = check_view(ndim, input, p.in_reg, "input"))) return rc;
+    if ((rc = check_view(ndim, output, p.req, "output"))) return rc;
+    return forward_impl(p, dtype, ncomp, *input, *output, arith,
+                        static_cast<hipStream_t>(stream), nullptr);
+}
+
+int itkg_separable_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                        const uint64_t* radii, const double* weights,
+                                        const itkg_region* largest, const void* input,
+                                        const itkg_region* requested, void* output,
+                                        itkg_arith arith) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (!requested) return fail(ITKG_ERR_INVALID_ARG, "null requested region");
+    Plan p;
+    int rc = make_plan_weights(ndim, radii, weights, largest, *requested, &p);
+    if (rc) return rc;
+    return host_forward(p, dtype, ncomp, input, output, arith);
 }
 
 int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,

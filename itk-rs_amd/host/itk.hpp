@@ -9,6 +9,8 @@
 //                                    input_region_for_output / generate_region)
 //   SourceRef (the blanket &S impl)  src/source.rs:62-75
 //   GaussianFilter                   src/filters/gaussian.rs:13-91
+//   MeanFilter, BoxMeanFilterN       src/filters/mean.rs:15-145
+//   BinomialBlurFilter               src/filters/binomial_blur.rs:17-90
 //   stream_full / split_region       src/source.rs:81-97, src/region_splitter.rs:8-39
 //   VecPixel                         src/pixel.rs:61-99
 //
@@ -18,6 +20,7 @@
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -276,6 +279,146 @@ template <class S>
 GaussianFilter<S> make_gaussian(S src, double sigma) {
     return GaussianFilter<S>(std::move(src), sigma);
 }
+
+// ---- the other convolve_axis filters (src/filters/mean.rs,
+// src/filters/binomial_blur.rs) ------------------------------------------
+
+// Arithmetic mode of MeanFilter, BoxMeanFilterN and BinomialBlurFilter
+// (EXACT reproduces the reference bit for bit).
+inline itkg_arith separable_arith = ITKG_EXACT;
+
+// box_kernel (src/filters/mean.rs:15-18).
+inline std::vector<double> box_kernel(uint64_t radius) {
+    std::vector<double> w(2 * radius + 1);
+    check(itkg_box_kernel(radius, w.data(), w.size()));
+    return w;
+}
+
+// One generate_region of a separable clamp-to-edge filter with per-axis
+// radii and weights (axis 0's weights first), from `input` (covering the
+// padded request) to a new image of `requested`.
+template <class P, size_t D>
+Image<P, D> separable_generate(const std::array<uint64_t, D>& radii, const std::vector<double>& weights,
+                               const Region<D>& bounds, const Image<P, D>& input,
+                               const Region<D>& requested, const std::array<double, D>& spacing,
+                               const std::array<double, D>& origin, itkg_arith arith) {
+    Image<P, D> out = Image<P, D>::allocate(requested, spacing, origin, P{});
+    const itkg_region cb = bounds.to_c(), cr = requested.to_c();
+    check(itkg_separable_generate_region_host((int32_t)D, PixelTraits<P>::dtype, PixelTraits<P>::ncomp,
+                                              radii.data(), weights.data(), &cb, input.data.data(), &cr,
+                                              out.data.data(), arith));
+    return out;
+}
+
+// BoxMeanFilterN<S, D> (src/filters/mean.rs:102-145): box mean with a
+// half-width per axis.  MeanFilter below is the same with one radius.
+template <class S>
+struct BoxMeanFilterN {
+    using pixel_type = typename S::pixel_type;
+    static constexpr size_t dim = S::dim;
+    using P = pixel_type;
+    static constexpr size_t D = dim;
+
+    S source;
+    std::array<uint64_t, D> radii;
+
+    BoxMeanFilterN(S src, std::array<uint64_t, D> r) : source(std::move(src)), radii(r) {}
+
+    Region<D> largest_region() const { return source.largest_region(); }
+    std::array<double, D> spacing() const { return source.spacing(); }
+    std::array<double, D> origin() const { return source.origin(); }
+    Region<D> input_region_for_output(const Region<D>& output) const {
+        return output.padded_per_axis(radii).clipped_to(source.largest_region());
+    }
+    Image<P, D> generate_region(Region<D> requested) const {
+        const Region<D> bounds = source.largest_region();
+        const Image<P, D> input = source.generate_region(requested.padded_per_axis(radii).clipped_to(bounds));
+        std::vector<double> w;
+        for (size_t d = 0; d < D; ++d) {
+            const auto k = box_kernel(radii[d]);
+            w.insert(w.end(), k.begin(), k.end());
+        }
+        return separable_generate<P, D>(radii, w, bounds, input, requested, spacing(), origin(),
+                                        separable_arith);
+    }
+};
+
+// MeanFilter<S> (src/filters/mean.rs:28-84) with the reference's two public
+// fields.
+template <class S>
+struct MeanFilter {
+    using pixel_type = typename S::pixel_type;
+    static constexpr size_t dim = S::dim;
+    using P = pixel_type;
+    static constexpr size_t D = dim;
+
+    S source;
+    uint64_t radius;
+
+    MeanFilter(S src, uint64_t r) : source(std::move(src)), radius(r) {}
+
+    std::array<uint64_t, D> radii() const {
+        std::array<uint64_t, D> r{};
+        r.fill(radius);
+        return r;
+    }
+    Region<D> largest_region() const { return source.largest_region(); }
+    std::array<double, D> spacing() const { return source.spacing(); }
+    std::array<double, D> origin() const { return source.origin(); }
+    Region<D> input_region_for_output(const Region<D>& output) const {
+        return output.padded_per_axis(radii()).clipped_to(source.largest_region());
+    }
+    Image<P, D> generate_region(Region<D> requested) const {
+        const Region<D> bounds = source.largest_region();
+        const Image<P, D> input = source.generate_region(requested.padded_per_axis(radii()).clipped_to(bounds));
+        std::vector<double> w;
+        const auto k = box_kernel(radius);
+        for (size_t d = 0; d < D; ++d) w.insert(w.end(), k.begin(), k.end());
+        return separable_generate<P, D>(radii(), w, bounds, input, requested, spacing(), origin(),
+                                        separable_arith);
+    }
+};
+
+// BinomialBlurFilter<S> (src/filters/binomial_blur.rs:17-90): `repetitions`
+// rounds of [1, 2, 1] / 4 on every axis, each one radius-1 separable
+// generate_region whose request is the final request padded by the rounds
+// still to come.
+template <class S>
+struct BinomialBlurFilter {
+    using pixel_type = typename S::pixel_type;
+    static constexpr size_t dim = S::dim;
+    using P = pixel_type;
+    static constexpr size_t D = dim;
+
+    S source;
+    uint64_t repetitions;
+
+    BinomialBlurFilter(S src, uint64_t n) : source(std::move(src)), repetitions(n) {}
+
+    static double effective_sigma_pixels(uint64_t n) { return std::sqrt((double)n / 4.0); }
+
+    Region<D> largest_region() const { return source.largest_region(); }
+    std::array<double, D> spacing() const { return source.spacing(); }
+    std::array<double, D> origin() const { return source.origin(); }
+    Region<D> input_region_for_output(const Region<D>& output) const {
+        return output.padded(repetitions).clipped_to(source.largest_region());
+    }
+    Image<P, D> generate_region(Region<D> requested) const {
+        const Region<D> bounds = source.largest_region();
+        Image<P, D> cur = source.generate_region(requested.padded(repetitions).clipped_to(bounds));
+        std::array<uint64_t, D> ones{};
+        ones.fill(1);
+        std::vector<double> w;
+        for (size_t d = 0; d < D; ++d) w.insert(w.end(), {0.25, 0.5, 0.25});
+        for (uint64_t rep = 0; rep < repetitions; ++rep) {
+            const uint64_t remaining = repetitions - rep - 1;
+            const Region<D> reg = remaining == 0 ? requested : requested.padded(remaining).clipped_to(bounds);
+            cur = separable_generate<P, D>(ones, w, bounds, cur, reg, spacing(), origin(),
+                                           separable_arith);
+        }
+        return cur;
+    }
+};
 
 template <size_t D>
 std::vector<Region<D>> split_region(const Region<D>& region, uint64_t num_pieces) {

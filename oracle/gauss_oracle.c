@@ -39,6 +39,21 @@
  *   adjoint (f64 scatter)        : dense transpose of the Kronecker operator
  *                                  and <Ax, y> = <x, A^T y>;
  *   ref_faithful (threaded)      : bit-exact against generate_region.
+ *
+ * The other convolve_axis filters, restated the same way:
+ *
+ *   box_kernel                      src/filters/mean.rs:15-18
+ *   MeanFilter / BoxMeanFilterN     src/filters/mean.rs:59-83, 126-144
+ *     ::generate_region
+ *   BinomialBlurFilter              src/filters/binomial_blur.rs:57-89
+ *     ::generate_region (its own per-pass padding rule, loop for loop)
+ *
+ *   mean / binomial (f32)          : interior-impulse closed forms (box
+ *                                  products; binomial coefficients / 4^n,
+ *                                  exact in f32), the reference's own tests
+ *                                  (constant, ramp interior, impulse
+ *                                  spread), brute-force dense operator in
+ *                                  f64 (tests/test_oracle_filters.py).
  */
 #include <math.h>
 #include <pthread.h>
@@ -231,6 +246,99 @@ int NAME(int D, int ncomp, const T* img, const int64_t* img_index,              
 }
 DEFINE_GEN(orc_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
 DEFINE_GEN(orc_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
+
+/* ---- MeanFilter / BoxMeanFilterN: src/filters/mean.rs:15-18, 59-83,
+ * 126-144.  Per-axis radii (MeanFilter: all equal); the same pass regions as
+ * generate_region, each pass convolve_axis with box_kernel(radii[axis]). */
+void orc_box_kernel(uint64_t radius, double* k) {
+    uint64_t len = 2 * radius + 1;
+    for (uint64_t i = 0; i < len; ++i) k[i] = 1.0 / (double)len;
+}
+
+#define DEFINE_MEAN(NAME, T, LEAF, CONV)                                         \
+int NAME(int D, int ncomp, const T* img, const int64_t* img_index,               \
+         const int64_t* img_size, const uint64_t* radii,                         \
+         const int64_t* req_index, const int64_t* req_size, T* out) {            \
+    if (D < 1 || D > ORC_MAX_DIM || ncomp < 1) return -1;                        \
+    orc_region bounds = region_make(D, img_index, img_size);                     \
+    orc_region req = region_make(D, req_index, req_size);                        \
+    orc_region in_reg = region_clipped(D, region_padded(D, req, radii), bounds); \
+    orc_region cur_reg;                                                          \
+    T* cur = LEAF(D, ncomp, img, bounds, in_reg, &cur_reg);                      \
+    for (int axis = 0; axis < D; ++axis) {                                       \
+        uint64_t klen = 2 * radii[axis] + 1;                                     \
+        double* kern = (double*)malloc(klen * sizeof(double));                   \
+        orc_box_kernel(radii[axis], kern);                                       \
+        orc_region out_reg;                                                      \
+        if (axis + 1 < D) {                                                      \
+            uint64_t rem[ORC_MAX_DIM] = {0};                                     \
+            for (int d = axis + 1; d < D; ++d) rem[d] = radii[d];                \
+            out_reg = region_clipped(D, region_padded(D, req, rem), bounds);     \
+        } else {                                                                 \
+            out_reg = req;                                                       \
+        }                                                                        \
+        int64_t n = region_len(D, out_reg);                                      \
+        T* nxt = (T*)malloc((size_t)(n > 0 ? n : 1) * ncomp * sizeof(T));       \
+        CONV(D, ncomp, cur, cur_reg, axis, kern, klen, out_reg, nxt);            \
+        free(kern);                                                              \
+        free(cur);                                                               \
+        cur = nxt;                                                               \
+        cur_reg = out_reg;                                                       \
+    }                                                                            \
+    memcpy(out, cur, (size_t)region_len(D, req) * ncomp * sizeof(T));            \
+    free(cur);                                                                   \
+    return 0;                                                                    \
+}
+DEFINE_MEAN(orc_mean_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
+DEFINE_MEAN(orc_mean_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
+
+/* ---- BinomialBlurFilter: src/filters/binomial_blur.rs:57-89, loop for
+ * loop: fetch the request padded by `reps` on every axis; for each
+ * repetition and axis, convolve_axis with [1/4, 1/2, 1/4] into the request
+ * padded by (reps - rep - 1) + (1 on the axes not yet done in this
+ * repetition), clipped to the bounds; the very last pass writes the request
+ * itself. */
+static const double ORC_BINOMIAL[3] = {0.25, 0.5, 0.25};
+
+#define DEFINE_BINOMIAL(NAME, T, LEAF, CONV)                                     \
+int NAME(int D, int ncomp, const T* img, const int64_t* img_index,               \
+         const int64_t* img_size, uint64_t reps,                                 \
+         const int64_t* req_index, const int64_t* req_size, T* out) {            \
+    if (D < 1 || D > ORC_MAX_DIM || ncomp < 1) return -1;                        \
+    orc_region bounds = region_make(D, img_index, img_size);                     \
+    orc_region req = region_make(D, req_index, req_size);                        \
+    uint64_t all[ORC_MAX_DIM] = {0};                                             \
+    for (int d = 0; d < D; ++d) all[d] = reps;                                   \
+    orc_region in_reg = region_clipped(D, region_padded(D, req, all), bounds);   \
+    orc_region cur_reg;                                                          \
+    T* cur = LEAF(D, ncomp, img, bounds, in_reg, &cur_reg);                      \
+    for (uint64_t rep = 0; rep < reps; ++rep) {                                  \
+        uint64_t remaining = reps - rep - 1;                                     \
+        for (int axis = 0; axis < D; ++axis) {                                   \
+            int is_last = rep + 1 == reps && axis + 1 == D;                      \
+            orc_region out_reg;                                                  \
+            if (is_last) {                                                       \
+                out_reg = req;                                                   \
+            } else {                                                             \
+                uint64_t pad[ORC_MAX_DIM] = {0};                                 \
+                for (int d = 0; d < D; ++d)                                      \
+                    pad[d] = d > axis ? remaining + 1 : remaining;               \
+                out_reg = region_clipped(D, region_padded(D, req, pad), bounds); \
+            }                                                                    \
+            int64_t n = region_len(D, out_reg);                                  \
+            T* nxt = (T*)malloc((size_t)(n > 0 ? n : 1) * ncomp * sizeof(T));   \
+            CONV(D, ncomp, cur, cur_reg, axis, ORC_BINOMIAL, 3, out_reg, nxt);   \
+            free(cur);                                                           \
+            cur = nxt;                                                           \
+            cur_reg = out_reg;                                                   \
+        }                                                                        \
+    }                                                                            \
+    memcpy(out, cur, (size_t)region_len(D, cur_reg) * ncomp * sizeof(T));        \
+    free(cur);                                                                   \
+    return 0;                                                                    \
+}
+DEFINE_BINOMIAL(orc_binomial_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
+DEFINE_BINOMIAL(orc_binomial_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
 
 /* ---- adjoint in f64, by scattering the forward taps -------------------- */
 /* No reference counterpart.  The forward operator is A = A_{D-1} ... A_0

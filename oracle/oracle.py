@@ -16,6 +16,12 @@ Three independent restatements live here:
   (A[i, clamp(i + off + k - r)] += w_k), applied as a Kronecker product --
   the brute-force check for the forward and the explicit transpose for the
   adjoint.
+
+The other convolve_axis filters: ``mean_generate_region`` and
+``binomial_generate_region`` (C restatements of src/filters/mean.rs:59-83,
+126-144 and src/filters/binomial_blur.rs:57-89), with
+``dense_mean_operator`` / ``dense_binomial_operator`` as their brute-force
+checks.
 """
 from __future__ import annotations
 
@@ -58,6 +64,19 @@ def lib():
         L.orc_ref_faithful_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, i64p, i64p,
                                            f64p, ctypes.c_double, i64p, i64p, ctypes.c_void_p,
                                            ctypes.c_int]
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.orc_box_kernel.restype = None
+        L.orc_box_kernel.argtypes = [ctypes.c_uint64, f64p]
+        for name in ("orc_mean_generate_region_f32", "orc_mean_generate_region_f64"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, i64p, i64p, u64p,
+                           i64p, i64p, ctypes.c_void_p]
+        for name in ("orc_binomial_generate_region_f32", "orc_binomial_generate_region_f64"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, i64p, i64p, ctypes.c_uint64,
+                           i64p, i64p, ctypes.c_void_p]
         L.orc_adjoint_f64.restype = ctypes.c_int
         L.orc_adjoint_f64.argtypes = [ctypes.c_int, ctypes.c_int, i64p, i64p, f64p, ctypes.c_double,
                                       i64p, i64p, ctypes.c_void_p, ctypes.c_void_p, i64p, i64p]
@@ -134,6 +153,44 @@ def generate_region(img: np.ndarray, img_index, spacing, sigma, req_index, req_s
     if rc != 0:
         raise ValueError("oracle rejected arguments")
     return out
+
+
+def box_kernel(radius: int) -> np.ndarray:
+    """src/filters/mean.rs:15-18 (via the C oracle)."""
+    out = np.zeros(2 * radius + 1, dtype=np.float64)
+    lib().orc_box_kernel(int(radius), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def _filter_call(stem, img, img_index, arg, req_index, req_size, ncomp):
+    img = np.ascontiguousarray(img)
+    if img.dtype not in (np.float32, np.float64):
+        raise TypeError("oracle supports float32 and float64 pixels")
+    D = len(img_index)
+    img_size = list(reversed(img.shape[:D]))
+    fn = getattr(lib(), stem + ("_f32" if img.dtype == np.float32 else "_f64"))
+    out = np.zeros(_shape(req_size, ncomp), dtype=img.dtype)
+    rc = fn(D, ncomp, img.ctypes.data, _i64(img_index), _i64(img_size), arg, _i64(req_index),
+            _i64(req_size), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    return out
+
+
+def mean_generate_region(img: np.ndarray, img_index, radii, req_index, req_size,
+                         ncomp: int = 1) -> np.ndarray:
+    """BoxMeanFilterN{source: img, radii}.generate_region(req) (MeanFilter:
+    equal radii) via the C oracle (src/filters/mean.rs:59-83, 126-144)."""
+    arr = (ctypes.c_uint64 * len(radii))(*[int(r) for r in radii])
+    return _filter_call("orc_mean_generate_region", img, img_index, arr, req_index, req_size, ncomp)
+
+
+def binomial_generate_region(img: np.ndarray, img_index, reps, req_index, req_size,
+                             ncomp: int = 1) -> np.ndarray:
+    """BinomialBlurFilter{source: img, repetitions}.generate_region(req) via
+    the C oracle (src/filters/binomial_blur.rs:57-89)."""
+    return _filter_call("orc_binomial_generate_region", img, img_index, int(reps), req_index,
+                        req_size, ncomp)
 
 
 def ref_faithful(img: np.ndarray, img_index, spacing, sigma, req_index, req_size, ncomp=1,
@@ -257,6 +314,39 @@ def dense_operator(img_index, img_size, spacing, sigma, req_index, req_size):
     for a in range(D):
         w = _np_kernel(sigma / spacing[a], radii[a])
         mats.append(axis_matrix(w, req_index[a], req_size[a], in_i[a], in_s[a]))
+    return mats, in_i, in_s
+
+
+def dense_mean_operator(img_index, img_size, radii, req_index, req_size):
+    """Per-axis f64 matrices of the box-mean filter (weights 1/(2r+1) written
+    out here, not taken from the oracle) and the input region they act on."""
+    D = len(radii)
+    in_i, in_s = clipped(*padded(req_index, req_size, radii), img_index, img_size)
+    mats = []
+    for a in range(D):
+        w = np.full(2 * radii[a] + 1, 1.0 / (2 * radii[a] + 1))
+        mats.append(axis_matrix(w, req_index[a], req_size[a], in_i[a], in_s[a]))
+    return mats, in_i, in_s
+
+
+def dense_binomial_operator(img_index, img_size, reps, req_index, req_size):
+    """Per-axis f64 matrices of `reps` rounds of [1, 2, 1] / 4: along each axis
+    round t maps the request padded by reps - t (clipped) to the request padded
+    by reps - t - 1; the product of the rounds' matrices, and the input
+    region."""
+    D = len(req_index)
+    w = np.array([0.25, 0.5, 0.25])
+    in_i, in_s = clipped(*padded(req_index, req_size, [reps] * D), img_index, img_size)
+    mats = []
+    for a in range(D):
+        M = np.eye(in_s[a])
+        lo, n = in_i[a], in_s[a]
+        for t in range(reps):
+            oi, os_ = clipped(*padded([req_index[a]], [req_size[a]], [reps - t - 1]),
+                              [img_index[a]], [img_size[a]])
+            M = axis_matrix(w, oi[0], os_[0], lo, n) @ M
+            lo, n = oi[0], os_[0]
+        mats.append(M)
     return mats, in_i, in_s
 
 

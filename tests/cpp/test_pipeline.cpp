@@ -1,5 +1,7 @@
-// The reference's pipeline tests (tests/pipeline.rs:19-153) and the Gaussian
-// unit tests (src/filters/gaussian.rs:107-127), written against the C++ host
+// The reference's pipeline tests (tests/pipeline.rs:19-153), the Gaussian
+// unit tests (src/filters/gaussian.rs:107-127) and the MeanFilter and
+// BinomialBlurFilter unit tests (src/filters/mean.rs:261-294,
+// src/filters/binomial_blur.rs:97-128), written against the C++ host
 // API (itk-rs_amd/host/itk.hpp), whose GaussianFilter runs on the GPU.
 // Prints one "ok <name>" line per test; exits non-zero on the first failure.
 #include <cmath>
@@ -162,6 +164,60 @@ static void test_error_is_thrown() {
     CHECK(thrown, "request outside the largest region must raise");
 }
 
+// src/filters/mean.rs:261-269
+static void test_mean_constant_image() {
+    MeanFilter<Image<float, 1>> f(solid_image<1>({10}, 5.0f), 2);
+    auto out = f.generate_region(f.largest_region());
+    for (float v : out.data) CHECK(std::fabs(v - 5.0f) < 1e-5f, "expected 5.0, got %g", v);
+}
+
+// src/filters/mean.rs:271-282
+static void test_mean_ramp_interior() {
+    auto img = solid_image<1>({20}, 0.0f);
+    for (int64_t i = 0; i < 20; ++i) img.set_pixel(Index<1>{{i}}, (float)i);
+    MeanFilter<Image<float, 1>> f(img, 2);
+    auto out = f.generate_region(f.largest_region());
+    for (int64_t i = 2; i < 18; ++i) {
+        const float v = out.get_pixel(Index<1>{{i}});
+        CHECK(std::fabs(v - (float)i) < 1e-4f, "at %lld: got %g", (long long)i, v);
+    }
+}
+
+// src/filters/mean.rs:284-294
+static void test_box_mean_2d_constant() {
+    BoxMeanFilterN<Image<float, 2>> f(solid_image<2>({8, 8}, 3.0f), {1, 2});
+    auto out = f.generate_region(f.largest_region());
+    for (float v : out.data) CHECK(std::fabs(v - 3.0f) < 1e-5f, "expected 3.0, got %g", v);
+}
+
+// src/filters/binomial_blur.rs:97-107
+static void test_binomial_constant_image_preserved() {
+    BinomialBlurFilter<Image<float, 1>> f(solid_image<1>({20}, 3.0f), 4);
+    auto out = f.generate_region(f.largest_region());
+    for (int64_t i = 4; i < 16; ++i) {
+        const float v = out.get_pixel(Index<1>{{i}});
+        CHECK(std::fabs(v - 3.0f) < 1e-5f, "at %lld: %g", (long long)i, v);
+    }
+}
+
+// src/filters/binomial_blur.rs:109-122
+static void test_binomial_impulse_spreads() {
+    auto img = solid_image<1>({21}, 0.0f);
+    img.set_pixel(Index<1>{{10}}, 1.0f);
+    BinomialBlurFilter<Image<float, 1>> f(img, 4);
+    auto out = f.generate_region(f.largest_region());
+    const float peak = out.get_pixel(Index<1>{{10}});
+    CHECK(peak > 0.0f && peak < 1.0f, "peak=%g", peak);
+    const float s = sum(out.data);
+    CHECK(std::fabs(s - 1.0f) < 0.05f, "sum=%g", s);
+}
+
+// src/filters/binomial_blur.rs:124-128
+static void test_binomial_effective_sigma() {
+    const double s = BinomialBlurFilter<Image<float, 1>>::effective_sigma_pixels(4);
+    CHECK(std::fabs(s - 1.0) < 1e-10, "sigma=%g", s);
+}
+
 int main() {
 #define RUN(t)                  \
     t();                        \
@@ -177,6 +233,12 @@ int main() {
     RUN(test_unary_filter_pipeline);
     RUN(test_subregion_matches_full_crop);
     RUN(test_error_is_thrown);
+    RUN(test_mean_constant_image);
+    RUN(test_mean_ramp_interior);
+    RUN(test_box_mean_2d_constant);
+    RUN(test_binomial_constant_image_preserved);
+    RUN(test_binomial_impulse_spreads);
+    RUN(test_binomial_effective_sigma);
     std::printf("all passed\n");
     return 0;
 }

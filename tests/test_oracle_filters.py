@@ -1,0 +1,157 @@
+"""Pins the oracle's restatements of the other convolve_axis filters --
+MeanFilter / BoxMeanFilterN (src/filters/mean.rs:15-18, 59-83, 126-144) and
+BinomialBlurFilter (src/filters/binomial_blur.rs:57-89) -- before the GPU
+path is checked against them.
+
+Checks: the reference's own unit tests for both filters (mean.rs:261-294,
+binomial_blur.rs:92-129), interior-impulse closed forms that are exact in
+f32 (box products; binomial coefficients / 4^n), and the brute-force dense
+per-axis operator in f64 with weights written out in the test, on requests
+that touch the image border (where the binomial's per-repetition clamp
+differs from one wide kernel).
+"""
+import math
+
+import numpy as np
+import pytest
+
+RNG = np.random.default_rng(4321)
+
+
+def rand(shape, ncomp=1, dtype=np.float32):
+    shp = tuple(shape) + ((ncomp,) if ncomp > 1 else ())
+    return RNG.random(shp).astype(dtype)
+
+
+# --- box_kernel (mean.rs:15-18) ---------------------------------------------
+
+@pytest.mark.parametrize("r", [0, 1, 2, 5, 13])
+def test_box_kernel(orc, r):
+    k = orc.box_kernel(r)
+    assert len(k) == 2 * r + 1
+    assert np.all(k == 1.0 / (2 * r + 1))
+
+
+# --- the reference's own tests ----------------------------------------------
+
+def test_mean_constant_image(orc):
+    # mean.rs:261-269
+    out = orc.mean_generate_region(np.full(10, 5.0, np.float32), [0], [2], [0], [10])
+    assert np.all(np.abs(out - 5.0) < 1e-5)
+
+
+def test_mean_ramp_interior(orc):
+    # mean.rs:271-282: a ramp's symmetric-window mean is the centre value
+    ramp = np.arange(20, dtype=np.float32)
+    out = orc.mean_generate_region(ramp, [0], [2], [0], [20])
+    for i in range(2, 18):
+        assert abs(out[i] - i) < 1e-4
+
+
+def test_box_mean_2d_constant(orc):
+    # mean.rs:284-294
+    out = orc.mean_generate_region(np.full((8, 8), 3.0, np.float32), [0, 0], [1, 2], [0, 0], [8, 8])
+    assert np.all(np.abs(out - 3.0) < 1e-5)
+
+
+def test_binomial_constant_image_preserved(orc):
+    # binomial_blur.rs:97-107
+    out = orc.binomial_generate_region(np.full(20, 3.0, np.float32), [0], 4, [0], [20])
+    assert np.all(np.abs(out[4:16] - 3.0) < 1e-5)
+
+
+def test_binomial_impulse_spreads(orc):
+    # binomial_blur.rs:109-122
+    x = np.zeros(21, np.float32)
+    x[10] = 1.0
+    out = orc.binomial_generate_region(x, [0], 4, [0], [21])
+    assert 0.0 < out[10] < 1.0
+    assert abs(float(out.sum()) - 1.0) < 0.05
+
+
+# --- closed forms (exact in f32) --------------------------------------------
+
+@pytest.mark.parametrize("D,n", [(1, 1), (1, 4), (2, 3), (3, 2)])
+def test_binomial_impulse_closed_form(orc, D, n):
+    # away from the border, n rounds of [1,2,1]/4 give C(2n, k) / 4^n on each
+    # axis; every value and partial sum is a short dyadic fraction, so the f32
+    # arithmetic is exact and the outer product is bit-exact
+    N = 4 * n + 5
+    c = N // 2
+    x = np.zeros((N,) * D, np.float32)
+    x[(c,) * D] = 1.0
+    out = orc.binomial_generate_region(x, [0] * D, n, [0] * D, [N] * D)
+    line = np.zeros(N)
+    for k in range(2 * n + 1):
+        line[c - n + k] = math.comb(2 * n, k) / 4.0 ** n
+    want = line
+    for _ in range(D - 1):
+        want = np.multiply.outer(want, line)
+    assert np.array_equal(out.astype(np.float64), want)
+
+
+@pytest.mark.parametrize("radii", [[1], [3], [2, 1], [1, 2, 3]])
+def test_mean_impulse_closed_form(orc, radii):
+    # one nonzero input: each pass multiplies by fl32(1/len) once, so the
+    # output is the running f32 product of the per-axis weights inside the
+    # box and zero outside
+    D = len(radii)
+    N = 2 * max(radii) + 5
+    c = N // 2
+    x = np.zeros((N,) * D, np.float32)
+    x[(c,) * D] = 1.0
+    out = orc.mean_generate_region(x, [0] * D, radii, [0] * D, [N] * D)
+    want = np.zeros((N,) * D, np.float32)
+    val = np.float32(1.0)
+    for a in range(D):
+        val = np.float32(val * np.float32(1.0 / (2 * radii[a] + 1)))
+    sl = tuple(slice(c - radii[D - 1 - i], c + radii[D - 1 - i] + 1) for i in range(D))
+    want[sl] = val
+    assert np.array_equal(out, want)
+
+
+# --- dense operator (brute force, f64) --------------------------------------
+
+@pytest.mark.parametrize("size,radii,req", [
+    ((9, 7, 6), (1, 2, 3), ((0, 0, 0), (9, 7, 6))),
+    ((12, 10, 8), (2, 2, 1), ((3, 1, 0), (5, 6, 4))),
+    ((20, 15), (4, 0), ((0, 7), (20, 8))),
+])
+def test_mean_vs_dense(orc, size, radii, req):
+    x = rand(tuple(reversed(size)), dtype=np.float64)
+    out = orc.mean_generate_region(x, [0] * len(size), list(radii), list(req[0]), list(req[1]))
+    mats, in_i, in_s = orc.dense_mean_operator([0] * len(size), list(size), list(radii),
+                                               list(req[0]), list(req[1]))
+    sub = x[tuple(slice(i, i + s) for i, s in zip(reversed(in_i), reversed(in_s)))]
+    assert np.allclose(out, orc.apply_dense(mats, sub), rtol=0, atol=1e-12)
+    out32 = orc.mean_generate_region(x.astype(np.float32), [0] * len(size), list(radii),
+                                     list(req[0]), list(req[1]))
+    assert np.max(np.abs(out32 - out)) < 1e-6
+
+
+@pytest.mark.parametrize("size,reps,req", [
+    ((9, 8, 7), 2, ((0, 0, 0), (9, 8, 7))),
+    ((14, 11), 3, ((0, 2), (6, 9))),        # touches the x = 0 border
+    ((16, 12, 10), 4, ((5, 0, 6), (6, 12, 4))),
+    ((10,), 6, ((0,), (10,))),               # radius 6 exceeds half the image
+])
+def test_binomial_vs_dense(orc, size, reps, req):
+    D = len(size)
+    x = rand(tuple(reversed(size)), dtype=np.float64)
+    out = orc.binomial_generate_region(x, [0] * D, reps, list(req[0]), list(req[1]))
+    mats, in_i, in_s = orc.dense_binomial_operator([0] * D, list(size), reps,
+                                                   list(req[0]), list(req[1]))
+    sub = x[tuple(slice(i, i + s) for i, s in zip(reversed(in_i), reversed(in_s)))]
+    assert np.allclose(out, orc.apply_dense(mats, sub), rtol=0, atol=1e-12)
+
+
+def test_binomial_is_not_one_wide_kernel_at_the_border(orc):
+    # n rounds of a clamped 3-tap kernel differ from one clamped 2n+1-tap
+    # binomial kernel next to the border: the oracle must follow the rounds
+    x = rand((12,), dtype=np.float64)
+    n = 3
+    out = orc.binomial_generate_region(x, [0], n, [0], [12])
+    w = np.array([math.comb(2 * n, k) / 4.0 ** n for k in range(2 * n + 1)])
+    wide = orc.apply_dense([orc.axis_matrix(w, 0, 12, 0, 12)], x)
+    assert np.allclose(out[n:-n], wide[n:-n], atol=1e-12)
+    assert not np.allclose(out[:n], wide[:n], atol=1e-9)

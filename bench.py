@@ -17,7 +17,9 @@ reference, the accuracy the target is stated at).  Also reported (under
     group in one captured graph, tile shapes and z runs picked by the
     autotuner) vs a loop of plain single-volume launches, and the batched
     adjoint over 16 uneven gradients vs a loop of plain adjoint launches,
-  - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration.
+  - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration,
+  - MeanFilter (radius 2, FAST and EXACT) and BinomialBlurFilter (4
+    repetitions) on a 512^3 volume.
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
 write) / the average kernel duration from HIP events on the launch stream,
@@ -264,6 +266,33 @@ def main():
         extras["c3_vec3_256x256x160"] = {"gvox_s": round(n3 * 10 / w3 / 1e9, 3),
                                          "hbm_frac_24B": round(24.0 * n3 / (np.mean(ms3) * 1e-3)
                                                                / 1e9 / HBM_PEAK_GBS, 4)}
+        del v3
+        torch.cuda.empty_cache()
+
+        # the other convolve_axis filters on the same 512^3 volume: MeanFilter
+        # (radius 2, one fused round trip) and BinomialBlurFilter (4 rounds,
+        # one radius-1 round trip each)
+        xm = torch.rand((512, 512, 512), device=dev)
+        ym = torch.empty_like(xm)
+        im = itk.Image(itk.Region([0, 0, 0], [512] * 3), (1.0,) * 3, (0.0,) * 3, xm)
+        om = itk.Image(im.region, (1.0,) * 3, (0.0,) * 3, ym)
+        nm = xm.numel()
+        for name, mode in (("fast", itk.FAST), ("exact", itk.EXACT)):
+            mf = itk.MeanFilter(im, 2)
+            mf.arith = mode
+            wm, msm = timed_loop(lambda: mf.generate_into(im, om), 20, 3, stream)
+            extras["mean_r2_512_" + name] = {
+                "gvox_s": round(nm * 20 / wm / 1e9, 3), "kernel_ms": round(float(np.mean(msm)), 4),
+                "hbm_frac": round(8.0 * nm / (np.mean(msm) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        bf = itk.BinomialBlurFilter(im, 4)
+        bf.arith = arith
+        wbn, msb = timed_loop(lambda: bf.generate_region(im.region), 10, 2, stream)
+        extras["binomial_n4_512"] = {
+            "gvox_s": round(nm * 10 / wbn / 1e9, 3), "ms": round(float(np.mean(msb)), 4),
+            "hbm_frac_4_round_trips": round(4 * 8.0 * nm / (np.mean(msb) * 1e-3) / 1e9
+                                            / HBM_PEAK_GBS, 4)}
+        del xm, ym, im, om
+        torch.cuda.empty_cache()
         result["extras"] = extras
 
     if not args.no_cpu:

@@ -238,6 +238,14 @@ class Image:
         return Image(region, tuple(spacing), tuple(origin),
                      torch.full(shape, float(fill), dtype=dtype, device=device))
 
+    @staticmethod
+    def empty(region: Region, spacing, origin, ncomp: int = 1, dtype=torch.float32,
+              device="cuda") -> "Image":
+        """An image whose pixels a kernel is about to write in full (no fill)."""
+        shape = tuple(reversed(region.size)) + ((ncomp,) if ncomp > 1 else ())
+        return Image(region, tuple(spacing), tuple(origin),
+                     torch.empty(shape, dtype=dtype, device=device))
+
     @property
     def ndim(self) -> int:
         return self.region.ndim
@@ -357,8 +365,8 @@ class GaussianFilter(ImageSource):
         input_region = requested.padded_per_axis(self._radii()).clipped_to(self.largest_region())
         inp = self.source.generate_region(input_region)
         dt, nc = _pixel_info(inp.data, requested.ndim)
-        out = Image.allocate(requested, self.spacing, self.origin, 0.0, nc, inp.data.dtype,
-                             inp.data.device)
+        out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype,
+                          inp.data.device)
         self.generate_into(inp, out)
         return out
 
@@ -490,8 +498,8 @@ class _SeparableFilter(ImageSource):
         input_region = requested.padded_per_axis(radii).clipped_to(self.largest_region())
         inp = self.source.generate_region(input_region)
         _, nc = _pixel_info(inp.data, requested.ndim)
-        out = Image.allocate(requested, self.spacing, self.origin, 0.0, nc, inp.data.dtype,
-                             inp.data.device)
+        out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype,
+                          inp.data.device)
         self.generate_into(inp, out)
         return out
 
@@ -571,15 +579,15 @@ class BinomialBlurFilter(ImageSource):
         for rep in range(n):
             remaining = n - rep - 1
             reg = requested if remaining == 0 else requested.padded(remaining).clipped_to(bounds)
-            out = Image.allocate(reg, self.spacing, self.origin, 0.0, nc, cur.data.dtype,
-                                 cur.data.device)
+            out = Image.empty(reg, self.spacing, self.origin, nc, cur.data.dtype,
+                              cur.data.device)
             _separable_into([1] * D, [BINOMIAL_KERNEL] * D, bounds, cur, out, self.arith)
             cur = out
         if n == 0:
             # no rounds: the input region is the request itself (a copy, as
             # the reference returns the upstream image)
-            out = Image.allocate(requested, self.spacing, self.origin, 0.0, nc, cur.data.dtype,
-                                 cur.data.device)
+            out = Image.empty(requested, self.spacing, self.origin, nc, cur.data.dtype,
+                              cur.data.device)
             out.data.copy_(cur.data)
             cur = out
         return cur
@@ -601,8 +609,8 @@ def gaussian(x: torch.Tensor, sigma: float, spacing=None, ndim: Optional[int] = 
     sp = tuple(spacing) if spacing is not None else (1.0,) * nd
     src = _full_image(x.contiguous() if not x.is_contiguous() else x, nd, sp)
     f = GaussianFilter(src, sigma)
-    out = Image.allocate(src.region, sp, src.origin, 0.0,
-                         x.shape[-1] if x.dim() > nd else 1, x.dtype, x.device)
+    out = Image.empty(src.region, sp, src.origin,
+                      x.shape[-1] if x.dim() > nd else 1, x.dtype, x.device)
     with torch.cuda.device(x.device):
         _check(lib.itkg_gaussian_generate_region(
             nd, _pixel_info(x, nd)[0], _pixel_info(x, nd)[1], float(sigma), _doubles(sp),
@@ -640,8 +648,8 @@ def mean_filter(x: torch.Tensor, radius, ndim: Optional[int] = None,
     src = _full_image(x.contiguous(), nd, (1.0,) * nd)
     f = BoxMeanFilterN(src, radius) if isinstance(radius, (list, tuple)) else MeanFilter(src, radius)
     f.arith = arith
-    out = Image.allocate(src.region, src.spacing, src.origin, 0.0,
-                         x.shape[-1] if x.dim() > nd else 1, x.dtype, x.device)
+    out = Image.empty(src.region, src.spacing, src.origin,
+                      x.shape[-1] if x.dim() > nd else 1, x.dtype, x.device)
     f.generate_into(src, out)
     return out.data
 

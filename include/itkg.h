@@ -125,8 +125,12 @@ int itkg_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
 
 /* Same contract on dense HOST buffers (what an ImageSource whose Image owns
  * a host Vec<P> hands over): `input` holds input_region_for_output(requested)
- * densely, `output` receives `requested` densely.  Copies in, computes and
- * copies out on an internal stream; returns when the output is written. */
+ * densely, `output` receives `requested` densely.  Returns when the output is
+ * written.  Requests over 32 MiB run as pieces along the slowest axis, as
+ * stream_full does (src/source.rs:81-97): two helper threads copy input
+ * chunks in and finished pieces out while the GPU computes, so the inbound
+ * and outbound transfers overlap; the output is bit-identical to one
+ * piece. */
 int itkg_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
                                        double sigma, const double* spacing,
                                        const itkg_region* largest, const void* input,

@@ -7,12 +7,14 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -115,6 +117,7 @@ void weights_of(double sigma_px, uint64_t radius, double* k) {
 int plan_regions(int D, const itkg_region* largest, const itkg_region& req, Plan* p) {
     if (region_len(D, req) > 0 && !contains(D, *largest, req))
         return fail(ITKG_ERR_INVALID_ARG, "requested region is not inside the largest region");
+    p->req = req;
     p->in_reg = clipped(D, padded(D, req, p->radii), *largest);
     p->reg[0] = p->in_reg;
     for (int a = 0; a < D; ++a) {
@@ -503,14 +506,12 @@ int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout
                           : backward_per_axis<float>(p, ncomp, gout, gin, accumulate, s, scratch);
 }
 
-// Host buffers: stage the input region in, run the plan, stage the
-// requested region out, on a private stream (synchronous for the caller).
-int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, void* output,
-                 itkg_arith arith) {
+// Host buffers, one piece: stage the input region in, run the plan, stage
+// the requested region out, on a private stream.
+int host_forward_serial(const Plan& p, itkg_dtype dtype, int ncomp, const void* input,
+                        void* output, itkg_arith arith) {
     const int ndim = p.D;
     const int64_t nin = region_len(ndim, p.in_reg), nout = region_len(ndim, p.req);
-    if (nout == 0) return ITKG_OK;
-    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
     int rc = ITKG_OK;
     const size_t px = dtype_size(dtype) * ncomp;
     hipStream_t s = nullptr;
@@ -539,6 +540,179 @@ int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, 
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
     (void)hipStreamDestroy(s);
+    return rc;
+}
+
+// Host buffers, pipelined (src/source.rs:81-97 applied to the transfers):
+// the request is split into pieces along its slowest axis.  One helper
+// thread copies the input region in, chunk by chunk along that axis; the
+// calling thread launches each piece's kernels on a compute stream once the
+// input planes the piece reads have arrived; a second helper thread copies
+// each finished piece out.  A copy from or to pageable memory holds its
+// calling thread until it is done, so the two directions get threads of
+// their own, and the inbound and outbound copies overlap.  The hand-offs are
+// ordered on the host: a chunk's copy has completed before the chunk is
+// announced, and a piece's kernels have completed before its copy out
+// starts (free here: a pageable copy holds its thread until it is done).  Each piece is
+// computed exactly as a streamed piece, so the output is bit-identical to
+// the serial form.  Pieces hold about kHostPieceBytes of output.
+constexpr int64_t kHostPieceBytes = 16ll << 20;
+constexpr int kHostMaxPieces = 32;
+
+int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, void* output,
+                 itkg_arith arith) {
+    const int D = p.D;
+    const int64_t nin = region_len(D, p.in_reg), nout = region_len(D, p.req);
+    if (nout == 0) return ITKG_OK;
+    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
+    const size_t px = dtype_size(dtype) * ncomp;
+    const int ax = D - 1;
+    const int64_t in_planes = p.in_reg.size[ax], out_planes = p.req.size[ax];
+    const int K = (int)std::min<int64_t>(
+        {(int64_t)kHostMaxPieces, out_planes, in_planes,
+         std::max<int64_t>(1, (nout * (int64_t)px) / kHostPieceBytes)});
+    if (K <= 1) return host_forward_serial(p, dtype, ncomp, input, output, arith);
+    const int64_t in_plane = nin / in_planes, out_plane = nout / out_planes;  // pixels per plane
+
+    // chunk j of the input planes: [ib[j], ib[j+1]); piece k of the request:
+    // output planes [ob[k], ob[k+1])
+    std::vector<int64_t> ib(K + 1), ob(K + 1);
+    for (int j = 0; j <= K; ++j) {
+        ib[j] = in_planes * j / K;
+        ob[j] = out_planes * j / K;
+    }
+    std::vector<Plan> piece(K);
+    std::vector<int> need(K);  // last input chunk a piece reads
+    for (int k = 0; k < K; ++k) {
+        itkg_region r = p.req;
+        r.index[ax] = p.req.index[ax] + ob[k];
+        r.size[ax] = ob[k + 1] - ob[k];
+        piece[k] = p;
+        if (int rc = plan_regions(D, &p.bounds, r, &piece[k])) return rc;
+        const int64_t last = piece[k].in_reg.index[ax] + piece[k].in_reg.size[ax] - 1 - p.in_reg.index[ax];
+        int j = 0;
+        while (j + 1 < K && ib[j + 1] <= last) ++j;
+        need[k] = j;
+    }
+
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    hipStream_t s_in = nullptr, s_comp = nullptr, s_out = nullptr;
+    std::vector<hipEvent_t> ev_comp(K, nullptr);
+    void *din = nullptr, *dout = nullptr;
+    int rc = ITKG_OK;
+    auto cleanup = [&]() {
+        for (hipStream_t st : {s_in, s_comp, s_out})
+            if (st) (void)hipStreamSynchronize(st);
+        if (din) (void)hipFree(din);
+        if (dout) (void)hipFree(dout);
+        for (auto ev : ev_comp)
+            if (ev) (void)hipEventDestroy(ev);
+        for (hipStream_t st : {s_in, s_comp, s_out})
+            if (st) (void)hipStreamDestroy(st);
+    };
+    for (hipStream_t* st : {&s_in, &s_comp, &s_out})
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+    for (int k = 0; k < K && e == hipSuccess; ++k)
+        e = hipEventCreateWithFlags(&ev_comp[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(&din, (size_t)nin * px);
+    if (e == hipSuccess) e = hipMalloc(&dout, (size_t)nout * px);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "host staging set-up");
+        cleanup();
+        return rc;
+    }
+
+    // progress shared with the helper threads
+    std::mutex mu;
+    std::condition_variable cv;
+    int in_done = 0, comp_done = 0;
+    bool abort = false;
+    int rc_in = ITKG_OK, rc_out = ITKG_OK;
+    std::string err_in, err_out;
+    auto publish = [&](int& counter, int value) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            counter = value;
+        }
+        cv.notify_all();
+    };
+    auto stop = [&]() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            abort = true;
+        }
+        cv.notify_all();
+    };
+    // waits until counter > idx or abort; returns false on abort
+    auto wait_for = [&](const int& counter, int idx) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return abort || counter > idx; });
+        return !abort;
+    };
+    const char* src = static_cast<const char*>(input);
+    char* dst = static_cast<char*>(output);
+
+    std::thread t_in([&]() {
+        hipError_t te = hipSetDevice(dev);
+        for (int j = 0; j < K && te == hipSuccess; ++j) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (abort) return;
+            }
+            const size_t off = (size_t)(ib[j] * in_plane) * px;
+            const size_t bytes = (size_t)((ib[j + 1] - ib[j]) * in_plane) * px;
+            te = hipMemcpyAsync(static_cast<char*>(din) + off, src + off, bytes,
+                                hipMemcpyHostToDevice, s_in);
+            // complete before it is announced; the kernels that read it are
+            // launched after that
+            if (te == hipSuccess) te = hipStreamSynchronize(s_in);
+            if (te == hipSuccess) publish(in_done, j + 1);
+        }
+        if (te != hipSuccess) {
+            rc_in = hip_fail(te, "copy in");
+            err_in = g_err;
+            stop();
+        }
+    });
+    std::thread t_out([&]() {
+        hipError_t te = hipSetDevice(dev);
+        for (int k = 0; k < K && te == hipSuccess; ++k) {
+            if (!wait_for(comp_done, k)) return;
+            const size_t off = (size_t)(ob[k] * out_plane) * px;
+            const size_t bytes = (size_t)((ob[k + 1] - ob[k]) * out_plane) * px;
+            // the piece's kernels have finished before its copy is issued
+            te = hipEventSynchronize(ev_comp[k]);
+            if (te == hipSuccess)
+                te = hipMemcpyAsync(dst + off, static_cast<char*>(dout) + off, bytes,
+                                    hipMemcpyDeviceToHost, s_out);
+            if (te == hipSuccess) te = hipStreamSynchronize(s_out);
+        }
+        if (te == hipSuccess) te = hipStreamSynchronize(s_out);
+        if (te != hipSuccess) {
+            rc_out = hip_fail(te, "copy out");
+            err_out = g_err;
+            stop();
+        }
+    });
+    const itkg_view vin = dense_view(D, din, p.in_reg);
+    const itkg_view vout = dense_view(D, dout, p.req);
+    for (int k = 0; k < K && !rc; ++k) {
+        if (!wait_for(in_done, need[k])) break;
+        rc = forward_impl(piece[k], dtype, ncomp, vin, vout, arith, s_comp, nullptr);
+        if (!rc) {
+            e = hipEventRecord(ev_comp[k], s_comp);
+            if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+        }
+        if (!rc) publish(comp_done, k + 1);
+    }
+    if (rc) stop();
+    t_in.join();
+    t_out.join();
+    if (!rc && rc_in) rc = fail(rc_in, "%s", err_in.c_str());
+    if (!rc && rc_out) rc = fail(rc_out, "%s", err_out.c_str());
+    cleanup();
     return rc;
 }
 

@@ -542,7 +542,8 @@ class BinomialBlurFilter(ImageSource):
     rounds of the [1, 2, 1] / 4 kernel on every axis.  Each round is one
     radius-1 separable generate_region whose requested region is the final
     request padded by the rounds still to come (the per-pass regions of
-    binomial_blur.rs:70-85); rounds ping-pong between device images."""
+    binomial_blur.rs:70-85); the rounds before the last ping-pong between two
+    device buffers the filter keeps between calls."""
 
     arith = EXACT
 
@@ -570,27 +571,56 @@ class BinomialBlurFilter(ImageSource):
         return _separable_input_region([self.repetitions] * output_region.ndim,
                                        self.largest_region(), output_region)
 
-    def generate_region(self, requested: Region) -> Image:
+    def _scratch(self, k: int, numel: int, dtype, device) -> torch.Tensor:
+        """Intermediate buffer k (0 or 1) of at least `numel` pixels, kept
+        across calls; a call on another stream first waits for the last one."""
+        bufs = self.__dict__.setdefault("_bufs", {})
+        t = bufs.get(k)
+        if t is None or t.numel() < numel or t.dtype != dtype or t.device != device:
+            t = torch.empty(max(numel, 1), dtype=dtype, device=device)
+            bufs[k] = t
+        return t[:numel]
+
+    def generate_into(self, inp: Image, out: Image) -> None:
+        """Write generate_region(out.region) into `out` from an upstream image
+        covering input_region_for_output(out.region); the rounds before the
+        last run in two buffers the filter keeps between calls."""
+        requested = out.region
         D = requested.ndim
         bounds = self.largest_region()
         n = self.repetitions
-        cur = self.source.generate_region(requested.padded(n).clipped_to(bounds))
-        _, nc = _pixel_info(cur.data, D)
+        _, nc = _pixel_info(inp.data, D)
+        if n == 0:
+            out.data.copy_(inp.data[inp._slices(requested)])
+            return
+        stream = torch.cuda.current_stream(inp.data.device)
+        last = self.__dict__.get("_stream")
+        if last is not None and last != stream:
+            stream.wait_stream(last)
+        self._stream = stream
+        cur = inp
         for rep in range(n):
             remaining = n - rep - 1
-            reg = requested if remaining == 0 else requested.padded(remaining).clipped_to(bounds)
-            out = Image.empty(reg, self.spacing, self.origin, nc, cur.data.dtype,
-                              cur.data.device)
-            _separable_into([1] * D, [BINOMIAL_KERNEL] * D, bounds, cur, out, self.arith)
-            cur = out
-        if n == 0:
-            # no rounds: the input region is the request itself (a copy, as
-            # the reference returns the upstream image)
-            out = Image.empty(requested, self.spacing, self.origin, nc, cur.data.dtype,
-                              cur.data.device)
-            out.data.copy_(cur.data)
-            cur = out
-        return cur
+            if remaining == 0:
+                nxt = out
+            else:
+                reg = requested.padded(remaining).clipped_to(bounds)
+                shape = tuple(reversed(reg.size)) + ((nc,) if nc > 1 else ())
+                numel = 1
+                for v in shape:
+                    numel *= v
+                buf = self._scratch(rep % 2, numel, inp.data.dtype, inp.data.device)
+                nxt = Image(reg, self.spacing, self.origin, buf.view(shape))
+            _separable_into([1] * D, [BINOMIAL_KERNEL] * D, bounds, cur, nxt, self.arith)
+            cur = nxt
+
+    def generate_region(self, requested: Region) -> Image:
+        D = requested.ndim
+        inp = self.source.generate_region(self.input_region_for_output(requested))
+        _, nc = _pixel_info(inp.data, D)
+        out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
+        self.generate_into(inp, out)
+        return out
 
 
 # ---------------------------------------------------------------------------

@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--arith", choices=["exact", "fast"], default="fast")
     ap.add_argument("--no-extras", action="store_true", help="headline loop only (profiling)")
+    ap.add_argument("--extras", default="",
+                    help="comma list of extras groups to run (default all): "
+                         "c2, c5, c4, c5_batch, c1_c3, filters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprof-derived HBM bytes per launch, if measured")
@@ -147,153 +150,165 @@ def main():
     }
 
     extras = {}
+    chosen = set(args.extras.split(",")) if args.extras else None
+
+    def want(name):
+        return chosen is None or name in chosen
+
     if not args.no_extras:
-        # reference point: one device copy of the same 512 MiB (read + write),
-        # the rate a bare stream of these bytes reaches on this device
-        _, msc = timed_loop(lambda: out.copy_(x), args.steps, 2, stream)
-        copy_gbs = 8.0 * n / (float(np.mean(msc)) * 1e-3) / 1e9
-        extras["device_copy_512MiB"] = {"gb_s": round(copy_gbs, 1),
-                                        "hbm_frac": round(copy_gbs / HBM_PEAK_GBS, 4),
-                                        "headline_vs_copy": round(achieved / copy_gbs, 4)}
-        # the other arithmetic mode on the same volume
-        other = itk.FAST if arith == itk.EXACT else itk.EXACT
-        f2 = itk.GaussianFilter(img, sigma)
-        f2.arith = other
-        w2, ms2 = timed_loop(lambda: f2.generate_into(img, oimg), args.steps, 2, stream)
-        extras["c2_" + ("fast" if other == itk.FAST else "exact")] = {
-            "gvox_s": round(n * args.steps / w2 / 1e9, 3),
-            "kernel_ms": round(float(np.mean(ms2)), 4),
-            "hbm_frac": round(8.0 * n / (np.mean(ms2) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        # 8-piece streamed form (device-resident stream_full)
-        ws, _ = timed_loop(lambda: itk.stream_full(f, 8), 5, 1, stream)
-        extras["c2_stream_full_8"] = {"gvox_s": round(n * 5 / ws / 1e9, 3)}
-        # accuracy of both modes against the f64 evaluation of the operator
-        outs = {}
-        for name, mode in (("fast", itk.FAST), ("exact", itk.EXACT)):
-            outs[name] = itk.gaussian(x, sigma, sp, arith=mode)
-        ref64 = itk.gaussian(x.double(), sigma, sp)
-        scale = float(ref64.abs().max())
-        extras["c2_max_rel_err"] = {
-            "fast_vs_f64": float((outs["fast"].double() - ref64).abs().max()) / scale,
-            "exact_vs_f64": float((outs["exact"].double() - ref64).abs().max()) / scale,
-            "fast_vs_exact": float((outs["fast"] - outs["exact"]).abs().max()) /
-                             float(outs["exact"].abs().max())}
-        del outs, ref64, out, oimg
-        torch.cuda.empty_cache()
+        if want("c2"):
+            # reference point: one device copy of the same 512 MiB (read + write),
+            # the rate a bare stream of these bytes reaches on this device
+            _, msc = timed_loop(lambda: out.copy_(x), args.steps, 2, stream)
+            copy_gbs = 8.0 * n / (float(np.mean(msc)) * 1e-3) / 1e9
+            extras["device_copy_512MiB"] = {"gb_s": round(copy_gbs, 1),
+                                            "hbm_frac": round(copy_gbs / HBM_PEAK_GBS, 4),
+                                            "headline_vs_copy": round(achieved / copy_gbs, 4)}
+            # the other arithmetic mode on the same volume
+            other = itk.FAST if arith == itk.EXACT else itk.EXACT
+            f2 = itk.GaussianFilter(img, sigma)
+            f2.arith = other
+            w2, ms2 = timed_loop(lambda: f2.generate_into(img, oimg), args.steps, 2, stream)
+            extras["c2_" + ("fast" if other == itk.FAST else "exact")] = {
+                "gvox_s": round(n * args.steps / w2 / 1e9, 3),
+                "kernel_ms": round(float(np.mean(ms2)), 4),
+                "hbm_frac": round(8.0 * n / (np.mean(ms2) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            # 8-piece streamed form (device-resident stream_full)
+            ws, _ = timed_loop(lambda: itk.stream_full(f, 8), 5, 1, stream)
+            extras["c2_stream_full_8"] = {"gvox_s": round(n * 5 / ws / 1e9, 3)}
+            # accuracy of both modes against the f64 evaluation of the operator
+            outs = {}
+            for name, mode in (("fast", itk.FAST), ("exact", itk.EXACT)):
+                outs[name] = itk.gaussian(x, sigma, sp, arith=mode)
+            ref64 = itk.gaussian(x.double(), sigma, sp)
+            scale = float(ref64.abs().max())
+            extras["c2_max_rel_err"] = {
+                "fast_vs_f64": float((outs["fast"].double() - ref64).abs().max()) / scale,
+                "exact_vs_f64": float((outs["exact"].double() - ref64).abs().max()) / scale,
+                "fast_vs_exact": float((outs["fast"] - outs["exact"]).abs().max()) /
+                                 float(outs["exact"].abs().max())}
+            del outs, ref64, out, oimg
+            torch.cuda.empty_cache()
 
-        # forward + adjoint at 256^3 (MALL-resident sizes: flush between reps)
-        x5, g5, s5, sp5 = wl.c5(dev)
-        n5 = x5.numel()
-        flushbuf = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
-        flush = lambda: flushbuf.fill_(1.0)  # noqa: E731
-        y5 = torch.empty_like(x5)
-        gi5 = torch.empty_like(x5)
-        im5 = itk.Image(itk.Region([0, 0, 0], [256] * 3), sp5, (0.0,) * 3, x5)
-        om5 = itk.Image(im5.region, sp5, (0.0,) * 3, y5)
-        go5 = itk.Image(im5.region, sp5, (0.0,) * 3, g5)
-        gin5 = itk.Image(im5.region, sp5, (0.0,) * 3, gi5)
-        f5 = itk.GaussianFilter(im5, s5)
-        f5.arith = arith
-        _, msf = timed_loop(lambda: f5.generate_into(im5, om5), 10, 2, stream, flush)
+        if want("c5"):
+            # forward + adjoint at 256^3 (MALL-resident sizes: flush between reps)
+            x5, g5, s5, sp5 = wl.c5(dev)
+            n5 = x5.numel()
+            flushbuf = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+            flush = lambda: flushbuf.fill_(1.0)  # noqa: E731
+            y5 = torch.empty_like(x5)
+            gi5 = torch.empty_like(x5)
+            im5 = itk.Image(itk.Region([0, 0, 0], [256] * 3), sp5, (0.0,) * 3, x5)
+            om5 = itk.Image(im5.region, sp5, (0.0,) * 3, y5)
+            go5 = itk.Image(im5.region, sp5, (0.0,) * 3, g5)
+            gin5 = itk.Image(im5.region, sp5, (0.0,) * 3, gi5)
+            f5 = itk.GaussianFilter(im5, s5)
+            f5.arith = arith
+            _, msf = timed_loop(lambda: f5.generate_into(im5, om5), 10, 2, stream, flush)
 
-        def fwd_adj():
-            f5.generate_into(im5, om5)
-            f5.backward_into(go5, gin5, False)
-        _, msfa = timed_loop(fwd_adj, 10, 2, stream, flush)
-        fwd_g = n5 / (np.mean(msf) * 1e-3) / 1e9
-        fa_g = n5 / (np.mean(msfa) * 1e-3) / 1e9
-        extras["c5_fwd_adjoint_256"] = {"fwd_gvox_s": round(fwd_g, 3),
-                                        "fwd_plus_adj_gvox_s": round(fa_g, 3),
-                                        "ratio": round(fa_g / fwd_g, 3)}
-        del x5, g5, y5, gi5, im5, om5, go5, gin5, flushbuf
-        torch.cuda.empty_cache()
+            def fwd_adj():
+                f5.generate_into(im5, om5)
+                f5.backward_into(go5, gin5, False)
+            _, msfa = timed_loop(fwd_adj, 10, 2, stream, flush)
+            fwd_g = n5 / (np.mean(msf) * 1e-3) / 1e9
+            fa_g = n5 / (np.mean(msfa) * 1e-3) / 1e9
+            extras["c5_fwd_adjoint_256"] = {"fwd_gvox_s": round(fwd_g, 3),
+                                            "fwd_plus_adj_gvox_s": round(fa_g, 3),
+                                            "ratio": round(fa_g / fwd_g, 3)}
+            del x5, g5, y5, gi5, im5, om5, go5, gin5, flushbuf
+            torch.cuda.empty_cache()
 
-        # batched: 64 uneven volumes vs a loop of plain launches
-        vols, params = wl.c4(dev)
-        outs = [torch.empty_like(v) for v in vols]
-        plan = itk.BatchPlan(vols, outs, [p[1] for p in params], [p[2] for p in params],
-                             direction=itk.BATCH_FORWARD, arith=arith)
-        nb = plan.voxels
-        plan.autotune(stream)  # z run per radius group, measured on these volumes
-        wb, _ = timed_loop(lambda: plan.run(stream), 5, 2, stream)
-        imgs = [itk.Image(itk.Region([0, 0, 0], list(reversed(v.shape))), p[2], (0.0,) * 3, v)
-                for v, p in zip(vols, params)]
-        oims = [itk.Image(i.region, i.spacing, i.origin, o) for i, o in zip(imgs, outs)]
-        fl = [itk.GaussianFilter(i, p[1]) for i, p in zip(imgs, params)]
-        for q in fl:
-            q.arith = arith
+        if want("c4"):
+            # batched: 64 uneven volumes vs a loop of plain launches
+            vols, params = wl.c4(dev)
+            outs = [torch.empty_like(v) for v in vols]
+            plan = itk.BatchPlan(vols, outs, [p[1] for p in params], [p[2] for p in params],
+                                 direction=itk.BATCH_FORWARD, arith=arith)
+            nb = plan.voxels
+            plan.autotune(stream)  # z run per radius group, measured on these volumes
+            wb, _ = timed_loop(lambda: plan.run(stream), 5, 2, stream)
+            imgs = [itk.Image(itk.Region([0, 0, 0], list(reversed(v.shape))), p[2], (0.0,) * 3, v)
+                    for v, p in zip(vols, params)]
+            oims = [itk.Image(i.region, i.spacing, i.origin, o) for i, o in zip(imgs, outs)]
+            fl = [itk.GaussianFilter(i, p[1]) for i, p in zip(imgs, params)]
+            for q in fl:
+                q.arith = arith
 
-        def loop():
-            for q, i, o in zip(fl, imgs, oims):
-                q.generate_into(i, o)
-        wl_, _ = timed_loop(loop, 5, 2, stream)
-        extras["c4_batched_64"] = {"voxels": nb, "batched_gvox_s": round(nb * 5 / wb / 1e9, 3),
-                                   "loop_gvox_s": round(nb * 5 / wl_ / 1e9, 3),
-                                   "speedup": round(wl_ / wb, 3)}
-        plan.close()
-        del vols, outs, imgs, oims, fl
-        torch.cuda.empty_cache()
+            def loop():
+                for q, i, o in zip(fl, imgs, oims):
+                    q.generate_into(i, o)
+            wl_, _ = timed_loop(loop, 5, 2, stream)
+            extras["c4_batched_64"] = {"voxels": nb, "batched_gvox_s": round(nb * 5 / wb / 1e9, 3),
+                                       "loop_gvox_s": round(nb * 5 / wl_ / 1e9, 3),
+                                       "speedup": round(wl_ / wb, 3)}
+            plan.close()
+            del vols, outs, imgs, oims, fl
+            torch.cuda.empty_cache()
 
-        # batched adjoint: 16 uneven gradients (extents 48..192), sigma 1.8
-        grads, gparams = wl.c5_batch(dev)
-        gouts = [torch.empty_like(q) for q in grads]
-        gplan = itk.BatchPlan(grads, gouts, [p[1] for p in gparams], [p[2] for p in gparams],
-                              direction=itk.BATCH_BACKWARD)
-        ng = gplan.voxels
-        gplan.autotune(stream)
-        wgb, _ = timed_loop(lambda: gplan.run(stream), 10, 2, stream)
+        if want("c5_batch"):
+            # batched adjoint: 16 uneven gradients (extents 48..192), sigma 1.8
+            grads, gparams = wl.c5_batch(dev)
+            gouts = [torch.empty_like(q) for q in grads]
+            gplan = itk.BatchPlan(grads, gouts, [p[1] for p in gparams], [p[2] for p in gparams],
+                                  direction=itk.BATCH_BACKWARD)
+            ng = gplan.voxels
+            gplan.autotune(stream)
+            wgb, _ = timed_loop(lambda: gplan.run(stream), 10, 2, stream)
 
-        def gloop():
-            for q, o, p in zip(grads, gouts, gparams):
-                itk.gaussian_backward(q, p[1], p[2], out=o)
-        wgl, _ = timed_loop(gloop, 10, 2, stream)
-        extras["c5_batched_adjoint_16"] = {"voxels": ng,
-                                           "batched_gvox_s": round(ng * 10 / wgb / 1e9, 3),
-                                           "loop_gvox_s": round(ng * 10 / wgl / 1e9, 3),
-                                           "speedup": round(wgl / wgb, 3)}
-        gplan.close()
-        del grads, gouts
-        torch.cuda.empty_cache()
+            def gloop():
+                for q, o, p in zip(grads, gouts, gparams):
+                    itk.gaussian_backward(q, p[1], p[2], out=o)
+            wgl, _ = timed_loop(gloop, 10, 2, stream)
+            extras["c5_batched_adjoint_16"] = {"voxels": ng,
+                                               "batched_gvox_s": round(ng * 10 / wgb / 1e9, 3),
+                                               "loop_gvox_s": round(ng * 10 / wgl / 1e9, 3),
+                                               "speedup": round(wgl / wgb, 3)}
+            gplan.close()
+            del grads, gouts
+            torch.cuda.empty_cache()
 
-        # 512^2 2-D and the VecPixel configuration
-        x1, s1, sp1 = wl.c1(dev)
-        w1, _ = timed_loop(lambda: itk.gaussian(x1, s1, sp1, arith=arith), 20, 3, stream)
-        extras["c1_512sq"] = {"gvox_s": round(x1.numel() * 20 / w1 / 1e9, 3)}
-        v3, s3, sp3 = wl.c3(dev)
-        w3, ms3 = timed_loop(lambda: itk.gaussian(v3, s3, sp3, ndim=3, arith=arith), 10, 2,
-                             stream)
-        n3 = v3.numel() // 3
-        extras["c3_vec3_256x256x160"] = {"gvox_s": round(n3 * 10 / w3 / 1e9, 3),
-                                         "hbm_frac_24B": round(24.0 * n3 / (np.mean(ms3) * 1e-3)
-                                                               / 1e9 / HBM_PEAK_GBS, 4)}
-        del v3
-        torch.cuda.empty_cache()
+        if want("c1_c3"):
+            # 512^2 2-D and the VecPixel configuration
+            x1, s1, sp1 = wl.c1(dev)
+            w1, _ = timed_loop(lambda: itk.gaussian(x1, s1, sp1, arith=arith), 20, 3, stream)
+            extras["c1_512sq"] = {"gvox_s": round(x1.numel() * 20 / w1 / 1e9, 3)}
+            v3, s3, sp3 = wl.c3(dev)
+            w3, ms3 = timed_loop(lambda: itk.gaussian(v3, s3, sp3, ndim=3, arith=arith), 10, 2,
+                                 stream)
+            n3 = v3.numel() // 3
+            extras["c3_vec3_256x256x160"] = {"gvox_s": round(n3 * 10 / w3 / 1e9, 3),
+                                             "hbm_frac_24B": round(24.0 * n3 / (np.mean(ms3) * 1e-3)
+                                                                   / 1e9 / HBM_PEAK_GBS, 4)}
+            del v3
+            torch.cuda.empty_cache()
 
-        # the other convolve_axis filters on the same 512^3 volume: MeanFilter
-        # (radius 2, one fused round trip) and BinomialBlurFilter (4 rounds,
-        # one radius-1 round trip each)
-        xm = torch.rand((512, 512, 512), device=dev)
-        ym = torch.empty_like(xm)
-        im = itk.Image(itk.Region([0, 0, 0], [512] * 3), (1.0,) * 3, (0.0,) * 3, xm)
-        om = itk.Image(im.region, (1.0,) * 3, (0.0,) * 3, ym)
-        nm = xm.numel()
-        for name, mode in (("fast", itk.FAST), ("exact", itk.EXACT)):
-            mf = itk.MeanFilter(im, 2)
-            mf.arith = mode
-            wm, msm = timed_loop(lambda: mf.generate_into(im, om), 20, 3, stream)
-            extras["mean_r2_512_" + name] = {
-                "gvox_s": round(nm * 20 / wm / 1e9, 3), "kernel_ms": round(float(np.mean(msm)), 4),
-                "hbm_frac": round(8.0 * nm / (np.mean(msm) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        bf = itk.BinomialBlurFilter(im, 4)
-        bf.arith = arith
-        wbn, msb = timed_loop(lambda: bf.generate_into(im, om), 20, 3, stream)
-        extras["binomial_n4_512"] = {
-            "gvox_s": round(nm * 20 / wbn / 1e9, 3), "ms": round(float(np.mean(msb)), 4),
-            "ms_median": round(float(np.median(msb)), 4), "ms_min": round(float(np.min(msb)), 4),
-            "hbm_frac_4_round_trips": round(4 * 8.0 * nm / (np.mean(msb) * 1e-3) / 1e9
-                                            / HBM_PEAK_GBS, 4)}
-        del xm, ym, im, om
-        torch.cuda.empty_cache()
+        if want("filters"):
+            # the other convolve_axis filters on the same 512^3 volume: MeanFilter
+            # (radius 2, one fused round trip) and BinomialBlurFilter (4 rounds,
+            # one radius-1 round trip each)
+            xm = torch.rand((512, 512, 512), device=dev)
+            ym = torch.empty_like(xm)
+            im = itk.Image(itk.Region([0, 0, 0], [512] * 3), (1.0,) * 3, (0.0,) * 3, xm)
+            om = itk.Image(im.region, (1.0,) * 3, (0.0,) * 3, ym)
+            nm = xm.numel()
+            for name, mode in (("fast", itk.FAST), ("exact", itk.EXACT)):
+                mf = itk.MeanFilter(im, 2)
+                mf.arith = mode
+                wm, msm = timed_loop(lambda: mf.generate_into(im, om), 20, 3, stream)
+                extras["mean_r2_512_" + name] = {
+                    "gvox_s": round(nm * 20 / wm / 1e9, 3), "kernel_ms": round(float(np.mean(msm)), 4),
+                    "hbm_frac": round(8.0 * nm / (np.mean(msm) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            bf = itk.BinomialBlurFilter(im, 4)
+            bf.arith = arith
+            wbn, msb = timed_loop(lambda: bf.generate_into(im, om), 20, 3, stream)
+            extras["binomial_n4_512"] = {
+                "gvox_s": round(nm * 20 / wbn / 1e9, 3), "ms": round(float(np.mean(msb)), 4),
+                "ms_median": round(float(np.median(msb)), 4), "ms_min": round(float(np.min(msb)), 4),
+                "hbm_frac_4_round_trips": round(4 * 8.0 * nm / (np.mean(msb) * 1e-3) / 1e9
+                                                / HBM_PEAK_GBS, 4)}
+            del xm, ym, im, om
+            torch.cuda.empty_cache()
+
         result["extras"] = extras
 
     if not args.no_cpu:

@@ -18,8 +18,8 @@ reference, the accuracy the target is stated at).  Also reported (under
     autotuner) vs a loop of plain single-volume launches, and the batched
     adjoint over 16 uneven gradients vs a loop of plain adjoint launches,
   - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration,
-  - MeanFilter (radius 2, FAST and EXACT) and BinomialBlurFilter (4
-    repetitions) on a 512^3 volume.
+  - MeanFilter (radius 2, FAST and EXACT), BinomialBlurFilter (4
+    repetitions) and UnsharpMaskFilter (sigma 2) on a 512^3 volume.
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
 write) / the average kernel duration from HIP events on the launch stream,
@@ -306,6 +306,14 @@ def main():
                 "ms_median": round(float(np.median(msb)), 4), "ms_min": round(float(np.min(msb)), 4),
                 "hbm_frac_4_round_trips": round(4 * 8.0 * nm / (np.mean(msb) * 1e-3) / 1e9
                                                 / HBM_PEAK_GBS, 4)}
+            # UnsharpMaskFilter, sigma 2 in pixels (radius 6): the blur (one
+            # fused round trip) and the per-pixel step (input, blur, output)
+            uf = itk.UnsharpMaskFilter(im, 2.0)
+            uf.arith = arith
+            wu, msu = timed_loop(lambda: uf.generate_into(im, om), 20, 3, stream)
+            extras["unsharp_s2_512"] = {
+                "gvox_s": round(nm * 20 / wu / 1e9, 3), "ms": round(float(np.mean(msu)), 4),
+                "hbm_frac_20B": round(20.0 * nm / (np.mean(msu) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             del xm, ym, im, om
             torch.cuda.empty_cache()
 

@@ -178,6 +178,34 @@ int itkg_separable_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t 
                                         const itkg_region* requested, void* output,
                                         itkg_arith arith);
 
+/* ---- Gaussian consumers with an index-space sigma ---------------------- */
+
+/* The Gaussian kernel UnsharpMaskFilter and CannyEdgeDetectionFilter build
+ * (src/filters/edges.rs:414-423, 563-572): radius ceil(3 sigma) (sigma in
+ * pixels, written to *radius), weights exp(-(d*d) / (2 sigma sigma)) over
+ * d = -radius..radius divided by their sequential sum, f64, to `out`
+ * (capacity `cap`). */
+int itkg_index_gaussian_kernel(double sigma, double* out, uint64_t cap, uint64_t* radius);
+
+/* UnsharpMaskFilter::generate_region, src/filters/edges.rs:406-452, for
+ * scalar f32 or f64 pixels (the reference's to_f64 of a vector pixel keeps
+ * only its first component): the blur of the request with the kernel above
+ * on every axis, then out = orig + amount * (orig - blur) where
+ * |orig - blur| > threshold, else orig, in f64, narrowed to the pixel type.
+ * input covers output->region padded by the radius, clipped to `largest`.
+ * arith selects the blur's arithmetic (EXACT: bit-identical). */
+int itkg_unsharp_mask_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, double amount,
+                                      double threshold, const itkg_region* largest,
+                                      const itkg_view* input, const itkg_view* output,
+                                      itkg_arith arith, void* stream);
+
+/* The same on dense host buffers (input: the padded, clipped request). */
+int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma,
+                                           double amount, double threshold,
+                                           const itkg_region* largest, const void* input,
+                                           const itkg_region* requested, void* output,
+                                           itkg_arith arith);
+
 /* ---- adjoint (no counterpart in the reference) ------------------------- */
 
 /* Transpose of itkg_gaussian_generate_region for the same (largest, sigma,

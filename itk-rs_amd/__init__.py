@@ -40,6 +40,8 @@ EXPORTS = (
     "itkg_batch_destroy", "itkg_last_error", "itkg_set_path", "itkg_version",
     "itkg_box_kernel", "itkg_separable_input_region_for_output",
     "itkg_separable_generate_region", "itkg_separable_generate_region_host",
+    "itkg_index_gaussian_kernel", "itkg_unsharp_mask_generate_region",
+    "itkg_unsharp_mask_generate_region_host",
 )
 
 
@@ -92,6 +94,14 @@ def _load():
         P(c_region), P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
     L.itkg_separable_generate_region_host.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, P(ctypes.c_uint64), P(ctypes.c_double),
+        P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int]
+    L.itkg_index_gaussian_kernel.argtypes = [ctypes.c_double, P(ctypes.c_double), ctypes.c_uint64,
+                                             P(ctypes.c_uint64)]
+    L.itkg_unsharp_mask_generate_region.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+        P(c_region), P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
+    L.itkg_unsharp_mask_generate_region_host.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
         P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int]
     L.itkg_gaussian_backward.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
@@ -618,6 +628,73 @@ class BinomialBlurFilter(ImageSource):
         D = requested.ndim
         inp = self.source.generate_region(self.input_region_for_output(requested))
         _, nc = _pixel_info(inp.data, D)
+        out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
+        self.generate_into(inp, out)
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Gaussian consumers with an index-space sigma (src/filters/edges.rs)
+# ---------------------------------------------------------------------------
+
+def index_gaussian_kernel(sigma: float) -> Tuple[int, List[float]]:
+    """The kernel of UnsharpMaskFilter / CannyEdgeDetectionFilter
+    (src/filters/edges.rs:414-423): (radius, f64 weights)."""
+    cap = 2 * MAX_RADIUS + 1
+    out = (ctypes.c_double * cap)()
+    r = ctypes.c_uint64()
+    _check(lib.itkg_index_gaussian_kernel(float(sigma), out, cap, ctypes.byref(r)))
+    return int(r.value), list(out)[:2 * int(r.value) + 1]
+
+
+class UnsharpMaskFilter(ImageSource):
+    """UnsharpMaskFilter<S> (src/filters/edges.rs:376-452): I + amount *
+    (I - blur(I)) where |I - blur(I)| > threshold; the same public fields
+    (`source`, `sigma` in pixels, `amount` = 0.5, `threshold` = 0).  Scalar
+    f32 / f64 pixels.  The blur's arithmetic is `arith` (EXACT by default)."""
+
+    arith = EXACT
+
+    def __init__(self, source, sigma: float):
+        self.source = source
+        self.sigma = float(sigma)
+        self.amount = 0.5
+        self.threshold = 0.0
+
+    def with_amount(self, a: float) -> "UnsharpMaskFilter":
+        self.amount = float(a)
+        return self
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def input_region_for_output(self, output_region: Region) -> Region:
+        """edges.rs:401-404: the request padded by ceil(3 sigma), clipped."""
+        return _separable_input_region([kernel_radius(self.sigma, 1.0)] * output_region.ndim,
+                                       self.largest_region(), output_region)
+
+    def generate_into(self, inp: Image, out: Image) -> None:
+        D = out.ndim
+        dt, nc = _pixel_info(inp.data, D)
+        if nc != 1:
+            raise ItkgError(ERR_UNSUPPORTED, "unsharp mask takes scalar pixels")
+        with torch.cuda.device(inp.data.device):
+            _check(lib.itkg_unsharp_mask_generate_region(
+                D, dt, self.sigma, self.amount, self.threshold,
+                ctypes.byref(self.largest_region().to_c()), ctypes.byref(inp.view()),
+                ctypes.byref(out.view()), self.arith, _stream_ptr(inp.data.device)))
+
+    def generate_region(self, requested: Region) -> Image:
+        inp = self.source.generate_region(self.input_region_for_output(requested))
+        _, nc = _pixel_info(inp.data, requested.ndim)
         out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
         self.generate_into(inp, out)
         return out

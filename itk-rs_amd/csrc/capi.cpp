@@ -15,6 +15,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -108,6 +109,24 @@ void weights_of(double sigma_px, uint64_t radius, double* k) {
     for (uint64_t i = 0; i < len; ++i) k[i] /= sum;
 }
 
+// The Gaussian kernel of UnsharpMaskFilter and CannyEdgeDetectionFilter
+// (src/filters/edges.rs:414-423, 563-572): radius ceil(3 sigma) with sigma in
+// index space, w_i = exp(-(d * d) / (2 sigma sigma)) for d = i - radius,
+// divided by their sequential sum.  (sigma = 0 gives NaN weights, as there.)
+uint64_t index_weights(double sigma, double* k, uint64_t cap) {
+    const uint64_t radius = radius_of(sigma, 1.0);
+    if (radius > (uint64_t)ITKG_MAX_RADIUS || 2 * radius + 1 > cap) return UINT64_MAX;
+    const uint64_t len = 2 * radius + 1;
+    for (uint64_t i = 0; i < len; ++i) {
+        const double d = (double)i - (double)radius;
+        k[i] = std::exp(-(d * d) / (2.0 * sigma * sigma));
+    }
+    double sum = 0.0;
+    for (uint64_t i = 0; i < len; ++i) sum += k[i];
+    for (uint64_t i = 0; i < len; ++i) k[i] /= sum;
+    return radius;
+}
+
 // ---- per-call plan (src/filters/gaussian.rs:59-90) ----------------------
 
 // The regions of a plan whose radii are set: the input region (the request
@@ -163,7 +182,8 @@ int make_plan(int D, double sigma, const double* spacing, const itkg_region* lar
 // BinomialBlurFilter hand convolve_axis their own kernels.  The weights must
 // be symmetric (w_k = w_{2r-k}): the adjoint kernels rely on it.
 int make_plan_weights(int D, const uint64_t* radii, const double* weights,
-                      const itkg_region* largest, const itkg_region& req, Plan* p) {
+                      const itkg_region* largest, const itkg_region& req, Plan* p,
+                      bool check_symmetric = true) {
     if (!radii || !weights) return fail(ITKG_ERR_INVALID_ARG, "null radii or weights");
     if (int rc = plan_common(D, largest, req, p)) return rc;
     const double* w = weights;
@@ -174,7 +194,7 @@ int make_plan_weights(int D, const uint64_t* radii, const double* weights,
                         (unsigned long long)r, ITKG_MAX_RADIUS);
         p->radii[d] = (int64_t)r;
         for (uint64_t k = 0; k <= 2 * r; ++k) {
-            if (!(w[k] == w[2 * r - k]))
+            if (check_symmetric && !(w[k] == w[2 * r - k]))
                 return fail(ITKG_ERR_INVALID_ARG, "axis %d weights are not symmetric", d);
             p->w[d][k] = w[k];
         }
@@ -874,6 +894,132 @@ int itkg_separable_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t 
     int rc = make_plan_weights(ndim, radii, weights, largest, *requested, &p);
     if (rc) return rc;
     return host_forward(p, dtype, ncomp, input, output, arith);
+}
+
+int itkg_index_gaussian_kernel(double sigma, double* out, uint64_t cap, uint64_t* radius) {
+    clear_error();
+    if (!out || !radius) return fail(ITKG_ERR_INVALID_ARG, "null output");
+    const uint64_t r = radius_of(sigma, 1.0);
+    if (r > (uint64_t)ITKG_MAX_RADIUS)
+        return fail(ITKG_ERR_UNSUPPORTED, "radius exceeds %d", ITKG_MAX_RADIUS);
+    if (cap < 2 * r + 1) return fail(ITKG_ERR_INVALID_ARG, "capacity below 2r+1");
+    *radius = index_weights(sigma, out, cap);
+    return ITKG_OK;
+}
+
+namespace {
+
+// UnsharpMaskFilter::generate_region (src/filters/edges.rs:406-452) on
+// device views: the blur of the requested region (the separable path with
+// the filter's own weights; each pass clamps to the same extents as the
+// reference's whole-buffer passes for every requested pixel), then the
+// per-pixel step.
+int unsharp_impl(int ndim, itkg_dtype dtype, double sigma, double amount, double threshold,
+                 const itkg_region* largest, const itkg_view& in, const itkg_view& out,
+                 itkg_arith arith, hipStream_t s) {
+    double w[kMaxTaps * ITKG_MAX_DIM];
+    const uint64_t r = radius_of(sigma, 1.0);
+    if (r > (uint64_t)ITKG_MAX_RADIUS)
+        return fail(ITKG_ERR_UNSUPPORTED, "radius exceeds %d", ITKG_MAX_RADIUS);
+    uint64_t radii[ITKG_MAX_DIM] = {0, 0, 0, 0};
+    for (int d = 0; d < ndim; ++d) {
+        index_weights(sigma, w + d * (2 * r + 1), kMaxTaps);
+        radii[d] = r;
+    }
+    Plan p;
+    int rc = make_plan_weights(ndim, radii, w, largest, out.region, &p, false);
+    if (rc) return rc;
+    if ((rc = check_view(ndim, &in, p.in_reg, "input"))) return rc;
+    if ((rc = check_view(ndim, &out, p.req, "output"))) return rc;
+    const int64_t n = region_len(ndim, p.req);
+    if (n == 0) return ITKG_OK;
+    const size_t px = dtype_size(dtype);
+    void* blur = nullptr;
+    hipError_t e = hipMallocAsync(&blur, (size_t)n * px, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(unsharp blur)");
+    const itkg_view bv = dense_view(ndim, blur, p.req);
+    rc = forward_impl(p, dtype, 1, in, bv, arith, s, nullptr);
+    if (!rc) {
+        auto fill = [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            UnsharpArgs<T> a{};
+            a.in = static_cast<const T*>(in.data) + view_offset(ndim, in, p.req.index);
+            a.blur = static_cast<const T*>(blur);
+            a.out = static_cast<T*>(out.data) + view_offset(ndim, out, p.req.index);
+            for (int d = 0; d < ITKG_MAX_DIM; ++d) {
+                a.n[d] = d < ndim ? p.req.size[d] : 1;
+                a.in_stride[d] = d < ndim ? in.stride[d] : 0;
+                a.blur_stride[d] = d < ndim ? bv.stride[d] : 0;
+                a.out_stride[d] = d < ndim ? out.stride[d] : 0;
+            }
+            a.amount = amount;
+            a.threshold = threshold;
+            return launch_unsharp<T>(a, s);
+        };
+        e = dtype == ITKG_F64 ? fill((double*)nullptr) : fill((float*)nullptr);
+        if (e != hipSuccess) rc = hip_fail(e, "unsharp launch");
+    }
+    (void)hipFreeAsync(blur, s);
+    return rc;
+}
+
+}  // namespace
+
+int itkg_unsharp_mask_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, double amount,
+                                      double threshold, const itkg_region* largest,
+                                      const itkg_view* input, const itkg_view* output,
+                                      itkg_arith arith, void* stream) {
+    clear_error();
+    if (!args_ok(ndim, dtype, 1)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype");
+    if (!input || !output || !largest) return fail(ITKG_ERR_INVALID_ARG, "null argument");
+    return unsharp_impl(ndim, dtype, sigma, amount, threshold, largest, *input, *output, arith,
+                        static_cast<hipStream_t>(stream));
+}
+
+int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma,
+                                           double amount, double threshold,
+                                           const itkg_region* largest, const void* input,
+                                           const itkg_region* requested, void* output,
+                                           itkg_arith arith) {
+    clear_error();
+    if (!args_ok(ndim, dtype, 1)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype");
+    if (!largest || !requested) return fail(ITKG_ERR_INVALID_ARG, "null region");
+    if (!region_ok(ndim, *largest) || !region_ok(ndim, *requested))
+        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
+    int64_t rr[ITKG_MAX_DIM] = {0, 0, 0, 0};
+    const uint64_t r = radius_of(sigma, 1.0);
+    if (r > (uint64_t)ITKG_MAX_RADIUS)
+        return fail(ITKG_ERR_UNSUPPORTED, "radius exceeds %d", ITKG_MAX_RADIUS);
+    for (int d = 0; d < ndim; ++d) rr[d] = (int64_t)r;
+    const itkg_region in_reg = clipped(ndim, padded(ndim, *requested, rr), *largest);
+    const int64_t nin = region_len(ndim, in_reg), nout = region_len(ndim, *requested);
+    if (nout == 0) return ITKG_OK;
+    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
+    const size_t px = dtype_size(dtype);
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    int rc = ITKG_OK;
+    void *din = nullptr, *dout = nullptr;
+    e = hipMallocAsync(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "unsharp host staging");
+    if (!rc) {
+        const itkg_view vin = dense_view(ndim, din, in_reg);
+        const itkg_view vout = dense_view(ndim, dout, *requested);
+        rc = unsharp_impl(ndim, dtype, sigma, amount, threshold, largest, vin, vout, arith, s);
+    }
+    if (!rc) {
+        e = hipMemcpyAsync(output, dout, (size_t)nout * px, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy out");
+    }
+    if (din) (void)hipFreeAsync(din, s);
+    if (dout) (void)hipFreeAsync(dout, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
+    return rc;
 }
 
 int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,

@@ -48,6 +48,21 @@ template <typename T>
 hipError_t launch_axis_adjoint_range(const AxisPassArgs<T>& a, int ncomp, int64_t g_lo,
                                      int64_t g_hi, hipStream_t s);
 
+// UnsharpMaskFilter's per-pixel step over region size n (axis 0 fastest;
+// strides in pixels): out = orig + amount * (orig - blur) where
+// |orig - blur| > threshold, else orig, in f64.
+template <typename T>
+struct UnsharpArgs {
+    const T* in;
+    const T* blur;
+    T* out;
+    int64_t n[4];
+    int64_t in_stride[4], blur_stride[4], out_stride[4];
+    double amount, threshold;
+};
+template <typename T>
+hipError_t launch_unsharp(const UnsharpArgs<T>& a, hipStream_t s);
+
 // Fused one-round-trip 3-D kernel for f32 scalar pixels (forward or adjoint).
 // Destination pixel j reads source pixels j + shift + k - r, k = 0..2r
 // (forward: clamped into the source; adjoint: zero outside it).

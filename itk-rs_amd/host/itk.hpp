@@ -11,6 +11,7 @@
 //   GaussianFilter                   src/filters/gaussian.rs:13-91
 //   MeanFilter, BoxMeanFilterN       src/filters/mean.rs:15-145
 //   BinomialBlurFilter               src/filters/binomial_blur.rs:17-90
+//   UnsharpMaskFilter                src/filters/edges.rs:376-452
 //   stream_full / split_region       src/source.rs:81-97, src/region_splitter.rs:8-39
 //   VecPixel                         src/pixel.rs:61-99
 //
@@ -417,6 +418,46 @@ struct BinomialBlurFilter {
                                            separable_arith);
         }
         return cur;
+    }
+};
+
+// UnsharpMaskFilter<S> (src/filters/edges.rs:376-452) with the reference's
+// public fields; scalar f32 / f64 pixels.  The blur's arithmetic is
+// separable_arith.
+template <class S>
+struct UnsharpMaskFilter {
+    using pixel_type = typename S::pixel_type;
+    static constexpr size_t dim = S::dim;
+    using P = pixel_type;
+    static constexpr size_t D = dim;
+    static_assert(PixelTraits<P>::ncomp == 1, "unsharp mask takes scalar pixels");
+
+    S source;
+    double sigma;
+    double amount = 0.5;
+    double threshold = 0.0;
+
+    UnsharpMaskFilter(S src, double s) : source(std::move(src)), sigma(s) {}
+    UnsharpMaskFilter with_amount(double a) && {
+        amount = a;
+        return std::move(*this);
+    }
+
+    Region<D> largest_region() const { return source.largest_region(); }
+    std::array<double, D> spacing() const { return source.spacing(); }
+    std::array<double, D> origin() const { return source.origin(); }
+    Region<D> input_region_for_output(const Region<D>& output) const {
+        return output.padded(kernel_radius(sigma, 1.0)).clipped_to(source.largest_region());
+    }
+    Image<P, D> generate_region(Region<D> requested) const {
+        const Region<D> bounds = source.largest_region();
+        const Image<P, D> input = source.generate_region(input_region_for_output(requested));
+        Image<P, D> out = Image<P, D>::allocate(requested, spacing(), origin(), P{});
+        const itkg_region cb = bounds.to_c(), cr = requested.to_c();
+        check(itkg_unsharp_mask_generate_region_host((int32_t)D, PixelTraits<P>::dtype, sigma, amount,
+                                                     threshold, &cb, input.data.data(), &cr,
+                                                     out.data.data(), separable_arith));
+        return out;
     }
 };
 

@@ -47,6 +47,8 @@
  *     ::generate_region
  *   BinomialBlurFilter              src/filters/binomial_blur.rs:57-89
  *     ::generate_region (its own per-pass padding rule, loop for loop)
+ *   UnsharpMaskFilter               src/filters/edges.rs:406-452
+ *     ::generate_region (whole-buffer passes, the f64 per-pixel step)
  *
  *   mean / binomial (f32)          : interior-impulse closed forms (box
  *                                  products; binomial coefficients / 4^n,
@@ -339,6 +341,71 @@ int NAME(int D, int ncomp, const T* img, const int64_t* img_index,              
 }
 DEFINE_BINOMIAL(orc_binomial_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
 DEFINE_BINOMIAL(orc_binomial_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
+
+/* ---- UnsharpMaskFilter: src/filters/edges.rs:406-452, loop for loop:
+ * radius ceil(3 sigma) (index space), its own kernel formula; the input is
+ * the request padded by the radius, clipped; every pass convolves the WHOLE
+ * padded buffer (its output region is the buffer's region); then, per
+ * requested pixel, in f64: edge = orig - blur, orig + amount * edge where
+ * |edge| > threshold, else orig, cast to T.  Scalar pixels. */
+void orc_index_kernel(double sigma, uint64_t radius, double* k) {
+    uint64_t len = 2 * radius + 1;
+    for (uint64_t i = 0; i < len; ++i) {
+        double d = (double)i - (double)radius;
+        k[i] = exp(-(d * d) / (2.0 * sigma * sigma));
+    }
+    double sum = 0.0;
+    for (uint64_t i = 0; i < len; ++i) sum += k[i];
+    for (uint64_t i = 0; i < len; ++i) k[i] /= sum;
+}
+
+#define DEFINE_UNSHARP(NAME, T, LEAF, CONV)                                      \
+int NAME(int D, const T* img, const int64_t* img_index, const int64_t* img_size, \
+         double sigma, double amount, double threshold,                          \
+         const int64_t* req_index, const int64_t* req_size, T* out) {            \
+    if (D < 1 || D > ORC_MAX_DIM) return -1;                                     \
+    orc_region bounds = region_make(D, img_index, img_size);                     \
+    orc_region req = region_make(D, req_index, req_size);                        \
+    uint64_t radius = orc_kernel_radius(sigma, 1.0);                             \
+    uint64_t radii[ORC_MAX_DIM] = {0};                                           \
+    for (int d = 0; d < D; ++d) radii[d] = radius;                               \
+    orc_region pad = region_clipped(D, region_padded(D, req, radii), bounds);    \
+    orc_region in_reg;                                                           \
+    T* input = LEAF(D, 1, img, bounds, pad, &in_reg);                            \
+    uint64_t klen = 2 * radius + 1;                                              \
+    double* kern = (double*)malloc(klen * sizeof(double));                       \
+    orc_index_kernel(sigma, radius, kern);                                       \
+    int64_t n = region_len(D, in_reg);                                           \
+    T* blurred = (T*)malloc((size_t)(n > 0 ? n : 1) * sizeof(T));               \
+    memcpy(blurred, input, (size_t)n * sizeof(T));                              \
+    for (int axis = 0; axis < D; ++axis) {                                       \
+        T* nxt = (T*)malloc((size_t)(n > 0 ? n : 1) * sizeof(T));               \
+        CONV(D, 1, blurred, in_reg, axis, kern, klen, in_reg, nxt);              \
+        free(blurred);                                                           \
+        blurred = nxt;                                                           \
+    }                                                                            \
+    int64_t m = region_len(D, req);                                              \
+    int64_t idx[ORC_MAX_DIM];                                                    \
+    for (int d = 0; d < D; ++d) idx[d] = req.index[d];                           \
+    for (int64_t o = 0; o < m; ++o) {                                            \
+        int64_t f = flat_index(D, in_reg, idx);                                  \
+        double orig = (double)input[f];                                          \
+        double blur = (double)blurred[f];                                        \
+        double edge = orig - blur;                                               \
+        double enhanced = fabs(edge) > threshold ? orig + amount * edge : orig;  \
+        out[o] = (T)enhanced;                                                    \
+        for (int d = 0; d < D; ++d) {                                            \
+            if (++idx[d] < req.index[d] + req.size[d]) break;                    \
+            idx[d] = req.index[d];                                               \
+        }                                                                        \
+    }                                                                            \
+    free(kern);                                                                  \
+    free(blurred);                                                               \
+    free(input);                                                                 \
+    return 0;                                                                    \
+}
+DEFINE_UNSHARP(orc_unsharp_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
+DEFINE_UNSHARP(orc_unsharp_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
 
 /* ---- adjoint in f64, by scattering the forward taps -------------------- */
 /* No reference counterpart.  The forward operator is A = A_{D-1} ... A_0

@@ -21,7 +21,7 @@ The other convolve_axis filters: ``mean_generate_region`` and
 ``binomial_generate_region`` (C restatements of src/filters/mean.rs:59-83,
 126-144 and src/filters/binomial_blur.rs:57-89), with
 ``dense_mean_operator`` / ``dense_binomial_operator`` as their brute-force
-checks.
+checks; ``unsharp_generate_region`` restates src/filters/edges.rs:406-452.
 """
 from __future__ import annotations
 
@@ -77,6 +77,13 @@ def lib():
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, i64p, i64p, ctypes.c_uint64,
                            i64p, i64p, ctypes.c_void_p]
+        L.orc_index_kernel.restype = None
+        L.orc_index_kernel.argtypes = [ctypes.c_double, ctypes.c_uint64, f64p]
+        for name in ("orc_unsharp_generate_region_f32", "orc_unsharp_generate_region_f64"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_int, ctypes.c_void_p, i64p, i64p, ctypes.c_double,
+                           ctypes.c_double, ctypes.c_double, i64p, i64p, ctypes.c_void_p]
         L.orc_adjoint_f64.restype = ctypes.c_int
         L.orc_adjoint_f64.argtypes = [ctypes.c_int, ctypes.c_int, i64p, i64p, f64p, ctypes.c_double,
                                       i64p, i64p, ctypes.c_void_p, ctypes.c_void_p, i64p, i64p]
@@ -191,6 +198,32 @@ def binomial_generate_region(img: np.ndarray, img_index, reps, req_index, req_si
     the C oracle (src/filters/binomial_blur.rs:57-89)."""
     return _filter_call("orc_binomial_generate_region", img, img_index, int(reps), req_index,
                         req_size, ncomp)
+
+
+def index_kernel(sigma: float) -> np.ndarray:
+    """src/filters/edges.rs:414-423 (via the C oracle): f64 weights, radius
+    ceil(3 sigma)."""
+    r = kernel_radius(sigma, 1.0)
+    out = np.zeros(2 * r + 1, dtype=np.float64)
+    lib().orc_index_kernel(float(sigma), r, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def unsharp_generate_region(img: np.ndarray, img_index, sigma, amount, threshold, req_index,
+                            req_size) -> np.ndarray:
+    """UnsharpMaskFilter{source: img, sigma, amount, threshold}.generate_region(req)
+    via the C oracle (src/filters/edges.rs:406-452), scalar pixels."""
+    img = np.ascontiguousarray(img)
+    D = len(img_index)
+    img_size = list(reversed(img.shape[:D]))
+    fn = lib().orc_unsharp_generate_region_f32 if img.dtype == np.float32 else \
+        lib().orc_unsharp_generate_region_f64
+    out = np.zeros(_shape(req_size, 1), dtype=img.dtype)
+    rc = fn(D, img.ctypes.data, _i64(img_index), _i64(img_size), float(sigma), float(amount),
+            float(threshold), _i64(req_index), _i64(req_size), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    return out
 
 
 def ref_faithful(img: np.ndarray, img_index, spacing, sigma, req_index, req_size, ncomp=1,

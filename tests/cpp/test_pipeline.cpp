@@ -218,6 +218,22 @@ static void test_binomial_effective_sigma() {
     CHECK(std::fabs(s - 1.0) < 1e-10, "sigma=%g", s);
 }
 
+// UnsharpMaskFilter (src/filters/edges.rs:406-452): amount 0 leaves the
+// image as it is; on an impulse the centre is sharpened above its value and
+// its neighbours go below zero.
+static void test_unsharp_mask() {
+    auto img = solid_image<2>({15, 15}, 0.0f);
+    img.set_pixel(Index<2>{{7, 7}}, 1.0f);
+    auto same = UnsharpMaskFilter<Image<float, 2>>(img, 1.0).with_amount(0.0);
+    auto out0 = same.generate_region(same.largest_region());
+    for (size_t i = 0; i < out0.data.size(); ++i)
+        CHECK(out0.data[i] == img.data[i], "amount 0 changed pixel %zu", i);
+    UnsharpMaskFilter<Image<float, 2>> f(img, 1.0);
+    auto out = f.generate_region(f.largest_region());
+    CHECK(out.get_pixel(Index<2>{{7, 7}}) > 1.0f, "centre %g", out.get_pixel(Index<2>{{7, 7}}));
+    CHECK(out.get_pixel(Index<2>{{8, 7}}) < 0.0f, "neighbour %g", out.get_pixel(Index<2>{{8, 7}}));
+}
+
 int main() {
 #define RUN(t)                  \
     t();                        \
@@ -239,6 +255,7 @@ int main() {
     RUN(test_binomial_constant_image_preserved);
     RUN(test_binomial_impulse_spreads);
     RUN(test_binomial_effective_sigma);
+    RUN(test_unsharp_mask);
     std::printf("all passed\n");
     return 0;
 }

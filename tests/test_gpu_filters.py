@@ -166,3 +166,57 @@ def test_argument_errors(itk, dev):
     with pytest.raises(itk.ItkgError) as e:
         itk.box_kernel(200)
     assert e.value.code == itk.ERR_UNSUPPORTED
+
+
+# --- UnsharpMaskFilter (src/filters/edges.rs:406-452) -----------------------
+
+@pytest.mark.parametrize("size,sigma,amount,thr,req,dtype", [
+    ((64, 48, 40), 1.0, 0.5, 0.0, None, np.float32),
+    ((70, 50, 30), 2.0, 1.5, 0.03, ((5, 0, 3), (40, 50, 20)), np.float32),
+    ((96, 80), 1.4, 0.5, 0.0, ((0, 10), (96, 30)), np.float32),
+    ((200,), 3.0, 0.8, 0.01, None, np.float32),
+    ((30, 20, 10), 0.8, 0.5, 0.0, ((4, 4, 2), (20, 10, 6)), np.float64),
+    ((20, 18, 16), 0.0, 0.5, 0.0, None, np.float32),    # NaN weights: the identity, as the reference
+    ((20, 18, 16), -1.0, 0.5, 0.0, None, np.float32),   # radius 0: the identity
+])
+def test_unsharp_exact_matches_oracle(itk, orc, dev, path, size, sigma, amount, thr, req, dtype):
+    D = len(size)
+    x = rand(tuple(reversed(size)), 1, dtype)
+    src = _img(itk, x, dev, D)
+    f = itk.UnsharpMaskFilter(src, sigma).with_amount(amount)
+    f.threshold = thr
+    reg = src.region if req is None else itk.Region(list(req[0]), list(req[1]))
+    out = f.generate_region(reg)
+    torch.cuda.synchronize()
+    ref = orc.unsharp_generate_region(x, [0] * D, sigma, amount, thr, list(reg.index), list(reg.size))
+    assert np.array_equal(out.data.cpu().numpy(), ref)
+
+
+def test_unsharp_fast_and_host_entry(itk, dev):
+    x = torch.rand((80, 96, 112), device=dev)
+    src = itk.Image(itk.Region([0, 0, 0], [112, 96, 80]), (1.0,) * 3, (0.0,) * 3, x)
+    ex = itk.UnsharpMaskFilter(src, 1.5)
+    fa = itk.UnsharpMaskFilter(src, 1.5)
+    fa.arith = itk.FAST
+    a, b = ex.generate_region(src.region).data, fa.generate_region(src.region).data
+    assert float((a - b).abs().max()) / float(a.abs().max()) <= 1e-5
+    # the host-buffer entry point gives the device result bit for bit
+    xh = x.cpu().numpy()
+    out = np.empty_like(xh)
+    reg = src.region.to_c()
+    rc = itk.lib.itkg_unsharp_mask_generate_region_host(3, itk.F32, 1.5, 0.5, 0.0, ctypes.byref(reg),
+                                                        xh.ctypes.data, ctypes.byref(reg),
+                                                        out.ctypes.data, itk.EXACT)
+    assert rc == 0, itk.lib.itkg_last_error()
+    assert np.array_equal(out, a.cpu().numpy())
+
+
+def test_unsharp_streamed_and_vector_refused(itk, dev):
+    x = torch.rand((40, 50, 60), device=dev)
+    src = itk.Image(itk.Region([0, 0, 0], [60, 50, 40]), (1.0,) * 3, (0.0,) * 3, x)
+    f = itk.UnsharpMaskFilter(src, 1.2)
+    assert torch.equal(f.generate_region(src.region).data, itk.stream_full(f, 5).data)
+    v = itk.Image(src.region, src.spacing, src.origin, torch.rand((40, 50, 60, 3), device=dev))
+    with pytest.raises(itk.ItkgError) as e:
+        itk.UnsharpMaskFilter(v, 1.0).generate_region(v.region)
+    assert e.value.code == itk.ERR_UNSUPPORTED

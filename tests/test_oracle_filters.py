@@ -1,7 +1,10 @@
 """Pins the oracle's restatements of the other convolve_axis filters --
-MeanFilter / BoxMeanFilterN (src/filters/mean.rs:15-18, 59-83, 126-144) and
-BinomialBlurFilter (src/filters/binomial_blur.rs:57-89) -- before the GPU
-path is checked against them.
+MeanFilter / BoxMeanFilterN (src/filters/mean.rs:15-18, 59-83, 126-144),
+BinomialBlurFilter (src/filters/binomial_blur.rs:57-89) and
+UnsharpMaskFilter (src/filters/edges.rs:406-452) -- before the GPU path is
+checked against them.  UnsharpMaskFilter has no tests in the reference: its
+pins are the kernel's closed form, identity cases, the dense operator and
+crop consistency.
 
 Checks: the reference's own unit tests for both filters (mean.rs:261-294,
 binomial_blur.rs:92-129), interior-impulse closed forms that are exact in
@@ -155,3 +158,64 @@ def test_binomial_is_not_one_wide_kernel_at_the_border(orc):
     wide = orc.apply_dense([orc.axis_matrix(w, 0, 12, 0, 12)], x)
     assert np.allclose(out[n:-n], wide[n:-n], atol=1e-12)
     assert not np.allclose(out[:n], wide[:n], atol=1e-9)
+
+
+# --- UnsharpMaskFilter (edges.rs:406-452) ------------------------------------
+
+@pytest.mark.parametrize("sigma", [0.5, 1.0, 1.7, 3.0])
+def test_index_kernel_closed_form(orc, sigma):
+    k = orc.index_kernel(sigma)
+    r = math.ceil(3.0 * sigma)
+    raw = [math.exp(-((i - r) ** 2) / (2.0 * sigma * sigma)) for i in range(2 * r + 1)]
+    s = 0.0
+    for v in raw:
+        s += v
+    assert len(k) == 2 * r + 1
+    assert np.array_equal(k, np.array(raw) / s)
+    assert np.array_equal(k, k[::-1])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_unsharp_identity_cases(orc, dtype):
+    x = rand((9, 11, 13), dtype=dtype)
+    idx, size = [0, 0, 0], [13, 11, 9]
+    # amount 0: orig + 0 * edge = orig; threshold +inf: no pixel passes;
+    # sigma < 0: radius 0, weight 1, so blur = orig and edge = 0
+    assert np.array_equal(orc.unsharp_generate_region(x, idx, 1.2, 0.0, 0.0, idx, size), x)
+    assert np.array_equal(orc.unsharp_generate_region(x, idx, 1.2, 0.7, math.inf, idx, size), x)
+    assert np.array_equal(orc.unsharp_generate_region(x, idx, -1.0, 0.7, 0.0, idx, size), x)
+
+
+@pytest.mark.parametrize("size,sigma,amount,thr,req", [
+    ((10, 9, 8), 1.0, 0.5, 0.0, ((0, 0, 0), (10, 9, 8))),
+    ((14, 12), 1.4, 1.5, 0.05, ((2, 0), (8, 12))),
+    ((11, 10, 9), 0.8, 0.5, 0.02, ((3, 2, 1), (6, 5, 5))),
+])
+def test_unsharp_vs_dense(orc, size, sigma, amount, thr, req):
+    D = len(size)
+    x = rand(tuple(reversed(size)), dtype=np.float64)
+    out = orc.unsharp_generate_region(x, [0] * D, sigma, amount, thr, list(req[0]), list(req[1]))
+    r = math.ceil(3.0 * sigma)
+    w = np.array([math.exp(-((i - r) ** 2) / (2.0 * sigma * sigma)) for i in range(2 * r + 1)])
+    w = w / w.sum()
+    in_i, in_s = orc.clipped(*orc.padded(list(req[0]), list(req[1]), [r] * D), [0] * D, list(size))
+    mats = [orc.axis_matrix(w, req[0][a], req[1][a], in_i[a], in_s[a]) for a in range(D)]
+    sub = x[tuple(slice(i, i + s) for i, s in zip(reversed(in_i), reversed(in_s)))]
+    blur = orc.apply_dense(mats, sub)
+    orig = x[tuple(slice(i, i + s) for i, s in zip(reversed(req[0]), reversed(req[1])))]
+    edge = orig - blur
+    want = np.where(np.abs(edge) > thr, orig + amount * edge, orig)
+    near = np.abs(np.abs(edge) - thr) < 1e-9     # a threshold tie could go either way
+    assert np.allclose(out[~near], want[~near], rtol=0, atol=1e-12)
+
+
+def test_unsharp_subregion_is_crop_of_whole(orc):
+    # each pass of the reference convolves the whole padded buffer; for the
+    # requested pixels that equals the Gaussian path's shrinking pass regions,
+    # so a sub-request is a crop of the whole-image result, bit for bit
+    x = rand((20, 22, 24))
+    full = orc.unsharp_generate_region(x, [0, 0, 0], 1.3, 0.6, 0.01, [0, 0, 0], [24, 22, 20])
+    for req in (((0, 0, 0), (10, 22, 5)), ((7, 3, 9), (9, 11, 6)), ((20, 18, 16), (4, 4, 4))):
+        sub = orc.unsharp_generate_region(x, [0, 0, 0], 1.3, 0.6, 0.01, list(req[0]), list(req[1]))
+        crop = full[tuple(slice(i, i + s) for i, s in zip(reversed(req[0]), reversed(req[1])))]
+        assert np.array_equal(sub, crop)

@@ -19,7 +19,8 @@ reference, the accuracy the target is stated at).  Also reported (under
     adjoint over 16 uneven gradients vs a loop of plain adjoint launches,
   - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration,
   - MeanFilter (radius 2, FAST and EXACT), BinomialBlurFilter (4
-    repetitions) and UnsharpMaskFilter (sigma 2) on a 512^3 volume.
+    repetitions), UnsharpMaskFilter (sigma 2) and RecursiveGaussianFilter
+    (sigma 2) on a 512^3 volume.
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
 write) / the average kernel duration from HIP events on the launch stream,
@@ -314,6 +315,13 @@ def main():
             extras["unsharp_s2_512"] = {
                 "gvox_s": round(nm * 20 / wu / 1e9, 3), "ms": round(float(np.mean(msu)), 4),
                 "hbm_frac_20B": round(20.0 * nm / (np.mean(msu) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            # RecursiveGaussianFilter, sigma 2: widen to f64, a causal and an
+            # anticausal line pass per axis (f64 in place), narrow
+            rf = itk.RecursiveGaussianFilter(im, 2.0)
+            wr, msr = timed_loop(lambda: rf.generate_into(im, om), 10, 2, stream)
+            extras["recursive_s2_512"] = {
+                "gvox_s": round(nm * 10 / wr / 1e9, 3), "ms": round(float(np.mean(msr)), 4),
+                "hbm_frac_104B": round(104.0 * nm / (np.mean(msr) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             del xm, ym, im, om
             torch.cuda.empty_cache()
 

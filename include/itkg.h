@@ -206,6 +206,33 @@ int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, doubl
                                            const itkg_region* requested, void* output,
                                            itkg_arith arith);
 
+/* ---- recursive (IIR) Gaussian ----------------------------------------- */
+
+/* yvv_coeffs, src/filters/recursive_gaussian.rs:33-47: the Young - van Vliet
+ * third-order coefficients (B, a1, a2, a3) for sigma in pixels (clamped below
+ * at 0.5), written to coef[0..3]. */
+int itkg_yvv_coefficients(double sigma_px, double* coef);
+
+/* RecursiveGaussianFilter::generate_region (axis = -1, every axis in order;
+ * src/filters/recursive_gaussian.rs:147-176) and
+ * RecursiveGaussianAxisFilter::generate_region (axis = 0..ndim-1;
+ * recursive_gaussian.rs:219-240), scalar f32 / f64 pixels: the whole largest
+ * region widened to f64; per filtered axis (of more than one sample), over
+ * every line along it, a causal then an anticausal pass (recursive_gaussian.rs:
+ * 55-86) with sigma / spacing[axis]; the requested region narrowed back.
+ * input covers `largest` (the filters' input_region_for_output); output's
+ * region is the request.  Bit-identical to the reference (f64 throughout). */
+int itkg_recursive_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, double sigma,
+                                            const double* spacing, int32_t axis,
+                                            const itkg_region* largest, const itkg_view* input,
+                                            const itkg_view* output, void* stream);
+
+/* The same on dense host buffers (input: the whole largest region). */
+int itkg_recursive_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma,
+                                                 const double* spacing, int32_t axis,
+                                                 const itkg_region* largest, const void* input,
+                                                 const itkg_region* requested, void* output);
+
 /* ---- adjoint (no counterpart in the reference) ------------------------- */
 
 /* Transpose of itkg_gaussian_generate_region for the same (largest, sigma,

@@ -41,7 +41,8 @@ EXPORTS = (
     "itkg_box_kernel", "itkg_separable_input_region_for_output",
     "itkg_separable_generate_region", "itkg_separable_generate_region_host",
     "itkg_index_gaussian_kernel", "itkg_unsharp_mask_generate_region",
-    "itkg_unsharp_mask_generate_region_host",
+    "itkg_unsharp_mask_generate_region_host", "itkg_yvv_coefficients",
+    "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
 )
 
 
@@ -103,6 +104,13 @@ def _load():
     L.itkg_unsharp_mask_generate_region_host.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
         P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int]
+    L.itkg_yvv_coefficients.argtypes = [ctypes.c_double, P(ctypes.c_double)]
+    L.itkg_recursive_gaussian_generate_region.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_double, P(ctypes.c_double), ctypes.c_int32,
+        P(c_region), P(c_view), P(c_view), ctypes.c_void_p]
+    L.itkg_recursive_gaussian_generate_region_host.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_double, P(ctypes.c_double), ctypes.c_int32,
+        P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p]
     L.itkg_gaussian_backward.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
         P(c_region), P(c_view), P(c_view), ctypes.c_int32, ctypes.c_void_p]
@@ -698,6 +706,75 @@ class UnsharpMaskFilter(ImageSource):
         out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
         self.generate_into(inp, out)
         return out
+
+
+# ---------------------------------------------------------------------------
+# recursive (IIR) Gaussian (src/filters/recursive_gaussian.rs)
+# ---------------------------------------------------------------------------
+
+def yvv_coefficients(sigma_px: float) -> Tuple[float, float, float, float]:
+    """yvv_coeffs (src/filters/recursive_gaussian.rs:33-47): (B, a1, a2, a3)."""
+    c = (ctypes.c_double * 4)()
+    _check(lib.itkg_yvv_coefficients(float(sigma_px), c))
+    return tuple(c)
+
+
+class RecursiveGaussianFilter(ImageSource):
+    """RecursiveGaussianFilter<S> (src/filters/recursive_gaussian.rs:129-177):
+    the Young - van Vliet recursive Gaussian along every axis (physical
+    sigma), f64 inside, on the whole image; scalar f32 / f64 pixels."""
+
+    _axis = -1
+
+    def __init__(self, source, sigma: float):
+        self.source = source
+        self.sigma = float(sigma)
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def input_region_for_output(self, output_region: Region) -> Region:
+        """recursive_gaussian.rs:156-158: the whole image (infinite support)."""
+        return self.largest_region()
+
+    def generate_into(self, inp: Image, out: Image) -> None:
+        D = out.ndim
+        dt, nc = _pixel_info(inp.data, D)
+        if nc != 1:
+            raise ItkgError(ERR_UNSUPPORTED, "the recursive Gaussian takes scalar pixels")
+        with torch.cuda.device(inp.data.device):
+            _check(lib.itkg_recursive_gaussian_generate_region(
+                D, dt, self.sigma, _doubles(self.spacing), self._axis,
+                ctypes.byref(self.largest_region().to_c()), ctypes.byref(inp.view()),
+                ctypes.byref(out.view()), _stream_ptr(inp.data.device)))
+
+    def generate_region(self, requested: Region) -> Image:
+        inp = self.source.generate_region(self.largest_region())
+        _, nc = _pixel_info(inp.data, requested.ndim)
+        out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
+        self.generate_into(inp, out)
+        return out
+
+
+class RecursiveGaussianAxisFilter(RecursiveGaussianFilter):
+    """RecursiveGaussianAxisFilter<S> (src/filters/recursive_gaussian.rs:181-241):
+    the same along one axis (`axis`, 0 = x)."""
+
+    def __init__(self, source, sigma: float, axis: int):
+        super().__init__(source, sigma)
+        self.axis = int(axis)
+
+    @property
+    def _axis(self):
+        return self.axis
 
 
 # ---------------------------------------------------------------------------

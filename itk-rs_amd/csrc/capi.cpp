@@ -1022,6 +1022,141 @@ int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, doubl
     return rc;
 }
 
+}  // extern "C"
+
+namespace itkg {
+namespace {
+
+// yvv_coeffs (src/filters/recursive_gaussian.rs:33-47), the same expressions
+// in the same order (f64::max ignores a NaN, as fmax does).
+void yvv_coefficients(double sigma, double c[4]) {
+    sigma = std::fmax(sigma, 0.5);
+    const double q = sigma >= 2.5 ? 0.98711 * sigma - 0.96330
+                                  : 3.97156 - 4.14554 * std::sqrt(1.0 - 0.26891 * sigma);
+    const double q2 = q * q;
+    const double q3 = q2 * q;
+    const double b0 = 1.57825 + 2.44413 * q + 1.4281 * q2 + 0.422205 * q3;
+    const double b1 = 2.44413 * q + 2.85619 * q2 + 1.26661 * q3;
+    const double b2 = -(1.4281 * q2 + 1.26661 * q3);
+    const double b3 = 0.422205 * q3;
+    c[0] = 1.0 - (b1 + b2 + b3) / b0;
+    c[1] = b1 / b0;
+    c[2] = b2 / b0;
+    c[3] = b3 / b0;
+}
+
+// RecursiveGaussianFilter / RecursiveGaussianAxisFilter::generate_region
+// (recursive_gaussian.rs:147-176, 219-240) on device views: the input covers
+// the whole largest region (the filters' input_region_for_output).
+int recursive_impl(int ndim, itkg_dtype dtype, double sigma, const double* spacing, int axis,
+                   const itkg_region& largest, const itkg_view& in, const itkg_view& out,
+                   hipStream_t s) {
+    if ((axis < -1 || axis >= ndim)) return fail(ITKG_ERR_INVALID_ARG, "axis %d of a %d-D image", axis, ndim);
+    if (!spacing) return fail(ITKG_ERR_INVALID_ARG, "null spacing");
+    if (!region_ok(ndim, largest) || !region_ok(ndim, out.region))
+        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
+    if (region_len(ndim, out.region) > 0 && !contains(ndim, largest, out.region))
+        return fail(ITKG_ERR_INVALID_ARG, "requested region is not inside the largest region");
+    int rc = check_view(ndim, &in, largest, "input");
+    if (rc) return rc;
+    if ((rc = check_view(ndim, &out, out.region, "output"))) return rc;
+    const int64_t total = region_len(ndim, largest);
+    if (region_len(ndim, out.region) == 0 || total == 0) return ITKG_OK;
+    int axes[ITKG_MAX_DIM];
+    double coef[ITKG_MAX_DIM][4];
+    int naxes = 0;
+    for (int d = 0; d < ndim; ++d) {
+        if (axis >= 0 && d != axis) continue;
+        axes[naxes] = d;
+        yvv_coefficients(sigma / spacing[d], coef[naxes]);
+        ++naxes;
+    }
+    void* work = nullptr;
+    hipError_t e = hipMallocAsync(&work, (size_t)total * sizeof(double), s);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(recursive work)");
+    auto run = [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        RecursiveArgs<T> a{};
+        a.in = static_cast<const T*>(in.data) + view_offset(ndim, in, largest.index);
+        a.work = static_cast<double*>(work);
+        a.out = static_cast<T*>(out.data);
+        for (int d = 0; d < ITKG_MAX_DIM; ++d) {
+            a.n[d] = d < ndim ? largest.size[d] : 1;
+            a.m[d] = d < ndim ? out.region.size[d] : 1;
+            a.off[d] = d < ndim ? out.region.index[d] - largest.index[d] : 0;
+            a.in_stride[d] = d < ndim ? in.stride[d] : 0;
+            a.out_stride[d] = d < ndim ? out.stride[d] : 0;
+        }
+        return launch_recursive<T>(a, axes, naxes, coef, s);
+    };
+    e = dtype == ITKG_F64 ? run((double*)nullptr) : run((float*)nullptr);
+    if (e != hipSuccess) rc = hip_fail(e, "recursive launch");
+    (void)hipFreeAsync(work, s);
+    return rc;
+}
+
+}  // namespace
+}  // namespace itkg
+
+extern "C" {
+
+int itkg_yvv_coefficients(double sigma_px, double* coef) {
+    clear_error();
+    if (!coef) return fail(ITKG_ERR_INVALID_ARG, "null output");
+    yvv_coefficients(sigma_px, coef);
+    return ITKG_OK;
+}
+
+int itkg_recursive_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, double sigma,
+                                            const double* spacing, int32_t axis,
+                                            const itkg_region* largest, const itkg_view* input,
+                                            const itkg_view* output, void* stream) {
+    clear_error();
+    if (!args_ok(ndim, dtype, 1)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype");
+    if (!largest || !input || !output) return fail(ITKG_ERR_INVALID_ARG, "null argument");
+    return recursive_impl(ndim, dtype, sigma, spacing, axis, *largest, *input, *output,
+                          static_cast<hipStream_t>(stream));
+}
+
+int itkg_recursive_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma,
+                                                 const double* spacing, int32_t axis,
+                                                 const itkg_region* largest, const void* input,
+                                                 const itkg_region* requested, void* output) {
+    clear_error();
+    if (!args_ok(ndim, dtype, 1)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype");
+    if (!largest || !requested) return fail(ITKG_ERR_INVALID_ARG, "null region");
+    if (!region_ok(ndim, *largest) || !region_ok(ndim, *requested))
+        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
+    const int64_t nin = region_len(ndim, *largest), nout = region_len(ndim, *requested);
+    if (nout == 0) return ITKG_OK;
+    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
+    const size_t px = dtype_size(dtype);
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    int rc = ITKG_OK;
+    void *din = nullptr, *dout = nullptr;
+    e = hipMallocAsync(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "recursive host staging");
+    if (!rc) {
+        const itkg_view vin = dense_view(ndim, din, *largest);
+        const itkg_view vout = dense_view(ndim, dout, *requested);
+        rc = recursive_impl(ndim, dtype, sigma, spacing, axis, *largest, vin, vout, s);
+    }
+    if (!rc) {
+        e = hipMemcpyAsync(output, dout, (size_t)nout * px, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy out");
+    }
+    if (din) (void)hipFreeAsync(din, s);
+    if (dout) (void)hipFreeAsync(dout, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
 int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
                            const double* spacing, const itkg_region* largest,
                            const itkg_view* grad_output, const itkg_view* grad_input,

@@ -63,6 +63,23 @@ struct UnsharpArgs {
 template <typename T>
 hipError_t launch_unsharp(const UnsharpArgs<T>& a, hipStream_t s);
 
+// Recursive Gaussian: `in` (strides in pixels) covers the whole region of
+// sizes n; work is its dense f64 copy; out (strides) receives the sub-region
+// of sizes m at offset off (pixels from the whole region's first corner).
+template <typename T>
+struct RecursiveArgs {
+    const T* in;
+    double* work;
+    T* out;
+    int64_t n[4], m[4], off[4];
+    int64_t in_stride[4], out_stride[4];
+};
+// Widen, run each listed axis's causal + anticausal lines with its
+// coefficients (B, a1, a2, a3), narrow the sub-region.
+template <typename T>
+hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxes,
+                            const double (*coef)[4], hipStream_t s);
+
 // Fused one-round-trip 3-D kernel for f32 scalar pixels (forward or adjoint).
 // Destination pixel j reads source pixels j + shift + k - r, k = 0..2r
 // (forward: clamped into the source; adjoint: zero outside it).

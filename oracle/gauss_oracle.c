@@ -49,6 +49,8 @@
  *     ::generate_region (its own per-pass padding rule, loop for loop)
  *   UnsharpMaskFilter               src/filters/edges.rs:406-452
  *     ::generate_region (whole-buffer passes, the f64 per-pixel step)
+ *   RecursiveGaussianFilter /       src/filters/recursive_gaussian.rs:33-240
+ *     RecursiveGaussianAxisFilter (yvv_coeffs, causal / anticausal lines)
  *
  *   mean / binomial (f32)          : interior-impulse closed forms (box
  *                                  products; binomial coefficients / 4^n,
@@ -406,6 +408,97 @@ int NAME(int D, const T* img, const int64_t* img_index, const int64_t* img_size,
 }
 DEFINE_UNSHARP(orc_unsharp_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
 DEFINE_UNSHARP(orc_unsharp_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
+
+/* ---- Recursive Gaussian: src/filters/recursive_gaussian.rs, loop for loop.
+ * yvv_coeffs (33-47); causal_pass / anticausal_pass (55-86) over every line
+ * of filter_axis (91-127), lines in iter_region order of the stripe region;
+ * RecursiveGaussianFilter (axis = -1, axes 0..D-1) and
+ * RecursiveGaussianAxisFilter (one axis): whole image in f64, then the
+ * requested region cast to T (147-176, 219-240).  Scalar pixels. */
+void orc_yvv_coeffs(double sigma, double* c) {
+    sigma = fmax(sigma, 0.5);
+    double q = sigma >= 2.5 ? 0.98711 * sigma - 0.96330
+                            : 3.97156 - 4.14554 * sqrt(1.0 - 0.26891 * sigma);
+    double q2 = q * q;
+    double q3 = q2 * q;
+    double b0 = 1.57825 + 2.44413 * q + 1.4281 * q2 + 0.422205 * q3;
+    double b1 = 2.44413 * q + 2.85619 * q2 + 1.26661 * q3;
+    double b2 = -(1.4281 * q2 + 1.26661 * q3);
+    double b3 = 0.422205 * q3;
+    c[0] = 1.0 - (b1 + b2 + b3) / b0;
+    c[1] = b1 / b0;
+    c[2] = b2 / b0;
+    c[3] = b3 / b0;
+}
+
+static void orc_filter_axis(int D, double* data, orc_region r, int axis, double sigma_px) {
+    int64_t size = r.size[axis];
+    if (size <= 1) return;
+    double c[4];
+    orc_yvv_coeffs(sigma_px, c);
+    double B = c[0], a1 = c[1], a2 = c[2], a3 = c[3];
+    orc_region stripe = r;
+    stripe.size[axis] = 1;
+    int64_t nl = region_len(D, stripe);
+    int64_t idx[ORC_MAX_DIM];
+    for (int d = 0; d < D; ++d) idx[d] = stripe.index[d];
+    int64_t* line = (int64_t*)malloc((size_t)size * sizeof(int64_t));
+    for (int64_t l = 0; l < nl; ++l) {
+        for (int64_t k = 0; k < size; ++k) {
+            int64_t p[ORC_MAX_DIM];
+            for (int d = 0; d < D; ++d) p[d] = idx[d];
+            p[axis] += k;
+            line[k] = flat_index(D, r, p);
+        }
+        double init = data[line[0]];
+        double ym1 = init, ym2 = init, ym3 = init;
+        for (int64_t i = 0; i < size; ++i) {
+            double v = B * data[line[i]] + a1 * ym1 + a2 * ym2 + a3 * ym3;
+            ym3 = ym2; ym2 = ym1; ym1 = v;
+            data[line[i]] = v;
+        }
+        init = data[line[size - 1]];
+        ym1 = init; ym2 = init; ym3 = init;
+        for (int64_t i = size - 1; i >= 0; --i) {
+            double v = B * data[line[i]] + a1 * ym1 + a2 * ym2 + a3 * ym3;
+            ym3 = ym2; ym2 = ym1; ym1 = v;
+            data[line[i]] = v;
+        }
+        for (int d = 0; d < D; ++d) {
+            if (++idx[d] < stripe.index[d] + stripe.size[d]) break;
+            idx[d] = stripe.index[d];
+        }
+    }
+    free(line);
+}
+
+#define DEFINE_RECURSIVE(NAME, T)                                                \
+int NAME(int D, const T* img, const int64_t* img_index, const int64_t* img_size, \
+         const double* spacing, double sigma, int axis,                          \
+         const int64_t* req_index, const int64_t* req_size, T* out) {            \
+    if (D < 1 || D > ORC_MAX_DIM || axis < -1 || axis >= D) return -1;          \
+    orc_region bounds = region_make(D, img_index, img_size);                     \
+    orc_region req = region_make(D, req_index, req_size);                        \
+    int64_t n = region_len(D, bounds);                                           \
+    double* data = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));   \
+    for (int64_t i = 0; i < n; ++i) data[i] = (double)img[i];                    \
+    for (int a = 0; a < D; ++a)                                                  \
+        if (axis < 0 || a == axis) orc_filter_axis(D, data, bounds, a, sigma / spacing[a]); \
+    int64_t m = region_len(D, req);                                              \
+    int64_t idx[ORC_MAX_DIM];                                                    \
+    for (int d = 0; d < D; ++d) idx[d] = req.index[d];                           \
+    for (int64_t o = 0; o < m; ++o) {                                            \
+        out[o] = (T)data[flat_index(D, bounds, idx)];                            \
+        for (int d = 0; d < D; ++d) {                                            \
+            if (++idx[d] < req.index[d] + req.size[d]) break;                    \
+            idx[d] = req.index[d];                                               \
+        }                                                                        \
+    }                                                                            \
+    free(data);                                                                  \
+    return 0;                                                                    \
+}
+DEFINE_RECURSIVE(orc_recursive_generate_region_f32, float)
+DEFINE_RECURSIVE(orc_recursive_generate_region_f64, double)
 
 /* ---- adjoint in f64, by scattering the forward taps -------------------- */
 /* No reference counterpart.  The forward operator is A = A_{D-1} ... A_0

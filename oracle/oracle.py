@@ -84,6 +84,13 @@ def lib():
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_int, ctypes.c_void_p, i64p, i64p, ctypes.c_double,
                            ctypes.c_double, ctypes.c_double, i64p, i64p, ctypes.c_void_p]
+        L.orc_yvv_coeffs.restype = None
+        L.orc_yvv_coeffs.argtypes = [ctypes.c_double, f64p]
+        for name in ("orc_recursive_generate_region_f32", "orc_recursive_generate_region_f64"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_int, ctypes.c_void_p, i64p, i64p, f64p, ctypes.c_double,
+                           ctypes.c_int, i64p, i64p, ctypes.c_void_p]
         L.orc_adjoint_f64.restype = ctypes.c_int
         L.orc_adjoint_f64.argtypes = [ctypes.c_int, ctypes.c_int, i64p, i64p, f64p, ctypes.c_double,
                                       i64p, i64p, ctypes.c_void_p, ctypes.c_void_p, i64p, i64p]
@@ -224,6 +231,57 @@ def unsharp_generate_region(img: np.ndarray, img_index, sigma, amount, threshold
     if rc != 0:
         raise ValueError("oracle rejected arguments")
     return out
+
+
+def yvv_coeffs(sigma: float) -> np.ndarray:
+    """src/filters/recursive_gaussian.rs:33-47 (via the C oracle): B, a1, a2, a3."""
+    out = np.zeros(4, dtype=np.float64)
+    lib().orc_yvv_coeffs(float(sigma), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def recursive_generate_region(img: np.ndarray, img_index, spacing, sigma, axis, req_index,
+                              req_size) -> np.ndarray:
+    """RecursiveGaussianFilter (axis -1) / RecursiveGaussianAxisFilter (axis)
+    .generate_region(req) via the C oracle (recursive_gaussian.rs:147-240)."""
+    img = np.ascontiguousarray(img)
+    D = len(img_index)
+    img_size = list(reversed(img.shape[:D]))
+    fn = lib().orc_recursive_generate_region_f32 if img.dtype == np.float32 else \
+        lib().orc_recursive_generate_region_f64
+    out = np.zeros(_shape(req_size, 1), dtype=img.dtype)
+    rc = fn(D, img.ctypes.data, _i64(img_index), _i64(img_size), _f64(spacing), float(sigma),
+            int(axis), _i64(req_index), _i64(req_size), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    return out
+
+
+def np_recursive(img: np.ndarray, spacing, sigma, axis) -> np.ndarray:
+    """Independent numpy restatement of the recursive Gaussian on the whole
+    image (f64 result): every line along an axis at once, positions in turn,
+    the same products and sums in the same order."""
+    D = len(spacing)
+    y = np.asarray(img, dtype=np.float64).copy()
+    for a in range(D):
+        if axis >= 0 and a != axis:
+            continue
+        ax = D - 1 - a                      # numpy axis of image axis a
+        n = y.shape[ax]
+        if n <= 1:
+            continue
+        B, a1, a2, a3 = (float(v) for v in yvv_coeffs(sigma / spacing[a]))
+        v = np.moveaxis(y, ax, 0)           # a view: writes go to y
+        for order in (range(n), range(n - 1, -1, -1)):
+            first = next(iter(order))
+            ym1 = v[first].copy()
+            ym2 = ym1.copy()
+            ym3 = ym1.copy()
+            for i in order:
+                cur = B * v[i] + a1 * ym1 + a2 * ym2 + a3 * ym3
+                ym3, ym2, ym1 = ym2, ym1, cur
+                v[i] = cur
+    return y
 
 
 def ref_faithful(img: np.ndarray, img_index, spacing, sigma, req_index, req_size, ncomp=1,

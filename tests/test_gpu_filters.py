@@ -220,3 +220,53 @@ def test_unsharp_streamed_and_vector_refused(itk, dev):
     with pytest.raises(itk.ItkgError) as e:
         itk.UnsharpMaskFilter(v, 1.0).generate_region(v.region)
     assert e.value.code == itk.ERR_UNSUPPORTED
+
+
+# --- Recursive Gaussian (src/filters/recursive_gaussian.rs) -----------------
+
+@pytest.mark.parametrize("size,spacing,sigma,axis,req,dtype", [
+    ((64, 48, 40), (1.0, 1.0, 1.0), 2.0, -1, None, np.float32),
+    ((70, 50, 30), (0.7, 0.7, 2.5), 3.0, -1, ((5, 0, 3), (40, 50, 20)), np.float32),
+    ((96, 80), (1.0, 1.0), 1.5, 1, None, np.float32),
+    ((200,), (1.0,), 0.3, -1, ((20,), (100,)), np.float32),
+    ((30, 20, 10), (1.0, 1.0, 1.0), 2.5, 0, ((4, 4, 2), (20, 10, 6)), np.float64),
+    ((1, 40, 12), (1.0, 1.0, 1.0), 1.0, -1, None, np.float32),   # an axis of one sample
+])
+def test_recursive_matches_oracle(itk, orc, dev, size, spacing, sigma, axis, req, dtype):
+    D = len(size)
+    x = rand(tuple(reversed(size)), 1, dtype)
+    src = itk.Image(itk.Region([0] * D, list(size)), spacing, (0.0,) * D, torch.from_numpy(x).to(dev))
+    f = itk.RecursiveGaussianFilter(src, sigma) if axis < 0 else \
+        itk.RecursiveGaussianAxisFilter(src, sigma, axis)
+    reg = src.region if req is None else itk.Region(list(req[0]), list(req[1]))
+    out = f.generate_region(reg)
+    torch.cuda.synchronize()
+    ref = orc.recursive_generate_region(x, [0] * D, list(spacing), sigma, axis, list(reg.index),
+                                        list(reg.size))
+    assert np.array_equal(out.data.cpu().numpy(), ref)
+
+
+def test_recursive_reference_tests_and_host_entry(itk, dev):
+    # recursive_gaussian.rs:252-300 through the GPU filter
+    imp = torch.zeros(51)
+    imp[25] = 1.0
+    im = itk.Image(itk.Region([0], [51]), (1.0,), (0.0,), imp.to(dev))
+    out = itk.RecursiveGaussianFilter(im, 2.0).generate_region(im.region).data.cpu()
+    assert abs(float(out.sum()) - 1.0) < 0.02 and float(out[25]) > float(out[28])
+    c = itk.Image(itk.Region([0, 0], [15, 15]), (1.0, 1.0), (0.0, 0.0), torch.full((15, 15), 4.0, device=dev))
+    out = itk.RecursiveGaussianFilter(c, 1.5).generate_region(c.region).data.cpu()
+    assert bool(((out[3:12, 3:12] - 4.0).abs() < 0.05).all())
+    # the host-buffer entry point equals the device one
+    x = torch.rand((30, 40, 50), device=dev)
+    src = itk.Image(itk.Region([0, 0, 0], [50, 40, 30]), (1.0,) * 3, (0.0,) * 3, x)
+    want = itk.RecursiveGaussianFilter(src, 2.2).generate_region(itk.Region([5, 0, 3], [30, 40, 20]))
+    xh = x.cpu().numpy()
+    got = np.empty((20, 40, 30), np.float32)
+    req = itk.Region([5, 0, 3], [30, 40, 20]).to_c()
+    rc = itk.lib.itkg_recursive_gaussian_generate_region_host(
+        3, itk.F32, 2.2, (ctypes.c_double * 3)(1.0, 1.0, 1.0), -1, ctypes.byref(src.region.to_c()),
+        xh.ctypes.data, ctypes.byref(req), got.ctypes.data)
+    assert rc == 0, itk.lib.itkg_last_error()
+    assert np.array_equal(got, want.data.cpu().numpy())
+    with pytest.raises(itk.ItkgError):
+        itk.RecursiveGaussianAxisFilter(src, 1.0, 3).generate_region(src.region)

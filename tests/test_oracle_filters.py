@@ -1,8 +1,9 @@
 """Pins the oracle's restatements of the other convolve_axis filters --
 MeanFilter / BoxMeanFilterN (src/filters/mean.rs:15-18, 59-83, 126-144),
 BinomialBlurFilter (src/filters/binomial_blur.rs:57-89) and
-UnsharpMaskFilter (src/filters/edges.rs:406-452) -- before the GPU path is
-checked against them.  UnsharpMaskFilter has no tests in the reference: its
+UnsharpMaskFilter (src/filters/edges.rs:406-452) and the recursive Gaussian
+(src/filters/recursive_gaussian.rs:33-240) -- before the GPU path is checked
+against them.  UnsharpMaskFilter has no tests in the reference: its
 pins are the kernel's closed form, identity cases, the dense operator and
 crop consistency.
 
@@ -219,3 +220,71 @@ def test_unsharp_subregion_is_crop_of_whole(orc):
         sub = orc.unsharp_generate_region(x, [0, 0, 0], 1.3, 0.6, 0.01, list(req[0]), list(req[1]))
         crop = full[tuple(slice(i, i + s) for i, s in zip(reversed(req[0]), reversed(req[1])))]
         assert np.array_equal(sub, crop)
+
+
+# --- Recursive Gaussian (recursive_gaussian.rs:33-240) ------------------------
+
+def _impulse(n):
+    x = np.zeros(n, np.float32)
+    x[n // 2] = 1.0
+    return x
+
+
+def test_recursive_reference_tests(orc):
+    # recursive_gaussian.rs:252-300
+    out = orc.recursive_generate_region(_impulse(51), [0], [1.0], 2.0, -1, [0], [51])
+    assert abs(float(out.sum()) - 1.0) < 0.02
+    assert out[25] > out[28]
+    c = np.full((15, 15), 4.0, np.float32)
+    out = orc.recursive_generate_region(c, [0, 0], [1.0, 1.0], 1.5, -1, [0, 0], [15, 15])
+    assert np.all(np.abs(out[3:12, 3:12] - 4.0) < 0.05)
+    out = orc.recursive_generate_region(_impulse(31), [0], [1.0], 2.0, 0, [0], [31])
+    assert abs(float(out.sum()) - 1.0) < 0.02
+
+
+@pytest.mark.parametrize("sigma", [0.1, 0.5, 1.0, 2.4, 2.5, 3.7, 10.0])
+def test_yvv_coefficients(orc, sigma):
+    c = orc.yvv_coeffs(sigma)
+    assert abs(float(c.sum()) - 1.0) < 1e-14         # unit DC gain: B + a1 + a2 + a3 = 1
+    if sigma < 0.5:
+        assert np.array_equal(c, orc.yvv_coeffs(0.5))  # clamped below at 0.5
+    assert np.array_equal(orc.yvv_coeffs(float("nan")), orc.yvv_coeffs(0.5))  # f64::max drops NaN
+
+
+@pytest.mark.parametrize("sigma,lo,hi", [(1.0, 1.08, 1.30), (1.5, 1.08, 1.30), (2.0, 1.08, 1.30),
+                                         (3.0, 1.05, 1.20), (5.0, 1.05, 1.20)])
+def test_recursive_impulse_is_gaussian_like(orc, sigma, lo, hi):
+    # the third-order recursion approximates a Gaussian: unit sum, symmetric,
+    # peak within 10% of a sampled Gaussian's, and an effective width a little
+    # above sigma (1.24 sigma at 1, 1.11 sigma at 5 for this form); a wrong
+    # branch of the q formula lands far outside these bands
+    n = 161
+    x = np.zeros(n)
+    x[80] = 1.0
+    h = orc.recursive_generate_region(x, [0], [1.0], sigma, -1, [0], [n])
+    g = np.exp(-0.5 * ((np.arange(n) - 80) / sigma) ** 2)
+    g /= g.sum()
+    assert abs(h.sum() - 1.0) < 1e-9
+    assert np.allclose(h, h[::-1], atol=1e-6 * h.max())   # zero-phase up to rounding and the ends
+    assert np.abs(h - g).max() < 0.1 * g.max()
+    i = np.arange(n)
+    sd = math.sqrt(float((h * (i - 80) ** 2).sum()))
+    assert lo * sigma < sd < hi * sigma
+
+
+@pytest.mark.parametrize("shape,spacing,sigma,axis", [
+    ((13, 9, 7), (1.0, 1.0, 1.0), 1.5, -1),
+    ((12, 10), (0.5, 2.0), 2.0, -1),
+    ((8, 11, 6), (1.0, 1.0, 1.0), 3.0, 1),
+    ((40,), (1.0,), 0.3, -1),
+    ((1, 10, 5), (1.0, 1.0, 1.0), 1.0, -1),      # an axis of one sample is left alone
+])
+def test_recursive_c_vs_numpy(orc, shape, spacing, sigma, axis):
+    D = len(shape)
+    x = rand(tuple(reversed(shape)), dtype=np.float64)
+    a = orc.recursive_generate_region(x, [0] * D, list(spacing), sigma, axis, [0] * D, list(shape))
+    b = orc.np_recursive(x, list(spacing), sigma, axis)
+    assert np.array_equal(a, b)
+    x32 = x.astype(np.float32)
+    a32 = orc.recursive_generate_region(x32, [0] * D, list(spacing), sigma, axis, [0] * D, list(shape))
+    assert np.array_equal(a32, orc.np_recursive(x32, list(spacing), sigma, axis).astype(np.float32))

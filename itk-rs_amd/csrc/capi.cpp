@@ -1062,6 +1062,12 @@ int recursive_impl(int ndim, itkg_dtype dtype, double sigma, const double* spaci
     if ((rc = check_view(ndim, &out, out.region, "output"))) return rc;
     const int64_t total = region_len(ndim, largest);
     if (region_len(ndim, out.region) == 0 || total == 0) return ITKG_OK;
+    // the widen / narrow grids take axis 1 and the plane count of axes 2-3
+    // as grid dimensions
+    const int64_t n1 = ndim > 1 ? largest.size[1] : 1;
+    const int64_t planes = (ndim > 2 ? largest.size[2] : 1) * (ndim > 3 ? largest.size[3] : 1);
+    if (n1 > 65535 || planes > 65535)
+        return fail(ITKG_ERR_UNSUPPORTED, "recursive Gaussian: axis 1 or axes 2-3 exceed 65535 rows");
     int axes[ITKG_MAX_DIM];
     double coef[ITKG_MAX_DIM][4];
     int naxes = 0;

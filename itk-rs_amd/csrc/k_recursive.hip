@@ -14,21 +14,32 @@ namespace itkg {
 
 namespace {
 
+// Grids here are (x blocks, rows of axis 1, planes of axes 2 and 3):
+// the block's row and plane come from its indices (no per-pixel division).
+__device__ __forceinline__ void plane_of(const int64_t n2, const int64_t n3, int64_t& i2,
+                                         int64_t& i3) {
+    i2 = blockIdx.z;
+    i3 = 0;
+    if (n3 > 1) {
+        i2 = (uint32_t)blockIdx.z % (uint32_t)n2;
+        i3 = (uint32_t)blockIdx.z / (uint32_t)n2;
+    }
+}
+
 // f64 copy of the whole (largest) region; `n` sizes, axis 0 fastest.
+constexpr int kRowBlock = 1024;  // pixels of a row per block (four per thread, 256 apart)
+
 template <typename T>
 __global__ __launch_bounds__(256) void widen_kernel(const RecursiveArgs<T> a) {
-    const int64_t total = a.n[0] * a.n[1] * a.n[2] * a.n[3];
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * 256) {
-        int64_t r = i;
-        const int64_t x = r % a.n[0];
-        r /= a.n[0];
-        const int64_t y = r % a.n[1];
-        r /= a.n[1];
-        const int64_t z = r % a.n[2];
-        const int64_t w = r / a.n[2];
-        a.work[i] = (double)a.in[x * a.in_stride[0] + y * a.in_stride[1] + z * a.in_stride[2] +
-                                 w * a.in_stride[3]];
+    const int64_t y = blockIdx.y;
+    int64_t z, w;
+    plane_of(a.n[2], a.n[3], z, w);
+    double* wrow = a.work + a.n[0] * (y + a.n[1] * (z + a.n[2] * w));
+    const T* irow = a.in + y * a.in_stride[1] + z * a.in_stride[2] + w * a.in_stride[3];
+#pragma unroll
+    for (int j = 0; j < kRowBlock / 256; ++j) {
+        const int64_t x = (int64_t)blockIdx.x * kRowBlock + j * 256 + threadIdx.x;
+        if (x < a.n[0]) wrow[x] = (double)irow[x * a.in_stride[0]];
     }
 }
 
@@ -55,46 +66,61 @@ __global__ __launch_bounds__(256) void line_kernel(double* work, const int64_t n
     double* p = work + base;
     double init = p[0];
     double ym1 = init, ym2 = init, ym3 = init;
-    for (int64_t i = 0; i < len; ++i) {
-        const double v = B * p[i * stride] + a1 * ym1 + a2 * ym2 + a3 * ym3;
+    auto step = [&](double x) {
+        const double v = B * x + a1 * ym1 + a2 * ym2 + a3 * ym3;
         ym3 = ym2;
         ym2 = ym1;
         ym1 = v;
-        p[i * stride] = v;
+        return v;
+    };
+    // Rows (stride 1) that start 16-byte aligned go in chunks of eight
+    // samples with 16-byte loads and stores, each lane through its own
+    // contiguous run; the samples still pass through `step` one by one.
+    const bool rows = stride == 1 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+    const int64_t body = rows ? len / 8 * 8 : 0;
+    for (int64_t i = 0; i < body; i += 8) {
+        double2* q = reinterpret_cast<double2*>(p + i);
+        double2 c[4] = {q[0], q[1], q[2], q[3]};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c[j].x = step(c[j].x);
+            c[j].y = step(c[j].y);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = c[j];
     }
+    for (int64_t i = body; i < len; ++i) p[i * stride] = step(p[i * stride]);
     init = ym1;  // the line's last value after the causal pass
     ym1 = ym2 = ym3 = init;
-    for (int64_t i = len - 1; i >= 0; --i) {
-        const double v = B * p[i * stride] + a1 * ym1 + a2 * ym2 + a3 * ym3;
-        ym3 = ym2;
-        ym2 = ym1;
-        ym1 = v;
-        p[i * stride] = v;
+    // backwards: the samples past the last whole chunk first, then the chunks
+    for (int64_t i = len - 1; i >= body; --i) p[i * stride] = step(p[i * stride]);
+    for (int64_t i = body - 8; i >= 0; i -= 8) {
+        double2* q = reinterpret_cast<double2*>(p + i);
+        double2 c[4] = {q[0], q[1], q[2], q[3]};
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {
+            c[j].y = step(c[j].y);
+            c[j].x = step(c[j].x);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = c[j];
     }
 }
 
 // requested region of the f64 image, narrowed to T
 template <typename T>
 __global__ __launch_bounds__(256) void narrow_kernel(const RecursiveArgs<T> a) {
-    const int64_t total = a.m[0] * a.m[1] * a.m[2] * a.m[3];
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * 256) {
-        int64_t r = i;
-        const int64_t x = r % a.m[0];
-        r /= a.m[0];
-        const int64_t y = r % a.m[1];
-        r /= a.m[1];
-        const int64_t z = r % a.m[2];
-        const int64_t w = r / a.m[2];
-        const int64_t src = (x + a.off[0]) + a.n[0] * ((y + a.off[1]) + a.n[1] * ((z + a.off[2]) +
-                                                                                a.n[2] * (w + a.off[3])));
-        a.out[x * a.out_stride[0] + y * a.out_stride[1] + z * a.out_stride[2] + w * a.out_stride[3]] =
-            (T)a.work[src];
+    const int64_t y = blockIdx.y;
+    int64_t z, w;
+    plane_of(a.m[2], a.m[3], z, w);
+    const double* wrow =
+        a.work + a.off[0] + a.n[0] * ((y + a.off[1]) + a.n[1] * ((z + a.off[2]) + a.n[2] * (w + a.off[3])));
+    T* orow = a.out + y * a.out_stride[1] + z * a.out_stride[2] + w * a.out_stride[3];
+#pragma unroll
+    for (int j = 0; j < kRowBlock / 256; ++j) {
+        const int64_t x = (int64_t)blockIdx.x * kRowBlock + j * 256 + threadIdx.x;
+        if (x < a.m[0]) orow[x * a.out_stride[0]] = (T)wrow[x];
     }
-}
-
-unsigned grid_for(int64_t items) {
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((items + 255) / 256, 1 << 20));
 }
 
 }  // namespace
@@ -104,7 +130,12 @@ hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxe
                             const double (*coef)[4], hipStream_t s) {
     const int64_t total = a.n[0] * a.n[1] * a.n[2] * a.n[3];
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(widen_kernel<T>, dim3(grid_for(total)), dim3(256), 0, s, a);
+    if (a.n[1] > 65535 || a.n[2] * a.n[3] > 65535 || a.m[1] > 65535 || a.m[2] * a.m[3] > 65535)
+        return hipErrorInvalidValue;  // grid rows / planes beyond the launch limit
+    hipLaunchKernelGGL(widen_kernel<T>,
+                       dim3((unsigned)((a.n[0] + kRowBlock - 1) / kRowBlock), (unsigned)a.n[1],
+                            (unsigned)(a.n[2] * a.n[3])),
+                       dim3(256), 0, s, a);
     for (int k = 0; k < naxes; ++k) {
         const int ax = axes[k];
         if (a.n[ax] <= 1) continue;  // a line of one sample is left as it is
@@ -114,7 +145,11 @@ hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxe
                            coef[k][2], coef[k][3]);
     }
     const int64_t m = a.m[0] * a.m[1] * a.m[2] * a.m[3];
-    if (m > 0) hipLaunchKernelGGL(narrow_kernel<T>, dim3(grid_for(m)), dim3(256), 0, s, a);
+    if (m > 0)
+        hipLaunchKernelGGL(narrow_kernel<T>,
+                           dim3((unsigned)((a.m[0] + kRowBlock - 1) / kRowBlock), (unsigned)a.m[1],
+                                (unsigned)(a.m[2] * a.m[3])),
+                           dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 //   MeanFilter, BoxMeanFilterN       src/filters/mean.rs:15-145
 //   BinomialBlurFilter               src/filters/binomial_blur.rs:17-90
 //   UnsharpMaskFilter                src/filters/edges.rs:376-452
+//   RecursiveGaussian(Axis)Filter    src/filters/recursive_gaussian.rs:129-241
 //   stream_full / split_region       src/source.rs:81-97, src/region_splitter.rs:8-39
 //   VecPixel                         src/pixel.rs:61-99
 //
@@ -457,6 +458,41 @@ struct UnsharpMaskFilter {
         check(itkg_unsharp_mask_generate_region_host((int32_t)D, PixelTraits<P>::dtype, sigma, amount,
                                                      threshold, &cb, input.data.data(), &cr,
                                                      out.data.data(), separable_arith));
+        return out;
+    }
+};
+
+// RecursiveGaussianFilter<S> (src/filters/recursive_gaussian.rs:129-177;
+// axis = -1) and RecursiveGaussianAxisFilter<S> (recursive_gaussian.rs:
+// 181-241; the `axis` field): the Young - van Vliet recursive Gaussian in
+// f64 over the whole image; scalar f32 / f64 pixels.
+template <class S>
+struct RecursiveGaussianFilter {
+    using pixel_type = typename S::pixel_type;
+    static constexpr size_t dim = S::dim;
+    using P = pixel_type;
+    static constexpr size_t D = dim;
+    static_assert(PixelTraits<P>::ncomp == 1, "the recursive Gaussian takes scalar pixels");
+
+    S source;
+    double sigma;
+    int axis = -1;
+
+    RecursiveGaussianFilter(S src, double s, int ax = -1) : source(std::move(src)), sigma(s), axis(ax) {}
+
+    Region<D> largest_region() const { return source.largest_region(); }
+    std::array<double, D> spacing() const { return source.spacing(); }
+    std::array<double, D> origin() const { return source.origin(); }
+    Region<D> input_region_for_output(const Region<D>&) const { return source.largest_region(); }
+    Image<P, D> generate_region(Region<D> requested) const {
+        const Region<D> bounds = source.largest_region();
+        const Image<P, D> input = source.generate_region(bounds);
+        Image<P, D> out = Image<P, D>::allocate(requested, spacing(), origin(), P{});
+        const itkg_region cb = bounds.to_c(), cr = requested.to_c();
+        const auto sp = spacing();
+        check(itkg_recursive_gaussian_generate_region_host((int32_t)D, PixelTraits<P>::dtype, sigma,
+                                                           sp.data(), axis, &cb, input.data.data(),
+                                                           &cr, out.data.data()));
         return out;
     }
 };

@@ -234,6 +234,28 @@ static void test_unsharp_mask() {
     CHECK(out.get_pixel(Index<2>{{8, 7}}) < 0.0f, "neighbour %g", out.get_pixel(Index<2>{{8, 7}}));
 }
 
+// src/filters/recursive_gaussian.rs:252-300
+static void test_recursive_gaussian() {
+    auto imp = solid_image<1>({51}, 0.0f);
+    imp.set_pixel(Index<1>{{25}}, 1.0f);
+    RecursiveGaussianFilter<Image<float, 1>> f(imp, 2.0);
+    auto out = f.generate_region(f.largest_region());
+    CHECK(std::fabs(sum(out.data) - 1.0f) < 0.02f, "energy=%g", sum(out.data));
+    CHECK(out.get_pixel(Index<1>{{25}}) > out.get_pixel(Index<1>{{28}}), "peak not at the centre");
+    RecursiveGaussianFilter<Image<float, 2>> c(solid_image<2>({15, 15}, 4.0f), 1.5);
+    auto oc = c.generate_region(c.largest_region());
+    for (int64_t i = 3; i < 12; ++i)
+        for (int64_t j = 3; j < 12; ++j) {
+            const float v = oc.get_pixel(Index<2>{{i, j}});
+            CHECK(std::fabs(v - 4.0f) < 0.05f, "at [%lld,%lld]: %g", (long long)i, (long long)j, v);
+        }
+    auto imp31 = solid_image<1>({31}, 0.0f);
+    imp31.set_pixel(Index<1>{{15}}, 1.0f);
+    RecursiveGaussianFilter<Image<float, 1>> a(imp31, 2.0, 0);
+    auto oa = a.generate_region(a.largest_region());
+    CHECK(std::fabs(sum(oa.data) - 1.0f) < 0.02f, "single axis energy=%g", sum(oa.data));
+}
+
 int main() {
 #define RUN(t)                  \
     t();                        \
@@ -256,6 +278,7 @@ int main() {
     RUN(test_binomial_impulse_spreads);
     RUN(test_binomial_effective_sigma);
     RUN(test_unsharp_mask);
+    RUN(test_recursive_gaussian);
     std::printf("all passed\n");
     return 0;
 }

@@ -1,0 +1,72 @@
+"""Seeded random configurations through the GPU forward path, each checked
+against the oracle: EXACT bit for bit, FAST within 1e-5 (max abs error /
+max abs value).  Shapes 1- to 3-D with odd extents, sigmas from below one
+pixel to radii past the fused set, anisotropic spacings, sub-region
+requests (inside and touching the border), 1 or 3 components, f32 and f64,
+and both kernel paths; then MeanFilter / BinomialBlurFilter the same way."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5
+
+
+def _case(seed):
+    rng = np.random.default_rng(1000 + seed)
+    D = int(rng.integers(1, 4))
+    size = [int(rng.integers(1, 70 if D == 3 else (300 if D == 2 else 3000))) for _ in range(D)]
+    sigma = float(rng.choice([0.3, 0.8, 1.3, 2.0, 3.1, 5.2, 6.5]))
+    spacing = [float(rng.choice([0.5, 0.7, 1.0, 1.0, 1.0, 2.5])) for _ in range(D)]
+    if rng.random() < 0.5:
+        spacing = [spacing[0]] * D
+    lo = [int(rng.integers(0, s)) for s in size]
+    req = ([l for l in lo], [int(rng.integers(1, s - l + 1)) for s, l in zip(size, lo)])
+    if rng.random() < 0.3:
+        req = ([0] * D, list(size))
+    ncomp = 3 if rng.random() < 0.25 else 1
+    dtype = np.float64 if rng.random() < 0.15 else np.float32
+    return D, size, sigma, spacing, req, ncomp, dtype
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_random_gaussian_parity(itk, orc, dev, seed):
+    D, size, sigma, spacing, req, ncomp, dtype = _case(seed)
+    rng = np.random.default_rng(seed)
+    x = rng.random(tuple(reversed(size)) + ((ncomp,) if ncomp > 1 else ())).astype(dtype)
+    src = itk.Image(itk.Region([0] * D, size), tuple(spacing), (0.0,) * D, torch.from_numpy(x).to(dev))
+    reg = itk.Region(list(req[0]), list(req[1]))
+    ref = orc.generate_region(x, [0] * D, spacing, sigma, list(reg.index), list(reg.size), ncomp)
+    for path in (itk.PATH_AUTO, itk.PATH_PER_AXIS):
+        itk.set_path(path)
+        try:
+            f = itk.GaussianFilter(src, sigma)
+            f.arith = itk.EXACT
+            got = f.generate_region(reg).data.cpu().numpy()
+            assert np.array_equal(got, ref), (seed, path, D, size, sigma, spacing, req, ncomp, dtype)
+            f.arith = itk.FAST
+            fast = f.generate_region(reg).data.cpu().numpy()
+            scale = float(np.abs(ref).max()) or 1.0
+            assert float(np.abs(fast.astype(np.float64) - ref).max()) / scale <= FAST_TOL, (seed, path)
+        finally:
+            itk.set_path(itk.PATH_AUTO)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_random_filters_parity(itk, orc, dev, seed):
+    D, size, _, _, req, ncomp, dtype = _case(seed + 500)
+    rng = np.random.default_rng(seed + 500)
+    x = rng.random(tuple(reversed(size)) + ((ncomp,) if ncomp > 1 else ())).astype(dtype)
+    src = itk.Image(itk.Region([0] * D, size), (1.0,) * D, (0.0,) * D, torch.from_numpy(x).to(dev))
+    reg = itk.Region(list(req[0]), list(req[1]))
+    radii = [int(rng.integers(0, 9)) for _ in range(D)]
+    f = itk.BoxMeanFilterN(src, radii)
+    got = f.generate_region(reg).data.cpu().numpy()
+    ref = orc.mean_generate_region(x, [0] * D, radii, list(reg.index), list(reg.size), ncomp)
+    assert np.array_equal(got, ref), (seed, D, size, radii, req, ncomp, dtype)
+    reps = int(rng.integers(0, 6))
+    b = itk.BinomialBlurFilter(src, reps)
+    got = b.generate_region(reg).data.cpu().numpy()
+    ref = orc.binomial_generate_region(x, [0] * D, reps, list(reg.index), list(reg.size), ncomp)
+    assert np.array_equal(got, ref), (seed, D, size, reps, req, ncomp, dtype)

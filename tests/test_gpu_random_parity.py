@@ -1,9 +1,11 @@
-"""Seeded random configurations through the GPU forward path, each checked
+"""Seeded random configurations through the GPU paths, each checked
 against the oracle: EXACT bit for bit, FAST within 1e-5 (max abs error /
 max abs value).  Shapes 1- to 3-D with odd extents, sigmas from below one
 pixel to radii past the fused set, anisotropic spacings, sub-region
 requests (inside and touching the border), 1 or 3 components, f32 and f64,
-and both kernel paths; then MeanFilter / BinomialBlurFilter the same way."""
+and both kernel paths; then MeanFilter / BinomialBlurFilter the same way;
+the adjoint against the oracle's f64 transpose; random batched plans
+against plain launches, forward and adjoint."""
 import numpy as np
 import pytest
 import torch
@@ -70,3 +72,60 @@ def test_random_filters_parity(itk, orc, dev, seed):
     got = b.generate_region(reg).data.cpu().numpy()
     ref = orc.binomial_generate_region(x, [0] * D, reps, list(reg.index), list(reg.size), ncomp)
     assert np.array_equal(got, ref), (seed, D, size, reps, req, ncomp, dtype)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_random_adjoint_parity(itk, orc, dev, seed):
+    """GaussianFilter.backward of a random sub-request against the oracle's
+    f64 transpose (scattered taps): within 1e-5 of its max |value| in f32,
+    1e-12 in f64."""
+    D, size, sigma, spacing, req, ncomp, dtype = _case(seed + 200)
+    rng = np.random.default_rng(seed + 200)
+    src = itk.Image(itk.Region([0] * D, size), tuple(spacing), (0.0,) * D, None)
+    reg = itk.Region(list(req[0]), list(req[1]))
+    g = rng.standard_normal(tuple(reversed(reg.size)) + ((ncomp,) if ncomp > 1 else ())).astype(dtype)
+    want, ii, isz = orc.adjoint_f64(g, [0] * D, size, spacing, sigma, list(reg.index), list(reg.size),
+                                    ncomp)
+    f = itk.GaussianFilter(src, sigma)
+    for path in (itk.PATH_AUTO, itk.PATH_PER_AXIS):
+        itk.set_path(path)
+        try:
+            gimg = itk.Image(reg, tuple(spacing), (0.0,) * D, torch.from_numpy(g).to(dev))
+            gin = f.backward(gimg)
+            assert list(gin.region.index) == ii and list(gin.region.size) == isz
+            got = gin.data.cpu().numpy().astype(np.float64)
+            scale = float(np.abs(want).max()) or 1.0
+            tol = 1e-12 if dtype == np.float64 else 1e-5
+            assert float(np.abs(got - want).max()) / scale <= tol, (seed, path, D, size, sigma, req)
+        finally:
+            itk.set_path(itk.PATH_AUTO)
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("direction", [0, 1])
+def test_random_batch_equals_plain(itk, dev, seed, direction):
+    """A batched plan of random uneven volumes (random sigma and spacing per
+    volume, some past the fused radii) equals each volume's plain launch bit
+    for bit, before and after autotuning."""
+    rng = np.random.default_rng(seed + 300)
+    nvol = int(rng.integers(1, 9))
+    vols, sig, sps = [], [], []
+    for _ in range(nvol):
+        shape = tuple(int(v) for v in rng.integers(4, 60, size=3))
+        vols.append(torch.from_numpy(rng.random(shape).astype(np.float32)).to(dev))
+        sig.append(float(rng.uniform(0.3, 4.0)))
+        sps.append((float(rng.uniform(0.5, 2.0)),) * 3)
+    outs = [torch.full_like(v, float("nan")) for v in vols]
+    plan = itk.BatchPlan(vols, outs, sig, sps, direction=direction, arith=itk.FAST)
+    s = torch.cuda.Stream()
+    for tune in (False, True):
+        if tune:
+            plan.autotune(s)
+        plan.run(s)
+        s.synchronize()
+        for v, o, sg, sp in zip(vols, outs, sig, sps):
+            ref = itk.gaussian(v, sg, sp, arith=itk.FAST) if direction == 0 else \
+                itk.gaussian_backward(v, sg, sp)
+            torch.cuda.synchronize()
+            assert torch.equal(o, ref), (seed, tune, tuple(v.shape), sg, sp)
+    plan.close()

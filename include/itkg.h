@@ -206,7 +206,23 @@ int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, doubl
                                            const itkg_region* requested, void* output,
                                            itkg_arith arith);
 
-/* ---- recursive (IIR) Gaussian ----------------------------------------- */
+/* CannyEdgeDetectionFilter::generate_region, src/filters/edges.rs:548-664,
+ * 2-D scalar f32 / f64 (other ndim: ITKG_ERR_UNSUPPORTED; the reference
+ * asserts D = 2): the whole image smoothed with the kernel above (arith
+ * selects its arithmetic; EXACT is the reference's), f64 gradient by clamped
+ * central differences, magnitude and atan2 direction, non-maximum
+ * suppression, hysteresis between lower and upper; the request as 1 / 0.
+ * input covers `largest` (the filter's input_region_for_output).  The
+ * hysteresis repeats until a pass adds nothing, waiting on the stream after
+ * each pass, so the call returns when the output is written.  Results equal
+ * the reference's wherever the device's atan2 bins a direction as glibc's
+ * does: they differ only for a gradient angle within a few ulps of 22.5 +
+ * 45 k degrees. */
+int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, double lower,
+                               double upper, const itkg_region* largest, const itkg_view* input,
+                               const itkg_view* output, itkg_arith arith, void* stream);
+
+/* ---- recursive (IIR) Gaussian ----------------------------------------- *//* ---- recursive (IIR) Gaussian ----------------------------------------- */
 
 /* yvv_coeffs, src/filters/recursive_gaussian.rs:33-47: the Young - van Vliet
  * third-order coefficients (B, a1, a2, a3) for sigma in pixels (clamped below

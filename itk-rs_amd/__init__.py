@@ -43,6 +43,7 @@ EXPORTS = (
     "itkg_index_gaussian_kernel", "itkg_unsharp_mask_generate_region",
     "itkg_unsharp_mask_generate_region_host", "itkg_yvv_coefficients",
     "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
+    "itkg_canny_generate_region",
 )
 
 
@@ -111,6 +112,9 @@ def _load():
     L.itkg_recursive_gaussian_generate_region_host.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_double, P(ctypes.c_double), ctypes.c_int32,
         P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p]
+    L.itkg_canny_generate_region.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+        P(c_region), P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
     L.itkg_gaussian_backward.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
         P(c_region), P(c_view), P(c_view), ctypes.c_int32, ctypes.c_void_p]
@@ -702,6 +706,54 @@ class UnsharpMaskFilter(ImageSource):
 
     def generate_region(self, requested: Region) -> Image:
         inp = self.source.generate_region(self.input_region_for_output(requested))
+        _, nc = _pixel_info(inp.data, requested.ndim)
+        out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
+        self.generate_into(inp, out)
+        return out
+
+
+class CannyEdgeDetectionFilter(ImageSource):
+    """CannyEdgeDetectionFilter<S> (src/filters/edges.rs:527-665), 2-D scalar
+    f32 / f64: smoothing with the index-space kernel, gradient, non-maximum
+    suppression and hysteresis on the GPU; output 1 / 0.  The same public
+    fields (`source`, `sigma`, `lower_threshold`, `upper_threshold`)."""
+
+    arith = EXACT
+
+    def __init__(self, source, sigma: float, lower: float, upper: float):
+        self.source = source
+        self.sigma = float(sigma)
+        self.lower_threshold = float(lower)
+        self.upper_threshold = float(upper)
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def input_region_for_output(self, output_region: Region) -> Region:
+        """edges.rs:548-550: the whole image."""
+        return self.largest_region()
+
+    def generate_into(self, inp: Image, out: Image) -> None:
+        D = out.ndim
+        dt, nc = _pixel_info(inp.data, D)
+        if nc != 1:
+            raise ItkgError(ERR_UNSUPPORTED, "Canny takes scalar pixels")
+        with torch.cuda.device(inp.data.device):
+            _check(lib.itkg_canny_generate_region(
+                D, dt, self.sigma, self.lower_threshold, self.upper_threshold,
+                ctypes.byref(self.largest_region().to_c()), ctypes.byref(inp.view()),
+                ctypes.byref(out.view()), self.arith, _stream_ptr(inp.data.device)))
+
+    def generate_region(self, requested: Region) -> Image:
+        inp = self.source.generate_region(self.largest_region())
         _, nc = _pixel_info(inp.data, requested.ndim)
         out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
         self.generate_into(inp, out)

@@ -1163,6 +1163,79 @@ int itkg_recursive_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype,
     return rc;
 }
 
+int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, double lower,
+                               double upper, const itkg_region* largest, const itkg_view* input,
+                               const itkg_view* output, itkg_arith arith, void* stream) {
+    clear_error();
+    if (ndim != 2) return fail(ITKG_ERR_UNSUPPORTED, "Canny takes 2-D images");
+    if (!args_ok(ndim, dtype, 1)) return fail(ITKG_ERR_INVALID_ARG, "bad dtype");
+    if (!largest || !input || !output) return fail(ITKG_ERR_INVALID_ARG, "null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const itkg_region& L = *largest;
+    if (!region_ok(2, L) || !region_ok(2, output->region))
+        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
+    if (region_len(2, output->region) > 0 && !contains(2, L, output->region))
+        return fail(ITKG_ERR_INVALID_ARG, "requested region is not inside the largest region");
+    int rc = check_view(2, input, L, "input");
+    if (rc) return rc;
+    if ((rc = check_view(2, output, output->region, "output"))) return rc;
+    const int64_t n = region_len(2, L);
+    if (n == 0 || region_len(2, output->region) == 0) return ITKG_OK;
+    if (L.size[1] > 65535 || output->region.size[1] > 65535)
+        return fail(ITKG_ERR_UNSUPPORTED, "Canny: more than 65535 rows");
+    // 1. smoothing (edges.rs:563-577): the index-space kernel over the whole
+    // image, which is the separable path's whole-image request
+    const uint64_t r = radius_of(sigma, 1.0);
+    if (r > (uint64_t)ITKG_MAX_RADIUS) return fail(ITKG_ERR_UNSUPPORTED, "radius exceeds %d", ITKG_MAX_RADIUS);
+    double w[2 * kMaxTaps];
+    index_weights(sigma, w, kMaxTaps);
+    index_weights(sigma, w + 2 * r + 1, kMaxTaps);
+    const uint64_t radii[2] = {r, r};
+    Plan p;
+    if ((rc = make_plan_weights(2, radii, w, largest, L, &p, false))) return rc;
+    const size_t px = dtype_size(dtype);
+    void *sm = nullptr, *mag = nullptr, *nms = nullptr, *bin = nullptr, *edges = nullptr, *flag = nullptr;
+    hipError_t e = hipMallocAsync(&sm, (size_t)n * px, s);
+    if (e == hipSuccess) e = hipMallocAsync(&mag, (size_t)n * 8, s);
+    if (e == hipSuccess) e = hipMallocAsync(&nms, (size_t)n * 8, s);
+    if (e == hipSuccess) e = hipMallocAsync(&bin, (size_t)n, s);
+    if (e == hipSuccess) e = hipMallocAsync(&edges, (size_t)n, s);
+    if (e == hipSuccess) e = hipMallocAsync(&flag, sizeof(int), s);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMallocAsync(Canny scratch)");
+    const itkg_view smv = dense_view(2, sm, L);
+    if (!rc) rc = forward_impl(p, dtype, 1, *input, smv, arith, s, nullptr);
+    // 2-4. gradient, suppression, hysteresis, the request as 1 / 0
+    if (!rc) {
+        auto run = [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            CannyArgs<T> a{};
+            a.smoothed = static_cast<const T*>(sm);
+            a.mag = static_cast<double*>(mag);
+            a.nms = static_cast<double*>(nms);
+            a.bin = static_cast<uint8_t*>(bin);
+            a.edges = static_cast<uint8_t*>(edges);
+            a.changed = static_cast<int*>(flag);
+            a.out = static_cast<T*>(output->data);
+            a.nx = L.size[0];
+            a.ny = L.size[1];
+            a.ox = output->region.index[0] - L.index[0];
+            a.oy = output->region.index[1] - L.index[1];
+            a.mx = output->region.size[0];
+            a.my = output->region.size[1];
+            a.out_stride[0] = output->stride[0];
+            a.out_stride[1] = output->stride[1];
+            a.lower = lower;
+            a.upper = upper;
+            return launch_canny<T>(a, s);
+        };
+        e = dtype == ITKG_F64 ? run((double*)nullptr) : run((float*)nullptr);
+        if (e != hipSuccess) rc = hip_fail(e, "Canny launch");
+    }
+    for (void* q : {sm, mag, nms, bin, edges, flag})
+        if (q) (void)hipFreeAsync(q, s);
+    return rc;
+}
+
 int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
                            const double* spacing, const itkg_region* largest,
                            const itkg_view* grad_output, const itkg_view* grad_input,

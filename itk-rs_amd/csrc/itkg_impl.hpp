@@ -80,6 +80,25 @@ template <typename T>
 hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxes,
                             const double (*coef)[4], hipStream_t s);
 
+// Canny after smoothing: `smoothed` is the dense nx x ny smoothed image;
+// mag, nms (f64), bin, edges (u8) are nx x ny scratch and `changed` one int;
+// out (strides in pixels) receives the mx x my request at offset (ox, oy).
+template <typename T>
+struct CannyArgs {
+    const T* smoothed;
+    double* mag;
+    double* nms;
+    uint8_t* bin;
+    uint8_t* edges;
+    int* changed;
+    T* out;
+    int64_t nx, ny, ox, oy, mx, my;
+    int64_t out_stride[2];
+    double lower, upper;
+};
+template <typename T>
+hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s);
+
 // Fused one-round-trip 3-D kernel for f32 scalar pixels (forward or adjoint).
 // Destination pixel j reads source pixels j + shift + k - r, k = 0..2r
 // (forward: clamped into the source; adjoint: zero outside it).

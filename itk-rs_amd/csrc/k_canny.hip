@@ -1,0 +1,148 @@
+// The steps of CannyEdgeDetectionFilter::generate_region after smoothing
+// (src/filters/edges.rs:588-664), 2-D, whole image of nx x ny (axis 0
+// fastest): f64 gradient by clamped central differences, magnitude
+// sqrt(gx*gx + gy*gy), direction atan2(gy, gx) in degrees modulo 180 binned
+// at 22.5 / 67.5 / 112.5 / 157.5; non-maximum suppression against the two
+// clamped neighbours along the bin; hysteresis (strong = nms >= upper, then
+// every pixel with nms >= lower 8-connected to an edge).  Compiled with
+// -ffp-contract=off: the products and sums round as the reference's do.
+#include "itkg_impl.hpp"
+
+namespace itkg {
+
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(256) void canny_grad_kernel(const T* sm, double* mag, uint8_t* bin,
+                                                         const int64_t nx, const int64_t ny) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= nx) return;
+    const int64_t ip = i + 1 > nx - 1 ? nx - 1 : i + 1, im = i - 1 < 0 ? 0 : i - 1;
+    const int64_t jp = j + 1 > ny - 1 ? ny - 1 : j + 1, jm = j - 1 < 0 ? 0 : j - 1;
+    const double gx = (double)sm[ip + j * nx] - (double)sm[im + j * nx];
+    const double gy = (double)sm[i + jp * nx] - (double)sm[i + jm * nx];
+    mag[i + j * nx] = sqrt(gx * gx + gy * gy);
+    // to_degrees multiplies by the f64 nearest 180/pi; rem_euclid(180)
+    const double deg = atan2(gy, gx) * (180.0 / 3.14159265358979323846);
+    double angle = fmod(deg, 180.0);
+    if (angle < 0.0) angle += 180.0;
+    bin[i + j * nx] = angle < 22.5 || angle >= 157.5 ? 0 : (angle < 67.5 ? 1 : (angle < 112.5 ? 2 : 3));
+}
+
+__global__ __launch_bounds__(256) void canny_nms_kernel(const double* mag, const uint8_t* bin,
+                                                        double* nms, uint8_t* edges, const int64_t nx,
+                                                        const int64_t ny, const double upper) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= nx) return;
+    const int b = bin[i + j * nx];
+    const int64_t di = b == 0 ? 1 : (b == 1 ? 1 : (b == 2 ? 0 : -1));
+    const int64_t dj = b == 0 ? 0 : 1;
+    auto at = [&](int64_t a, int64_t c) {
+        a = a < 0 ? 0 : (a > nx - 1 ? nx - 1 : a);
+        c = c < 0 ? 0 : (c > ny - 1 ? ny - 1 : c);
+        return mag[a + c * nx];
+    };
+    const double m = mag[i + j * nx];
+    const double v = (m >= at(i + di, j + dj) && m >= at(i - di, j - dj)) ? m : 0.0;
+    nms[i + j * nx] = v;
+    edges[i + j * nx] = v >= upper ? 1 : 0;
+}
+
+// One hysteresis pass: each block grows the edges inside its 32 x 32 tile
+// (in LDS, with a one-pixel ring read from the global edge map) until the
+// tile is stable, writes them back and raises *changed if it added any.
+constexpr int kTile = 32;
+
+__global__ __launch_bounds__(256) void canny_hysteresis_kernel(const double* nms, uint8_t* edges,
+                                                               const int64_t nx, const int64_t ny,
+                                                               const double lower, int* changed) {
+    __shared__ uint8_t e[kTile + 2][kTile + 2];
+    __shared__ uint8_t weak[kTile][kTile];
+    __shared__ int grew;
+    const int64_t x0 = (int64_t)blockIdx.x * kTile, y0 = (int64_t)blockIdx.y * kTile;
+    for (int k = threadIdx.x; k < (kTile + 2) * (kTile + 2); k += 256) {
+        const int ty = k / (kTile + 2), tx = k % (kTile + 2);
+        const int64_t gx = x0 + tx - 1, gy = y0 + ty - 1;
+        e[ty][tx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? edges[gx + gy * nx] : 0;
+    }
+    for (int k = threadIdx.x; k < kTile * kTile; k += 256) {
+        const int ty = k / kTile, tx = k % kTile;
+        const int64_t gx = x0 + tx, gy = y0 + ty;
+        weak[ty][tx] = (gx < nx && gy < ny) ? (nms[gx + gy * nx] >= lower ? 1 : 0) : 0;
+    }
+    bool any = false;
+    for (;;) {
+        if (threadIdx.x == 0) grew = 0;
+        __syncthreads();
+        bool mine = false;
+        for (int k = threadIdx.x; k < kTile * kTile; k += 256) {
+            const int ty = k / kTile, tx = k % kTile;
+            if (weak[ty][tx] && !e[ty + 1][tx + 1]) {
+                if (e[ty][tx] | e[ty][tx + 1] | e[ty][tx + 2] | e[ty + 1][tx] | e[ty + 1][tx + 2] |
+                    e[ty + 2][tx] | e[ty + 2][tx + 1] | e[ty + 2][tx + 2]) {
+                    e[ty + 1][tx + 1] = 1;
+                    mine = true;
+                }
+            }
+        }
+        if (mine) grew = 1;
+        __syncthreads();
+        if (!grew) break;
+        any = true;
+        __syncthreads();
+    }
+    if (!any) return;
+    for (int k = threadIdx.x; k < kTile * kTile; k += 256) {
+        const int ty = k / kTile, tx = k % kTile;
+        const int64_t gx = x0 + tx, gy = y0 + ty;
+        if (gx < nx && gy < ny && e[ty + 1][tx + 1]) edges[gx + gy * nx] = 1;
+    }
+    if (threadIdx.x == 0) *changed = 1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void canny_out_kernel(const uint8_t* edges, T* out,
+                                                        const int64_t nx, const int64_t ox,
+                                                        const int64_t oy, const int64_t mx,
+                                                        const int64_t s0, const int64_t s1) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= mx) return;
+    out[i * s0 + j * s1] = edges[(i + ox) + (j + oy) * nx] ? (T)1.0 : (T)0.0;
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
+    const int64_t nx = a.nx, ny = a.ny;
+    if (ny > 65535 || a.my > 65535 || (ny + kTile - 1) / kTile > 65535) return hipErrorInvalidValue;
+    const dim3 g((unsigned)((nx + 255) / 256), (unsigned)ny);
+    hipLaunchKernelGGL(canny_grad_kernel<T>, g, dim3(256), 0, s, a.smoothed, a.mag, a.bin, nx, ny);
+    hipLaunchKernelGGL(canny_nms_kernel, g, dim3(256), 0, s, a.mag, a.bin, a.nms, a.edges, nx, ny,
+                       a.upper);
+    hipError_t e = hipGetLastError();
+    const dim3 gt((unsigned)((nx + kTile - 1) / kTile), (unsigned)((ny + kTile - 1) / kTile));
+    // passes until one adds nothing (each pass grows every tile to its own
+    // fixed point; a chain crossing k tiles needs about k passes)
+    for (int pass = 0; e == hipSuccess; ++pass) {
+        int host = 0;
+        e = hipMemsetAsync(a.changed, 0, sizeof(int), s);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.nms, a.edges, nx, ny,
+                           a.lower, a.changed);
+        e = hipMemcpyAsync(&host, a.changed, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess || !host) break;
+    }
+    if (e != hipSuccess) return e;
+    if (a.mx > 0 && a.my > 0)
+        hipLaunchKernelGGL(canny_out_kernel<T>, dim3((unsigned)((a.mx + 255) / 256), (unsigned)a.my),
+                           dim3(256), 0, s, a.edges, a.out, nx, a.ox, a.oy, a.mx, a.out_stride[0],
+                           a.out_stride[1]);
+    return hipGetLastError();
+}
+
+template hipError_t launch_canny<float>(const CannyArgs<float>&, hipStream_t);
+template hipError_t launch_canny<double>(const CannyArgs<double>&, hipStream_t);
+
+}  // namespace itkg

@@ -51,6 +51,7 @@
  *     ::generate_region (whole-buffer passes, the f64 per-pixel step)
  *   RecursiveGaussianFilter /       src/filters/recursive_gaussian.rs:33-240
  *     RecursiveGaussianAxisFilter (yvv_coeffs, causal / anticausal lines)
+ *   CannyEdgeDetectionFilter        src/filters/edges.rs:548-664 (2-D)
  *
  *   mean / binomial (f32)          : interior-impulse closed forms (box
  *                                  products; binomial coefficients / 4^n,
@@ -499,6 +500,102 @@ int NAME(int D, const T* img, const int64_t* img_index, const int64_t* img_size,
 }
 DEFINE_RECURSIVE(orc_recursive_generate_region_f32, float)
 DEFINE_RECURSIVE(orc_recursive_generate_region_f64, double)
+
+/* ---- CannyEdgeDetectionFilter: src/filters/edges.rs:548-664, loop for
+ * loop (2-D): smooth the whole image with the index-space kernel (every
+ * pass over the whole buffer), gradient by clamped central differences in
+ * f64, magnitude sqrt(gx*gx + gy*gy) and direction atan2(gy, gx); non-
+ * maximum suppression along the direction binned to 0/45/90/135 degrees;
+ * hysteresis: strong = nms >= upper, then sweeps adding every pixel with
+ * nms >= lower next to an edge (8 neighbours, clamped) until none changes;
+ * the requested region as 1 / 0. */
+#define DEFINE_CANNY(NAME, T, LEAF, CONV)                                        \
+int NAME(const T* img, const int64_t* img_index, const int64_t* img_size,        \
+         double sigma, double lower, double upper,                               \
+         const int64_t* req_index, const int64_t* req_size, T* out) {            \
+    const int D = 2;                                                             \
+    orc_region bounds = region_make(D, img_index, img_size);                     \
+    orc_region req = region_make(D, req_index, req_size);                        \
+    orc_region full_reg;                                                         \
+    T* full = LEAF(D, 1, img, bounds, bounds, &full_reg);                        \
+    uint64_t radius = orc_kernel_radius(sigma, 1.0);                             \
+    uint64_t klen = 2 * radius + 1;                                              \
+    double* kern = (double*)malloc(klen * sizeof(double));                       \
+    orc_index_kernel(sigma, radius, kern);                                       \
+    int64_t n = region_len(D, full_reg);                                         \
+    T* sm = (T*)malloc((size_t)(n > 0 ? n : 1) * sizeof(T));                     \
+    memcpy(sm, full, (size_t)n * sizeof(T));                                     \
+    for (int axis = 0; axis < D; ++axis) {                                       \
+        T* nxt = (T*)malloc((size_t)(n > 0 ? n : 1) * sizeof(T));               \
+        CONV(D, 1, sm, full_reg, axis, kern, klen, full_reg, nxt);               \
+        free(sm);                                                                \
+        sm = nxt;                                                                \
+    }                                                                            \
+    int64_t nx = bounds.size[0], ny = bounds.size[1];                            \
+    int64_t ox = bounds.index[0], oy = bounds.index[1];                          \
+    double* smd = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));     \
+    for (int64_t i = 0; i < n; ++i) smd[i] = (double)sm[i];                      \
+    double* mag = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));      \
+    double* dir = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));      \
+    for (int64_t j = 0; j < ny; ++j)                                             \
+        for (int64_t i = 0; i < nx; ++i) {                                       \
+            int64_t ip = i + 1 > nx - 1 ? nx - 1 : i + 1, im = i - 1 < 0 ? 0 : i - 1; \
+            int64_t jp = j + 1 > ny - 1 ? ny - 1 : j + 1, jm = j - 1 < 0 ? 0 : j - 1; \
+            double gx = smd[ip + j * nx] - smd[im + j * nx];                     \
+            double gy = smd[i + jp * nx] - smd[i + jm * nx];                     \
+            mag[i + j * nx] = sqrt(gx * gx + gy * gy);                           \
+            dir[i + j * nx] = atan2(gy, gx);                                     \
+        }                                                                        \
+    double* nms = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));      \
+    for (int64_t j = 0; j < ny; ++j)                                             \
+        for (int64_t i = 0; i < nx; ++i) {                                       \
+            int64_t f = i + j * nx;                                              \
+            double m = mag[f];                                                   \
+            double deg = dir[f] * (180.0 / 3.14159265358979323846);              \
+            double angle = fmod(deg, 180.0);                                     \
+            if (angle < 0) angle += 180.0;                                       \
+            int64_t di, dj;                                                      \
+            if (angle < 22.5 || angle >= 157.5) { di = 1; dj = 0; }              \
+            else if (angle < 67.5) { di = 1; dj = 1; }                           \
+            else if (angle < 112.5) { di = 0; dj = 1; }                          \
+            else { di = -1; dj = 1; }                                            \
+            int64_t a_i = i + di, a_j = j + dj, b_i = i - di, b_j = j - dj;      \
+            a_i = a_i < 0 ? 0 : (a_i > nx - 1 ? nx - 1 : a_i);                   \
+            a_j = a_j < 0 ? 0 : (a_j > ny - 1 ? ny - 1 : a_j);                   \
+            b_i = b_i < 0 ? 0 : (b_i > nx - 1 ? nx - 1 : b_i);                   \
+            b_j = b_j < 0 ? 0 : (b_j > ny - 1 ? ny - 1 : b_j);                   \
+            if (m >= mag[a_i + a_j * nx] && m >= mag[b_i + b_j * nx]) nms[f] = m; \
+        }                                                                        \
+    unsigned char* edges = (unsigned char*)calloc((size_t)(n > 0 ? n : 1), 1);   \
+    for (int64_t f = 0; f < n; ++f) if (nms[f] >= upper) edges[f] = 1;          \
+    int changed = 1;                                                             \
+    while (changed) {                                                            \
+        changed = 0;                                                             \
+        for (int64_t j = 0; j < ny; ++j)                                         \
+            for (int64_t i = 0; i < nx; ++i) {                                   \
+                int64_t f = i + j * nx;                                          \
+                if (!edges[f] && nms[f] >= lower) {                              \
+                    for (int64_t dj = -1; dj <= 1; ++dj)                         \
+                        for (int64_t di = -1; di <= 1; ++di) {                   \
+                            int64_t ni = i + di, nj = j + dj;                    \
+                            ni = ni < 0 ? 0 : (ni > nx - 1 ? nx - 1 : ni);       \
+                            nj = nj < 0 ? 0 : (nj > ny - 1 ? ny - 1 : nj);       \
+                            if (edges[ni + nj * nx]) { edges[f] = 1; changed = 1; } \
+                        }                                                        \
+                }                                                                \
+            }                                                                    \
+    }                                                                            \
+    int64_t m = region_len(D, req), o = 0;                                       \
+    for (int64_t j = req.index[1]; j < req.index[1] + req.size[1]; ++j)          \
+        for (int64_t i = req.index[0]; i < req.index[0] + req.size[0]; ++i)      \
+            out[o++] = edges[(i - ox) + (j - oy) * nx] ? (T)1.0 : (T)0.0;        \
+    (void)m;                                                                     \
+    free(edges); free(nms); free(dir); free(mag); free(smd);                     \
+    free(sm); free(full); free(kern);                                            \
+    return 0;                                                                    \
+}
+DEFINE_CANNY(orc_canny_generate_region_f32, float, leaf_copy_f32, conv_axis_f32)
+DEFINE_CANNY(orc_canny_generate_region_f64, double, leaf_copy_f64, conv_axis_f64)
 
 /* ---- adjoint in f64, by scattering the forward taps -------------------- */
 /* No reference counterpart.  The forward operator is A = A_{D-1} ... A_0

@@ -91,6 +91,11 @@ def lib():
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_int, ctypes.c_void_p, i64p, i64p, f64p, ctypes.c_double,
                            ctypes.c_int, i64p, i64p, ctypes.c_void_p]
+        for name in ("orc_canny_generate_region_f32", "orc_canny_generate_region_f64"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.c_void_p, i64p, i64p, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_double, i64p, i64p, ctypes.c_void_p]
         L.orc_adjoint_f64.restype = ctypes.c_int
         L.orc_adjoint_f64.argtypes = [ctypes.c_int, ctypes.c_int, i64p, i64p, f64p, ctypes.c_double,
                                       i64p, i64p, ctypes.c_void_p, ctypes.c_void_p, i64p, i64p]
@@ -282,6 +287,51 @@ def np_recursive(img: np.ndarray, spacing, sigma, axis) -> np.ndarray:
                 ym3, ym2, ym1 = ym2, ym1, cur
                 v[i] = cur
     return y
+
+
+def canny_generate_region(img: np.ndarray, img_index, sigma, lower, upper, req_index,
+                          req_size) -> np.ndarray:
+    """CannyEdgeDetectionFilter{source: img, sigma, lower, upper}.generate_region(req)
+    via the C oracle (src/filters/edges.rs:548-664), 2-D scalar."""
+    img = np.ascontiguousarray(img)
+    img_size = list(reversed(img.shape[:2]))
+    fn = lib().orc_canny_generate_region_f32 if img.dtype == np.float32 else \
+        lib().orc_canny_generate_region_f64
+    out = np.zeros(_shape(req_size, 1), dtype=img.dtype)
+    rc = fn(img.ctypes.data, _i64(img_index), _i64(img_size), float(sigma), float(lower),
+            float(upper), _i64(req_index), _i64(req_size), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    return out
+
+
+def np_canny(smoothed: np.ndarray, lower: float, upper: float) -> np.ndarray:
+    """Independent restatement of Canny's steps after smoothing (whole 2-D
+    image, numpy order [y, x]): vectorised gradient, magnitude, direction and
+    suppression; hysteresis as the connected components (8-neighbour) of
+    nms >= lower that hold a pixel with nms >= upper -- the fixed point the
+    reference's sweeps reach, by a different algorithm."""
+    from scipy import ndimage
+    sm = smoothed.astype(np.float64)
+    ny, nx = sm.shape
+    xi = np.arange(nx)
+    yi = np.arange(ny)
+    gx = sm[:, np.minimum(xi + 1, nx - 1)] - sm[:, np.maximum(xi - 1, 0)]
+    gy = sm[np.minimum(yi + 1, ny - 1), :] - sm[np.maximum(yi - 1, 0), :]
+    mag = np.sqrt(gx * gx + gy * gy)
+    ang = np.fmod(np.arctan2(gy, gx) * (180.0 / math.pi), 180.0)
+    ang = np.where(ang < 0, ang + 180.0, ang)
+    di = np.where((ang < 22.5) | (ang >= 157.5), 1, np.where(ang < 67.5, 1, np.where(ang < 112.5, 0, -1)))
+    dj = np.where((ang < 22.5) | (ang >= 157.5), 0, 1)
+    J, I = np.meshgrid(yi, xi, indexing="ij")
+    ma = mag[np.clip(J + dj, 0, ny - 1), np.clip(I + di, 0, nx - 1)]
+    mb = mag[np.clip(J - dj, 0, ny - 1), np.clip(I - di, 0, nx - 1)]
+    nms = np.where((mag >= ma) & (mag >= mb), mag, 0.0)
+    weak = nms >= lower
+    lab, nlab = ndimage.label(weak, structure=np.ones((3, 3)))
+    strong_labels = np.unique(lab[(nms >= upper) & weak])
+    edges = np.isin(lab, strong_labels[strong_labels > 0]) | (nms >= upper)
+    return edges.astype(smoothed.dtype)
 
 
 def ref_faithful(img: np.ndarray, img_index, spacing, sigma, req_index, req_size, ncomp=1,

@@ -270,3 +270,48 @@ def test_recursive_reference_tests_and_host_entry(itk, dev):
     assert np.array_equal(got, want.data.cpu().numpy())
     with pytest.raises(itk.ItkgError):
         itk.RecursiveGaussianAxisFilter(src, 1.0, 3).generate_region(src.region)
+
+
+# --- CannyEdgeDetectionFilter (src/filters/edges.rs:548-664) ---------------
+
+def _canny_images():
+    rng = np.random.default_rng(21)
+    yield "noise", rng.random((90, 120)).astype(np.float32), 1.0, 0.02, 0.06
+    yy, xx = np.mgrid[0:128, 0:160]
+    disc = (((xx - 80.0) ** 2 + (yy - 60.0) ** 2) < 40.0 ** 2).astype(np.float32)
+    yield "disc", disc + 0.05 * rng.random(disc.shape).astype(np.float32), 1.5, 0.05, 0.2
+    ramp = (xx / 160.0 + 0.3 * np.sin(yy / 6.0)).astype(np.float32)
+    yield "waves", ramp, 1.2, 0.01, 0.04
+    spiral = np.zeros((200, 200), np.float32)
+    t = np.linspace(0, 6 * np.pi, 4000)
+    sx = (100 + 2.8 * t * np.cos(t)).astype(int)
+    sy = (100 + 2.8 * t * np.sin(t)).astype(int)
+    spiral[np.clip(sy, 0, 199), np.clip(sx, 0, 199)] = 1.0   # one long chain for hysteresis
+    yield "spiral", spiral, 0.8, 0.02, 0.5
+
+
+@pytest.mark.parametrize("case", list(range(4)))
+def test_canny_matches_oracle(itk, orc, dev, case):
+    name, x, sigma, lower, upper = list(_canny_images())[case]
+    ny, nx = x.shape
+    src = itk.Image(itk.Region([0, 0], [nx, ny]), (1.0, 1.0), (0.0, 0.0), torch.from_numpy(x).to(dev))
+    f = itk.CannyEdgeDetectionFilter(src, sigma, lower, upper)
+    for req in (src.region, itk.Region([nx // 4, ny // 3], [nx // 2, ny // 2])):
+        out = f.generate_region(req)
+        torch.cuda.synchronize()
+        ref = orc.canny_generate_region(x, [0, 0], sigma, lower, upper, list(req.index), list(req.size))
+        assert np.array_equal(out.data.cpu().numpy(), ref), (name, req)
+    assert float(f.generate_region(src.region).data.sum()) > 0, name   # the case has edges
+
+
+def test_canny_f64_and_errors(itk, orc, dev):
+    rng = np.random.default_rng(5)
+    x = rng.random((50, 60))
+    src = itk.Image(itk.Region([0, 0], [60, 50]), (1.0, 1.0), (0.0, 0.0), torch.from_numpy(x).to(dev))
+    out = itk.CannyEdgeDetectionFilter(src, 1.0, 0.02, 0.08).generate_region(src.region)
+    ref = orc.canny_generate_region(x, [0, 0], 1.0, 0.02, 0.08, [0, 0], [60, 50])
+    assert np.array_equal(out.data.cpu().numpy(), ref)
+    v = itk.Image(itk.Region([0, 0, 0], [8, 8, 8]), (1.0,) * 3, (0.0,) * 3, torch.rand((8, 8, 8), device=dev))
+    with pytest.raises(itk.ItkgError) as e:
+        itk.CannyEdgeDetectionFilter(v, 1.0, 0.1, 0.2).generate_region(v.region)
+    assert e.value.code == itk.ERR_UNSUPPORTED

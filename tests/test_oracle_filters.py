@@ -1,11 +1,12 @@
 """Pins the oracle's restatements of the other convolve_axis filters --
 MeanFilter / BoxMeanFilterN (src/filters/mean.rs:15-18, 59-83, 126-144),
-BinomialBlurFilter (src/filters/binomial_blur.rs:57-89) and
-UnsharpMaskFilter (src/filters/edges.rs:406-452) and the recursive Gaussian
-(src/filters/recursive_gaussian.rs:33-240) -- before the GPU path is checked
-against them.  UnsharpMaskFilter has no tests in the reference: its
-pins are the kernel's closed form, identity cases, the dense operator and
-crop consistency.
+BinomialBlurFilter (src/filters/binomial_blur.rs:57-89),
+UnsharpMaskFilter (src/filters/edges.rs:406-452), the recursive Gaussian
+(src/filters/recursive_gaussian.rs:33-240) and CannyEdgeDetectionFilter
+(src/filters/edges.rs:548-664) -- before the GPU path is checked against
+them.  UnsharpMaskFilter and Canny have no tests in the reference: their
+pins are closed forms, identity cases, the dense operator, an independent
+restatement and crop consistency.
 
 Checks: the reference's own unit tests for both filters (mean.rs:261-294,
 binomial_blur.rs:92-129), interior-impulse closed forms that are exact in
@@ -288,3 +289,42 @@ def test_recursive_c_vs_numpy(orc, shape, spacing, sigma, axis):
     x32 = x.astype(np.float32)
     a32 = orc.recursive_generate_region(x32, [0] * D, list(spacing), sigma, axis, [0] * D, list(shape))
     assert np.array_equal(a32, orc.np_recursive(x32, list(spacing), sigma, axis).astype(np.float32))
+
+
+# --- CannyEdgeDetectionFilter (edges.rs:548-664) -------------------------------
+
+@pytest.mark.parametrize("seed,sigma,lower,upper", [(1, 1.0, 0.05, 0.15), (2, 1.5, 0.02, 0.1),
+                                                    (3, 0.8, 0.1, 0.3)])
+def test_canny_c_vs_numpy(orc, seed, sigma, lower, upper):
+    # independent: the blur as the dense f64 operator of the index-space
+    # kernel (whole-image passes clamp at the image border), then np_canny;
+    # random data leave no magnitude or threshold ties for rounding to flip
+    rng = np.random.default_rng(seed)
+    x = rng.random((40, 52)).astype(np.float64)
+    got = orc.canny_generate_region(x, [0, 0], sigma, lower, upper, [0, 0], [52, 40])
+    r = math.ceil(3.0 * sigma)
+    w = np.array([math.exp(-((i - r) ** 2) / (2.0 * sigma * sigma)) for i in range(2 * r + 1)])
+    w = w / w.sum()
+    blur = orc.apply_dense([orc.axis_matrix(w, 0, 52, 0, 52), orc.axis_matrix(w, 0, 40, 0, 40)], x)
+    assert np.array_equal(got, orc.np_canny(blur, lower, upper))
+
+
+def test_canny_step_and_constant(orc):
+    x = np.zeros((30, 40), np.float32)
+    x[:, 20:] = 1.0                                   # a vertical step between x = 19 and 20
+    out = orc.canny_generate_region(x, [0, 0], 1.0, 0.1, 0.3, [0, 0], [40, 30])
+    cols = np.nonzero(out.any(axis=0))[0]
+    assert set(cols.tolist()) <= {19, 20} and len(cols) >= 1
+    assert out[:, cols[0]].all()                      # the whole edge column is found
+    c = np.full((30, 40), 3.0, np.float32)
+    assert not orc.canny_generate_region(c, [0, 0], 1.0, 0.1, 0.3, [0, 0], [40, 30]).any()
+    # upper above every magnitude: no strong pixel, no edges
+    assert not orc.canny_generate_region(x, [0, 0], 1.0, 0.1, 1e9, [0, 0], [40, 30]).any()
+
+
+def test_canny_subregion_is_crop(orc):
+    rng = np.random.default_rng(9)
+    x = rng.random((33, 45)).astype(np.float32)
+    full = orc.canny_generate_region(x, [0, 0], 1.2, 0.05, 0.12, [0, 0], [45, 33])
+    sub = orc.canny_generate_region(x, [0, 0], 1.2, 0.05, 0.12, [7, 4], [20, 15])
+    assert np.array_equal(sub, full[4:19, 7:27])

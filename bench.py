@@ -20,7 +20,7 @@ reference, the accuracy the target is stated at).  Also reported (under
   - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration,
   - MeanFilter (radius 2, FAST and EXACT), BinomialBlurFilter (4
     repetitions), UnsharpMaskFilter (sigma 2) and RecursiveGaussianFilter
-    (sigma 2) on a 512^3 volume.
+    (sigma 2) on a 512^3 volume; CannyEdgeDetectionFilter on a 2048^2 image.
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
 write) / the average kernel duration from HIP events on the launch stream,
@@ -323,6 +323,22 @@ def main():
                 "gvox_s": round(nm * 10 / wr / 1e9, 3), "ms": round(float(np.mean(msr)), 4),
                 "hbm_frac_104B": round(104.0 * nm / (np.mean(msr) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             del xm, ym, im, om
+            torch.cuda.empty_cache()
+            # CannyEdgeDetectionFilter on a 2048^2 disc with noise (sigma 1.5):
+            # smoothing, gradient, suppression, hysteresis passes to the fixed
+            # point (each pass waits on the stream)
+            yy, xx = torch.meshgrid(torch.arange(2048, device=dev), torch.arange(2048, device=dev),
+                                    indexing="ij")
+            disc = (((xx - 1024.0) ** 2 + (yy - 1000.0) ** 2) < 600.0 ** 2).float()
+            xc = disc + 0.05 * torch.rand((2048, 2048), device=dev)
+            ic = itk.Image(itk.Region([0, 0], [2048, 2048]), (1.0, 1.0), (0.0, 0.0), xc)
+            oc = itk.Image(ic.region, (1.0, 1.0), (0.0, 0.0), torch.empty_like(xc))
+            cf = itk.CannyEdgeDetectionFilter(ic, 1.5, 0.05, 0.2)
+            wc, msc2 = timed_loop(lambda: cf.generate_into(ic, oc), 10, 2, stream)
+            extras["canny_2048sq"] = {"mpix_s": round(xc.numel() * 10 / wc / 1e6, 1),
+                                      "ms": round(wc / 10 * 1e3, 4),
+                                      "edge_pixels": int(oc.data.sum().item())}
+            del xc, oc, ic, disc, xx, yy
             torch.cuda.empty_cache()
 
         result["extras"] = extras

@@ -222,7 +222,7 @@ int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, dou
                                double upper, const itkg_region* largest, const itkg_view* input,
                                const itkg_view* output, itkg_arith arith, void* stream);
 
-/* ---- recursive (IIR) Gaussian ----------------------------------------- *//* ---- recursive (IIR) Gaussian ----------------------------------------- */
+/* ---- recursive (IIR) Gaussian ----------------------------------------- */
 
 /* yvv_coeffs, src/filters/recursive_gaussian.rs:33-47: the Young - van Vliet
  * third-order coefficients (B, a1, a2, a3) for sigma in pixels (clamped below

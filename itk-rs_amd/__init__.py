@@ -830,6 +830,106 @@ class RecursiveGaussianAxisFilter(RecursiveGaussianFilter):
 
 
 # ---------------------------------------------------------------------------
+# in-crate GaussianFilter callers: the adapters whose whole work is the blur
+# ---------------------------------------------------------------------------
+
+class BSplineDownsampleFilter(ImageSource):
+    """BSplineDownsampleFilter<S, P> (src/filters/spatial.rs:1128-1170):
+    GaussianFilter with sigma 1.0 over the source's whole largest region,
+    then every second pixel along each axis from the region's first index.
+    Like the reference it returns the whole output region whatever the
+    request."""
+
+    def __init__(self, source):
+        self.source = source
+
+    def largest_region(self) -> Region:
+        r = self.source.largest_region()
+        return Region(list(r.index), [(s + 1) // 2 for s in r.size])
+
+    @property
+    def spacing(self):
+        return tuple(2.0 * s for s in self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def generate_region(self, requested: Region) -> Image:
+        smoothed = GaussianFilter(self.source, 1.0).generate_region(self.source.largest_region())
+        D = smoothed.ndim
+        pick = smoothed.data[(slice(None, None, 2),) * D]
+        return Image(self.largest_region(), self.spacing, self.origin, pick.contiguous())
+
+
+class SliceBySliceFilter(ImageSource):
+    """SliceBySliceFilter<S, P> (src/filters/spatial.rs:1284-1340): a 2-D
+    GaussianFilter (sigma over the slice's x and y spacing) on every z slice
+    of a 3-D image, each slice's buffer being the pulled input's.  Here one
+    separable call over the volume with z radius 0 (the weight 1.0, which
+    leaves each value as the y pass wrote it)."""
+
+    arith = EXACT
+
+    def __init__(self, source, sigma: float):
+        self.source = source
+        self.sigma = float(sigma)
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def generate_region(self, requested: Region) -> Image:
+        if requested.ndim != 3:
+            raise ValueError("SliceBySliceFilter takes 3-D images")
+        inp = self.source.generate_region(requested)
+        radii = [kernel_radius(self.sigma, sp) for sp in inp.spacing[:2]] + [0]
+        weights = [build_kernel(self.sigma / sp, r) for sp, r in zip(inp.spacing[:2], radii)]
+        weights.append([1.0])
+        _, nc = _pixel_info(inp.data, 3)
+        out = Image.empty(inp.region, inp.spacing, inp.origin, nc, inp.data.dtype,
+                          inp.data.device)
+        _separable_into(radii, weights, inp.region, inp, out, self.arith)
+        return out
+
+
+class DisplacementFieldToBSplineFilter(ImageSource):
+    """DisplacementFieldToBSplineFilter<S> (src/filters/spatial.rs:1489-1525):
+    a 2-D field of VecPixel<f32, 2>, each component smoothed by GaussianFilter
+    (sigma 2.0 by default) over the pulled input's own region.  One call with
+    two interleaved components: each component's arithmetic is the scalar
+    filter's."""
+
+    def __init__(self, source, sigma: float = 2.0):
+        self.source = source
+        self.sigma = float(sigma)
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def generate_region(self, requested: Region) -> Image:
+        inp = self.source.generate_region(requested)
+        if inp.ndim != 2 or _pixel_info(inp.data, 2)[1] != 2:
+            raise ValueError("DisplacementFieldToBSplineFilter takes 2-D fields of 2 components")
+        return GaussianFilter(inp, self.sigma).generate_region(inp.region)
+
+
+# ---------------------------------------------------------------------------
 # functional / autograd entry points
 # ---------------------------------------------------------------------------
 

@@ -45,6 +45,44 @@ int hip_fail(hipError_t e, const char* what) {
 
 void clear_error() { g_err.clear(); }
 
+// Scratch memory of the entry points comes from a stream-ordered pool per
+// device that the library owns and that keeps up to kPoolKeepBytes of freed
+// memory across synchronisations.  A device's default pool returns freed
+// memory to the driver at every synchronisation, and a caller that waits
+// between calls would then pay for mapping each call's scratch again.
+constexpr uint64_t kPoolKeepBytes = 8ull << 30;
+constexpr int kPoolDevices = 64;
+
+hipError_t pool_alloc(void** ptr, size_t bytes, hipStream_t s) {
+    static std::mutex mu;
+    static hipMemPool_t pools[kPoolDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kPoolDevices) return hipErrorInvalidDevice;
+    hipMemPool_t pool;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!pools[dev]) {
+            hipMemPoolProps props{};
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            hipMemPool_t created = nullptr;
+            if ((e = hipMemPoolCreate(&created, &props)) != hipSuccess) return e;
+            uint64_t keep = kPoolKeepBytes;
+            if ((e = hipMemPoolSetAttribute(created, hipMemPoolAttrReleaseThreshold, &keep)) !=
+                hipSuccess) {
+                (void)hipMemPoolDestroy(created);
+                return e;
+            }
+            pools[dev] = created;
+        }
+        pool = pools[dev];
+    }
+    return hipMallocFromPoolAsync(ptr, bytes, pool, s);
+}
+
 // ---- region algebra (src/image.rs:17-64) --------------------------------
 
 int64_t region_len(int D, const itkg_region& r) {
@@ -271,8 +309,8 @@ int forward_per_axis(const Plan& p, int ncomp, const itkg_view& in, const itkg_v
     void* own = nullptr;
     const size_t sb = scratch_bytes(p, sizeof(T) == 8 ? ITKG_F64 : ITKG_F32, ncomp);
     if (D > 1 && !scratch && sb > 0) {
-        hipError_t e = hipMallocAsync(&own, sb, s);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(per-axis scratch)");
+        hipError_t e = pool_alloc(&own, sb, s);
+        if (e != hipSuccess) return hip_fail(e, "pool_alloc(per-axis scratch)");
         scratch = own;
     }
     char* buf[2] = {static_cast<char*>(scratch), static_cast<char*>(scratch) + sb / 2};
@@ -485,8 +523,8 @@ int backward_per_axis(const Plan& p, int ncomp, const itkg_view& gout, const itk
     void* own = nullptr;
     const size_t sb = scratch_bytes(p, sizeof(T) == 8 ? ITKG_F64 : ITKG_F32, ncomp);
     if (D > 1 && !scratch && sb > 0) {
-        hipError_t e = hipMallocAsync(&own, sb, s);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(adjoint scratch)");
+        hipError_t e = pool_alloc(&own, sb, s);
+        if (e != hipSuccess) return hip_fail(e, "pool_alloc(adjoint scratch)");
         scratch = own;
     }
     char* buf[2] = {static_cast<char*>(scratch), static_cast<char*>(scratch) + sb / 2};
@@ -538,10 +576,10 @@ int host_forward_serial(const Plan& p, itkg_dtype dtype, int ncomp, const void* 
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     void *din = nullptr, *dout = nullptr;
-    e = hipMallocAsync(&din, (size_t)nin * px, s);
-    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
+    e = pool_alloc(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = pool_alloc(&dout, (size_t)nout * px, s);
     if (e != hipSuccess) {
-        rc = hip_fail(e, "hipMallocAsync(host staging)");
+        rc = hip_fail(e, "pool_alloc(host staging)");
     } else {
         e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) rc = hip_fail(e, "copy in");
@@ -935,8 +973,8 @@ int unsharp_impl(int ndim, itkg_dtype dtype, double sigma, double amount, double
     if (n == 0) return ITKG_OK;
     const size_t px = dtype_size(dtype);
     void* blur = nullptr;
-    hipError_t e = hipMallocAsync(&blur, (size_t)n * px, s);
-    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(unsharp blur)");
+    hipError_t e = pool_alloc(&blur, (size_t)n * px, s);
+    if (e != hipSuccess) return hip_fail(e, "pool_alloc(unsharp blur)");
     const itkg_view bv = dense_view(ndim, blur, p.req);
     rc = forward_impl(p, dtype, 1, in, bv, arith, s, nullptr);
     if (!rc) {
@@ -1001,8 +1039,8 @@ int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, doubl
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     int rc = ITKG_OK;
     void *din = nullptr, *dout = nullptr;
-    e = hipMallocAsync(&din, (size_t)nin * px, s);
-    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
+    e = pool_alloc(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = pool_alloc(&dout, (size_t)nout * px, s);
     if (e == hipSuccess) e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = hip_fail(e, "unsharp host staging");
     if (!rc) {
@@ -1078,8 +1116,8 @@ int recursive_impl(int ndim, itkg_dtype dtype, double sigma, const double* spaci
         ++naxes;
     }
     void* work = nullptr;
-    hipError_t e = hipMallocAsync(&work, (size_t)total * sizeof(double), s);
-    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(recursive work)");
+    hipError_t e = pool_alloc(&work, (size_t)total * sizeof(double), s);
+    if (e != hipSuccess) return hip_fail(e, "pool_alloc(recursive work)");
     auto run = [&](auto* tag) {
         using T = std::remove_pointer_t<decltype(tag)>;
         RecursiveArgs<T> a{};
@@ -1142,8 +1180,8 @@ int itkg_recursive_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype,
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     int rc = ITKG_OK;
     void *din = nullptr, *dout = nullptr;
-    e = hipMallocAsync(&din, (size_t)nin * px, s);
-    if (e == hipSuccess) e = hipMallocAsync(&dout, (size_t)nout * px, s);
+    e = pool_alloc(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = pool_alloc(&dout, (size_t)nout * px, s);
     if (e == hipSuccess) e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = hip_fail(e, "recursive host staging");
     if (!rc) {
@@ -1194,14 +1232,27 @@ int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, dou
     Plan p;
     if ((rc = make_plan_weights(2, radii, w, largest, L, &p, false))) return rc;
     const size_t px = dtype_size(dtype);
+    // the six scratch buffers carved from one allocation: each stream-ordered
+    // free costs the host about 0.1 ms, more than this filter's kernels at
+    // moderate sizes
+    auto up = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    const size_t o_mag = up((size_t)n * px), o_nms = o_mag + up((size_t)n * 8);
+    const size_t o_bin = o_nms + up((size_t)n * 8), o_edges = o_bin + up((size_t)n);
+    const size_t o_flag = o_edges + up((size_t)n), total_bytes = o_flag + 256;
+    void* block = nullptr;
     void *sm = nullptr, *mag = nullptr, *nms = nullptr, *bin = nullptr, *edges = nullptr, *flag = nullptr;
-    hipError_t e = hipMallocAsync(&sm, (size_t)n * px, s);
-    if (e == hipSuccess) e = hipMallocAsync(&mag, (size_t)n * 8, s);
-    if (e == hipSuccess) e = hipMallocAsync(&nms, (size_t)n * 8, s);
-    if (e == hipSuccess) e = hipMallocAsync(&bin, (size_t)n, s);
-    if (e == hipSuccess) e = hipMallocAsync(&edges, (size_t)n, s);
-    if (e == hipSuccess) e = hipMallocAsync(&flag, sizeof(int), s);
-    if (e != hipSuccess) rc = hip_fail(e, "hipMallocAsync(Canny scratch)");
+    hipError_t e = pool_alloc(&block, total_bytes, s);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "pool_alloc(Canny scratch)");
+    } else {
+        char* base = static_cast<char*>(block);
+        sm = base;
+        mag = base + o_mag;
+        nms = base + o_nms;
+        bin = base + o_bin;
+        edges = base + o_edges;
+        flag = base + o_flag;
+    }
     const itkg_view smv = dense_view(2, sm, L);
     if (!rc) rc = forward_impl(p, dtype, 1, *input, smv, arith, s, nullptr);
     // 2-4. gradient, suppression, hysteresis, the request as 1 / 0
@@ -1231,8 +1282,7 @@ int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, dou
         e = dtype == ITKG_F64 ? run((double*)nullptr) : run((float*)nullptr);
         if (e != hipSuccess) rc = hip_fail(e, "Canny launch");
     }
-    for (void* q : {sm, mag, nms, bin, edges, flag})
-        if (q) (void)hipFreeAsync(q, s);
+    if (block) (void)hipFreeAsync(block, s);
     return rc;
 }
 

@@ -4,10 +4,13 @@
 //   y[n] = B x[n] + a1 y[n-1] + a2 y[n-2] + a3 y[n-3]
 // (recursive_gaussian.rs:55-86), each started from the line's first
 // (causal) or last (anticausal) value; then the requested region narrowed to
-// the pixel type.  One thread runs one line; consecutive threads take
+// the pixel type.  The widening happens as the first pass reads the input
+// and the narrowing as the last pass writes the output.  One thread runs one line; consecutive threads take
 // neighbouring lines, which are adjacent in memory (coalesced) for every
 // axis but axis 0.  Compiled with -ffp-contract=off: each product and sum
 // rounds on its own, in the reference's order.
+#include <type_traits>
+
 #include "itkg_impl.hpp"
 
 namespace itkg {
@@ -43,28 +46,92 @@ __global__ __launch_bounds__(256) void widen_kernel(const RecursiveArgs<T> a) {
     }
 }
 
+// Where a line pass reads its samples (bit 0: the input image, widened to
+// f64, instead of the f64 work image) and writes its results (bit 1: the
+// requested part of each line, narrowed to T, into the output instead of
+// the work image).  The first filtered axis reads the input and the last
+// writes the output, so no pass of its own widens or narrows.
+constexpr int kFromIn = 1, kToOut = 2;
+
+// Samples per chunk of a row (axis 0) line: 16 f64 samples are one whole
+// 128-byte line of the work image, so no chunk leaves half a cache line to
+// be fetched (or written back) again by the next.
+#ifndef ITKG_REC_CHUNK
+#define ITKG_REC_CHUNK 32
+#endif
+constexpr int kChunk = ITKG_REC_CHUNK;
+static_assert(kChunk % 4 == 0, "chunks are whole float4 / double2 groups");
+
+template <typename T>
+__device__ __forceinline__ void load_chunk(const T* q, double* c) {
+    if constexpr (std::is_same_v<T, float>) {
+#pragma unroll
+        for (int j = 0; j < kChunk / 4; ++j) {
+            const float4 u = reinterpret_cast<const float4*>(q)[j];
+            c[4 * j] = u.x, c[4 * j + 1] = u.y, c[4 * j + 2] = u.z, c[4 * j + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kChunk / 2; ++j) {
+            const double2 t = reinterpret_cast<const double2*>(q)[j];
+            c[2 * j] = t.x, c[2 * j + 1] = t.y;
+        }
+    }
+}
+
+__device__ __forceinline__ void store_chunk(double* q, const double* c) {
+#pragma unroll
+    for (int j = 0; j < kChunk / 2; ++j)
+        reinterpret_cast<double2*>(q)[j] = double2{c[2 * j], c[2 * j + 1]};
+}
+
 // One line per thread along `axis`: lines enumerate the other axes, the
-// lowest of them fastest.
-__global__ __launch_bounds__(256) void line_kernel(double* work, const int64_t n0, const int64_t n1,
-                                                   const int64_t n2, const int64_t n3, const int axis,
+// lowest of them fastest.  With kToOut, lines outside the request (in the
+// other axes) are not needed and not run.  ROWS (axis 0) compiles the chunked
+// row path; the other axes' kernels leave it out, and with it the chunk's
+// registers, which would cost them occupancy.
+template <typename T, int IO, bool ROWS>
+__global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, const int axis,
                                                    const double B, const double a1, const double a2,
                                                    const double a3) {
-    const int64_t n[4] = {n0, n1, n2, n3};
+    constexpr bool from_in = (IO & kFromIn) != 0, to_out = (IO & kToOut) != 0;
     int64_t pitch[4];
     pitch[0] = 1;
-    for (int d = 1; d < 4; ++d) pitch[d] = pitch[d - 1] * n[d - 1];
-    const int64_t len = n[axis], stride = pitch[axis];
-    const int64_t lines = n0 * n1 * n2 * n3 / len;
+    for (int d = 1; d < 4; ++d) pitch[d] = pitch[d - 1] * a.n[d - 1];
+    const int64_t len = a.n[axis], stride = pitch[axis];
+    const int64_t lines = a.n[0] * a.n[1] * a.n[2] * a.n[3] / len;
     const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (l >= lines) return;
-    int64_t rem = l, base = 0;
+    int64_t rem = l, base = 0, ibase = 0, obase = 0;
     for (int d = 0; d < 4; ++d) {
         if (d == axis) continue;
-        base += (rem % n[d]) * pitch[d];
-        rem /= n[d];
+        const int64_t c = rem % a.n[d];
+        rem /= a.n[d];
+        base += c * pitch[d];
+        ibase += c * a.in_stride[d];
+        if constexpr (to_out) {
+            const int64_t o = c - a.off[d];
+            if (o < 0 || o >= a.m[d]) return;
+            obase += o * a.out_stride[d];
+        }
     }
-    double* p = work + base;
-    double init = p[0];
+    double* p = a.work + base;
+    const T* q = a.in + ibase;
+    T* o = a.out + obase;
+    const int64_t is = a.in_stride[axis], os = a.out_stride[axis];
+    const int64_t lo = a.off[axis], hi = lo + a.m[axis];
+    auto src = [&](int64_t i) -> double {
+        if constexpr (from_in) return (double)q[i * is];
+        else return p[i * stride];
+    };
+    auto put = [&](int64_t i, double v) {
+        if constexpr (to_out) {
+            if (i >= lo && i < hi) o[(i - lo) * os] = (T)v;
+        } else {
+            p[i * stride] = v;
+        }
+    };
+    double init = src(0);
     double ym1 = init, ym2 = init, ym3 = init;
     auto step = [&](double x) {
         const double v = B * x + a1 * ym1 + a2 * ym2 + a3 * ym3;
@@ -73,37 +140,38 @@ __global__ __launch_bounds__(256) void line_kernel(double* work, const int64_t n
         ym1 = v;
         return v;
     };
-    // Rows (stride 1) that start 16-byte aligned go in chunks of eight
+    // Rows (stride 1) that start 16-byte aligned go in chunks of kChunk
     // samples with 16-byte loads and stores, each lane through its own
     // contiguous run; the samples still pass through `step` one by one.
-    const bool rows = stride == 1 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
-    const int64_t body = rows ? len / 8 * 8 : 0;
-    for (int64_t i = 0; i < body; i += 8) {
-        double2* q = reinterpret_cast<double2*>(p + i);
-        double2 c[4] = {q[0], q[1], q[2], q[3]};
+    bool rows = ROWS && stride == 1 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+    if constexpr (from_in) rows = rows && is == 1 && (reinterpret_cast<uintptr_t>(q) & 15) == 0;
+    const int64_t body = rows ? len / kChunk * kChunk : 0;
+    if constexpr (ROWS)
+    for (int64_t i = 0; i < body; i += kChunk) {
+        double c[kChunk];
+        if constexpr (from_in) load_chunk(q + i, c);
+        else load_chunk(p + i, c);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            c[j].x = step(c[j].x);
-            c[j].y = step(c[j].y);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) q[j] = c[j];
+        for (int j = 0; j < kChunk; ++j) c[j] = step(c[j]);
+        store_chunk(p + i, c);
     }
-    for (int64_t i = body; i < len; ++i) p[i * stride] = step(p[i * stride]);
+    for (int64_t i = body; i < len; ++i) p[i * stride] = step(src(i));
     init = ym1;  // the line's last value after the causal pass
     ym1 = ym2 = ym3 = init;
     // backwards: the samples past the last whole chunk first, then the chunks
-    for (int64_t i = len - 1; i >= body; --i) p[i * stride] = step(p[i * stride]);
-    for (int64_t i = body - 8; i >= 0; i -= 8) {
-        double2* q = reinterpret_cast<double2*>(p + i);
-        double2 c[4] = {q[0], q[1], q[2], q[3]};
+    for (int64_t i = len - 1; i >= body; --i) put(i, step(p[i * stride]));
+    if constexpr (ROWS)
+    for (int64_t i = body - kChunk; i >= 0; i -= kChunk) {
+        double c[kChunk];
+        load_chunk(p + i, c);
 #pragma unroll
-        for (int j = 3; j >= 0; --j) {
-            c[j].y = step(c[j].y);
-            c[j].x = step(c[j].x);
+        for (int j = kChunk - 1; j >= 0; --j) c[j] = step(c[j]);
+        if constexpr (to_out) {
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) put(i + j, c[j]);
+        } else {
+            store_chunk(p + i, c);
         }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) q[j] = c[j];
     }
 }
 
@@ -123,6 +191,28 @@ __global__ __launch_bounds__(256) void narrow_kernel(const RecursiveArgs<T> a) {
     }
 }
 
+template <typename T, int IO>
+void launch_line_io(bool rows, dim3 grid, hipStream_t s, const RecursiveArgs<T>& a, int ax,
+                    const double* c) {
+    if (rows)
+        hipLaunchKernelGGL((line_kernel<T, IO, true>), grid, dim3(256), 0, s, a, ax, c[0], c[1], c[2],
+                           c[3]);
+    else
+        hipLaunchKernelGGL((line_kernel<T, IO, false>), grid, dim3(256), 0, s, a, ax, c[0], c[1], c[2],
+                           c[3]);
+}
+
+template <typename T>
+void launch_line(int io, bool rows, dim3 grid, hipStream_t s, const RecursiveArgs<T>& a, int ax,
+                 const double* c) {
+    switch (io) {
+        case 0: launch_line_io<T, 0>(rows, grid, s, a, ax, c); break;
+        case kFromIn: launch_line_io<T, kFromIn>(rows, grid, s, a, ax, c); break;
+        case kToOut: launch_line_io<T, kToOut>(rows, grid, s, a, ax, c); break;
+        default: launch_line_io<T, kFromIn | kToOut>(rows, grid, s, a, ax, c); break;
+    }
+}
+
 }  // namespace
 
 template <typename T>
@@ -132,24 +222,31 @@ hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxe
     if (total == 0) return hipSuccess;
     if (a.n[1] > 65535 || a.n[2] * a.n[3] > 65535 || a.m[1] > 65535 || a.m[2] * a.m[3] > 65535)
         return hipErrorInvalidValue;  // grid rows / planes beyond the launch limit
-    hipLaunchKernelGGL(widen_kernel<T>,
-                       dim3((unsigned)((a.n[0] + kRowBlock - 1) / kRowBlock), (unsigned)a.n[1],
-                            (unsigned)(a.n[2] * a.n[3])),
-                       dim3(256), 0, s, a);
-    for (int k = 0; k < naxes; ++k) {
-        const int ax = axes[k];
-        if (a.n[ax] <= 1) continue;  // a line of one sample is left as it is
-        const int64_t lines = total / a.n[ax];
-        hipLaunchKernelGGL(line_kernel, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s,
-                           a.work, a.n[0], a.n[1], a.n[2], a.n[3], ax, coef[k][0], coef[k][1],
-                           coef[k][2], coef[k][3]);
-    }
+    int eff[4], ne = 0;  // the listed axes with lines of more than one sample
+    for (int k = 0; k < naxes; ++k)
+        if (a.n[axes[k]] > 1) eff[ne++] = k;  // a line of one sample is left as it is
     const int64_t m = a.m[0] * a.m[1] * a.m[2] * a.m[3];
-    if (m > 0)
-        hipLaunchKernelGGL(narrow_kernel<T>,
-                           dim3((unsigned)((a.m[0] + kRowBlock - 1) / kRowBlock), (unsigned)a.m[1],
-                                (unsigned)(a.m[2] * a.m[3])),
+    if (ne == 0) {
+        hipLaunchKernelGGL(widen_kernel<T>,
+                           dim3((unsigned)((a.n[0] + kRowBlock - 1) / kRowBlock), (unsigned)a.n[1],
+                                (unsigned)(a.n[2] * a.n[3])),
                            dim3(256), 0, s, a);
+        if (m > 0)
+            hipLaunchKernelGGL(narrow_kernel<T>,
+                               dim3((unsigned)((a.m[0] + kRowBlock - 1) / kRowBlock),
+                                    (unsigned)a.m[1], (unsigned)(a.m[2] * a.m[3])),
+                               dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
+    if (m == 0) return hipSuccess;
+    for (int e = 0; e < ne; ++e) {
+        const int k = eff[e], ax = axes[k];
+        const int64_t lines = total / a.n[ax];
+        const dim3 grid((unsigned)((lines + 255) / 256));
+        const double* c = coef[k];
+        const int io = (e == 0 ? kFromIn : 0) | (e == ne - 1 ? kToOut : 0);
+        launch_line<T>(io, ax == 0, grid, s, a, ax, c);
+    }
     return hipGetLastError();
 }
 

@@ -222,6 +222,12 @@ int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, dou
                                double upper, const itkg_region* largest, const itkg_view* input,
                                const itkg_view* output, itkg_arith arith, void* stream);
 
+/* The same on dense host buffers (input: the whole largest region; output:
+ * the request). */
+int itkg_canny_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma, double lower,
+                                    double upper, const itkg_region* largest, const void* input,
+                                    const itkg_region* requested, void* output, itkg_arith arith);
+
 /* ---- recursive (IIR) Gaussian ----------------------------------------- */
 
 /* yvv_coeffs, src/filters/recursive_gaussian.rs:33-47: the Young - van Vliet

@@ -43,7 +43,7 @@ EXPORTS = (
     "itkg_index_gaussian_kernel", "itkg_unsharp_mask_generate_region",
     "itkg_unsharp_mask_generate_region_host", "itkg_yvv_coefficients",
     "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
-    "itkg_canny_generate_region",
+    "itkg_canny_generate_region", "itkg_canny_generate_region_host",
 )
 
 

@@ -1286,6 +1286,46 @@ int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, dou
     return rc;
 }
 
+int itkg_canny_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma, double lower,
+                                    double upper, const itkg_region* largest, const void* input,
+                                    const itkg_region* requested, void* output, itkg_arith arith) {
+    clear_error();
+    if (ndim != 2) return fail(ITKG_ERR_UNSUPPORTED, "Canny takes 2-D images");
+    if (!args_ok(ndim, dtype, 1)) return fail(ITKG_ERR_INVALID_ARG, "bad dtype");
+    if (!largest || !requested) return fail(ITKG_ERR_INVALID_ARG, "null region");
+    if (!region_ok(2, *largest) || !region_ok(2, *requested))
+        return fail(ITKG_ERR_INVALID_ARG, "negative region size");
+    const int64_t nin = region_len(2, *largest), nout = region_len(2, *requested);
+    if (nout == 0) return ITKG_OK;
+    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
+    const size_t px = dtype_size(dtype);
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    int rc = ITKG_OK;
+    void* block = nullptr;  // input and output staging in one allocation
+    const size_t o_out = ((size_t)nin * px + 255) & ~(size_t)255;
+    e = pool_alloc(&block, o_out + (size_t)nout * px, s);
+    void* din = block;
+    void* dout = block ? static_cast<char*>(block) + o_out : nullptr;
+    if (e == hipSuccess) e = hipMemcpyAsync(din, input, (size_t)nin * px, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "Canny host staging");
+    if (!rc) {
+        const itkg_view vin = dense_view(2, din, *largest);
+        const itkg_view vout = dense_view(2, dout, *requested);
+        rc = itkg_canny_generate_region(2, dtype, sigma, lower, upper, largest, &vin, &vout, arith, s);
+    }
+    if (!rc) {
+        e = hipMemcpyAsync(output, dout, (size_t)nout * px, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy out");
+    }
+    if (block) (void)hipFreeAsync(block, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
 int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
                            const double* spacing, const itkg_region* largest,
                            const itkg_view* grad_output, const itkg_view* grad_input,

@@ -12,6 +12,7 @@
 //   MeanFilter, BoxMeanFilterN       src/filters/mean.rs:15-145
 //   BinomialBlurFilter               src/filters/binomial_blur.rs:17-90
 //   UnsharpMaskFilter                src/filters/edges.rs:376-452
+//   CannyEdgeDetectionFilter         src/filters/edges.rs:526-664
 //   RecursiveGaussian(Axis)Filter    src/filters/recursive_gaussian.rs:129-241
 //   stream_full / split_region       src/source.rs:81-97, src/region_splitter.rs:8-39
 //   VecPixel                         src/pixel.rs:61-99
@@ -458,6 +459,43 @@ struct UnsharpMaskFilter {
         check(itkg_unsharp_mask_generate_region_host((int32_t)D, PixelTraits<P>::dtype, sigma, amount,
                                                      threshold, &cb, input.data.data(), &cr,
                                                      out.data.data(), separable_arith));
+        return out;
+    }
+};
+
+// CannyEdgeDetectionFilter<S> (src/filters/edges.rs:526-664) with the
+// reference's public fields; 2-D scalar f32 / f64 pixels; the request as
+// 1 / 0.  The smoothing's arithmetic is separable_arith.
+template <class S>
+struct CannyEdgeDetectionFilter {
+    using pixel_type = typename S::pixel_type;
+    static constexpr size_t dim = S::dim;
+    using P = pixel_type;
+    static constexpr size_t D = dim;
+    static_assert(D == 2, "Canny takes 2-D images");
+    static_assert(PixelTraits<P>::ncomp == 1, "Canny takes scalar pixels");
+
+    S source;
+    double sigma;
+    double lower_threshold;
+    double upper_threshold;
+
+    CannyEdgeDetectionFilter(S src, double s, double lower, double upper)
+        : source(std::move(src)), sigma(s), lower_threshold(lower), upper_threshold(upper) {}
+
+    Region<D> largest_region() const { return source.largest_region(); }
+    std::array<double, D> spacing() const { return source.spacing(); }
+    std::array<double, D> origin() const { return source.origin(); }
+    Region<D> input_region_for_output(const Region<D>&) const { return source.largest_region(); }
+    Image<P, D> generate_region(Region<D> requested) const {
+        const Region<D> bounds = source.largest_region();
+        const Image<P, D> input = source.generate_region(bounds);
+        Image<P, D> out = Image<P, D>::allocate(requested, spacing(), origin(), P{});
+        const itkg_region cb = bounds.to_c(), cr = requested.to_c();
+        check(itkg_canny_generate_region_host((int32_t)D, PixelTraits<P>::dtype, sigma,
+                                              lower_threshold, upper_threshold, &cb,
+                                              input.data.data(), &cr, out.data.data(),
+                                              separable_arith));
         return out;
     }
 };

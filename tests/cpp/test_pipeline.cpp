@@ -234,6 +234,33 @@ static void test_unsharp_mask() {
     CHECK(out.get_pixel(Index<2>{{8, 7}}) < 0.0f, "neighbour %g", out.get_pixel(Index<2>{{8, 7}}));
 }
 
+// CannyEdgeDetectionFilter (src/filters/edges.rs:526-664): a vertical step
+// between x = 19 and 20 gives edges only in those two columns, covering a
+// whole column; a constant image gives none; a sub-request is a crop.
+static void test_canny() {
+    auto img = solid_image<2>({40, 30}, 0.0f);
+    for (int64_t y = 0; y < 30; ++y)
+        for (int64_t x = 20; x < 40; ++x) img.set_pixel(Index<2>{{x, y}}, 1.0f);
+    CannyEdgeDetectionFilter<Image<float, 2>> f(img, 1.0, 0.1, 0.3);
+    auto out = f.generate_region(f.largest_region());
+    bool full_column = false;
+    for (int64_t x = 0; x < 40; ++x) {
+        int64_t on = 0;
+        for (int64_t y = 0; y < 30; ++y) on += out.get_pixel(Index<2>{{x, y}}) == 1.0f;
+        CHECK(on == 0 || x == 19 || x == 20, "edge pixels in column %lld", (long long)x);
+        full_column = full_column || on == 30;
+    }
+    CHECK(full_column, "no whole edge column");
+    const Region<2> sub = Region<2>::make({15, 5}, {10, 12});
+    auto part = f.generate_region(sub);
+    for (int64_t y = 5; y < 17; ++y)
+        for (int64_t x = 15; x < 25; ++x)
+            CHECK(part.get_pixel(Index<2>{{x, y}}) == out.get_pixel(Index<2>{{x, y}}),
+                  "sub-request differs at %lld,%lld", (long long)x, (long long)y);
+    CannyEdgeDetectionFilter<Image<float, 2>> c(solid_image<2>({40, 30}, 3.0f), 1.0, 0.1, 0.3);
+    CHECK(sum(c.generate_region(c.largest_region()).data) == 0.0f, "edges on a constant image");
+}
+
 // src/filters/recursive_gaussian.rs:252-300
 static void test_recursive_gaussian() {
     auto imp = solid_image<1>({51}, 0.0f);
@@ -278,6 +305,7 @@ int main() {
     RUN(test_binomial_impulse_spreads);
     RUN(test_binomial_effective_sigma);
     RUN(test_unsharp_mask);
+    RUN(test_canny);
     RUN(test_recursive_gaussian);
     std::printf("all passed\n");
     return 0;

@@ -60,6 +60,7 @@ constexpr int kFromIn = 1, kToOut = 2;
 #define ITKG_REC_CHUNK 32
 #endif
 constexpr int kChunk = ITKG_REC_CHUNK;
+constexpr int kGroup = 16;  // samples per load group of a strided line
 static_assert(kChunk % 4 == 0, "chunks are whole float4 / double2 groups");
 
 template <typename T>
@@ -155,11 +156,36 @@ __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, con
         for (int j = 0; j < kChunk; ++j) c[j] = step(c[j]);
         store_chunk(p + i, c);
     }
-    for (int64_t i = body; i < len; ++i) p[i * stride] = step(src(i));
+    // Samples outside the chunks (every sample of a strided line) go in
+    // groups of kGroup: the group's loads are issued together, ahead of its
+    // steps and stores, so each wave keeps kGroup loads in flight (one at a
+    // time would leave HBM latency-bound at this occupancy).
+    int64_t i = body;
+    for (; i + kGroup <= len; i += kGroup) {
+        double c[kGroup];
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) c[j] = src(i + j);
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) c[j] = step(c[j]);
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) p[(i + j) * stride] = c[j];
+    }
+    for (; i < len; ++i) p[i * stride] = step(src(i));
     init = ym1;  // the line's last value after the causal pass
     ym1 = ym2 = ym3 = init;
-    // backwards: the samples past the last whole chunk first, then the chunks
-    for (int64_t i = len - 1; i >= body; --i) put(i, step(p[i * stride]));
+    // backwards: the samples past the last whole chunk first (groups from
+    // the top, then single samples), then the chunks
+    i = len;
+    for (; i - kGroup >= body; i -= kGroup) {
+        double c[kGroup];
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) c[j] = p[(i - kGroup + j) * stride];
+#pragma unroll
+        for (int j = kGroup - 1; j >= 0; --j) c[j] = step(c[j]);
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) put(i - kGroup + j, c[j]);
+    }
+    for (; i > body; --i) put(i - 1, step(p[(i - 1) * stride]));
     if constexpr (ROWS)
     for (int64_t i = body - kChunk; i >= 0; i -= kChunk) {
         double c[kChunk];

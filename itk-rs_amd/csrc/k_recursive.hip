@@ -147,14 +147,26 @@ __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, con
     bool rows = ROWS && stride == 1 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
     if constexpr (from_in) rows = rows && is == 1 && (reinterpret_cast<uintptr_t>(q) & 15) == 0;
     const int64_t body = rows ? len / kChunk * kChunk : 0;
-    if constexpr (ROWS)
-    for (int64_t i = 0; i < body; i += kChunk) {
-        double c[kChunk];
-        if constexpr (from_in) load_chunk(q + i, c);
-        else load_chunk(p + i, c);
+    // Each chunk's successor is loaded before the chunk's steps (two chunks
+    // of loads in flight per wave).
+    if constexpr (ROWS) {
+        double nxt[kChunk];
+        if (body > 0) {
+            if constexpr (from_in) load_chunk(q, nxt);
+            else load_chunk(p, nxt);
+        }
+        for (int64_t i = 0; i < body; i += kChunk) {
+            double c[kChunk];
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) c[j] = step(c[j]);
-        store_chunk(p + i, c);
+            for (int j = 0; j < kChunk; ++j) c[j] = nxt[j];
+            if (i + kChunk < body) {
+                if constexpr (from_in) load_chunk(q + i + kChunk, nxt);
+                else load_chunk(p + i + kChunk, nxt);
+            }
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) c[j] = step(c[j]);
+            store_chunk(p + i, c);
+        }
     }
     // Samples outside the chunks (every sample of a strided line) go in
     // groups of kGroup: the group's loads are issued together, ahead of its
@@ -186,17 +198,22 @@ __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, con
         for (int j = 0; j < kGroup; ++j) put(i - kGroup + j, c[j]);
     }
     for (; i > body; --i) put(i - 1, step(p[(i - 1) * stride]));
-    if constexpr (ROWS)
-    for (int64_t i = body - kChunk; i >= 0; i -= kChunk) {
-        double c[kChunk];
-        load_chunk(p + i, c);
+    if constexpr (ROWS) {
+        double nxt[kChunk];
+        if (body > 0) load_chunk(p + body - kChunk, nxt);
+        for (int64_t i = body - kChunk; i >= 0; i -= kChunk) {
+            double c[kChunk];
 #pragma unroll
-        for (int j = kChunk - 1; j >= 0; --j) c[j] = step(c[j]);
-        if constexpr (to_out) {
+            for (int j = 0; j < kChunk; ++j) c[j] = nxt[j];
+            if (i - kChunk >= 0) load_chunk(p + i - kChunk, nxt);
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) put(i + j, c[j]);
-        } else {
-            store_chunk(p + i, c);
+            for (int j = kChunk - 1; j >= 0; --j) c[j] = step(c[j]);
+            if constexpr (to_out) {
+#pragma unroll
+                for (int j = 0; j < kChunk; ++j) put(i + j, c[j]);
+            } else {
+                store_chunk(p + i, c);
+            }
         }
     }
 }

@@ -172,15 +172,26 @@ __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, con
     // groups of kGroup: the group's loads are issued together, ahead of its
     // steps and stores, so each wave keeps kGroup loads in flight (one at a
     // time would leave HBM latency-bound at this occupancy).
+    // The next group is loaded before the current group's steps (two groups
+    // in flight); it holds samples this pass has not written yet.
     int64_t i = body;
-    for (; i + kGroup <= len; i += kGroup) {
-        double c[kGroup];
+    if (i + kGroup <= len) {
+        double nxt[kGroup];
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) c[j] = src(i + j);
+        for (int j = 0; j < kGroup; ++j) nxt[j] = src(i + j);
+        for (; i + kGroup <= len; i += kGroup) {
+            double c[kGroup];
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) c[j] = step(c[j]);
+            for (int j = 0; j < kGroup; ++j) c[j] = nxt[j];
+            if (i + 2 * kGroup <= len) {
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) p[(i + j) * stride] = c[j];
+                for (int j = 0; j < kGroup; ++j) nxt[j] = src(i + kGroup + j);
+            }
+#pragma unroll
+            for (int j = 0; j < kGroup; ++j) c[j] = step(c[j]);
+#pragma unroll
+            for (int j = 0; j < kGroup; ++j) p[(i + j) * stride] = c[j];
+        }
     }
     for (; i < len; ++i) p[i * stride] = step(src(i));
     init = ym1;  // the line's last value after the causal pass
@@ -188,14 +199,23 @@ __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, con
     // backwards: the samples past the last whole chunk first (groups from
     // the top, then single samples), then the chunks
     i = len;
-    for (; i - kGroup >= body; i -= kGroup) {
-        double c[kGroup];
+    if (i - kGroup >= body) {
+        double nxt[kGroup];
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) c[j] = p[(i - kGroup + j) * stride];
+        for (int j = 0; j < kGroup; ++j) nxt[j] = p[(i - kGroup + j) * stride];
+        for (; i - kGroup >= body; i -= kGroup) {
+            double c[kGroup];
 #pragma unroll
-        for (int j = kGroup - 1; j >= 0; --j) c[j] = step(c[j]);
+            for (int j = 0; j < kGroup; ++j) c[j] = nxt[j];
+            if (i - 2 * kGroup >= body) {
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) put(i - kGroup + j, c[j]);
+                for (int j = 0; j < kGroup; ++j) nxt[j] = p[(i - 2 * kGroup + j) * stride];
+            }
+#pragma unroll
+            for (int j = kGroup - 1; j >= 0; --j) c[j] = step(c[j]);
+#pragma unroll
+            for (int j = 0; j < kGroup; ++j) put(i - kGroup + j, c[j]);
+        }
     }
     for (; i > body; --i) put(i - 1, step(p[(i - 1) * stride]));
     if constexpr (ROWS) {

@@ -27,9 +27,10 @@ write) / the average kernel duration from HIP events on the launch stream,
 against 8 TB/s.  cpu_baseline: the oracle's threaded restatement of the
 reference's loop structure on the host cores, over the same whole volume.
 
-The default warmup (200 launches, about 55 ms of sustained load) brings the
-GPU to its steady clocks: after 3 warmup launches the kernel still runs about
-10% slower, and 1000 timed launches after the warmup average the same as 100.
+Before the W warmup steps the headline launch runs untimed for about 200 ms
+of GPU time (`settle`): the GPU needs tens of milliseconds of sustained load to
+reach its steady clocks (after 3 launches the kernel still runs about 12%
+slower), and 1000 timed launches after the settle average the same as 100.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 200] [--arith fast|exact]
 """
@@ -58,6 +59,9 @@ def parse():
                     help="comma list of extras groups to run (default all): "
                          "c2, c5, c4, c5_batch, c1_c3, filters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--keep-gc", action="store_true",
+                    help="leave Python's garbage collector on in the timed regions and report "
+                         "its pauses (gc_pauses_ms)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprof-derived HBM bytes per launch, if measured")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -66,31 +70,89 @@ def parse():
     return ap.parse_args()
 
 
+def settle(fn, stream, gpu_ms=200.0, max_calls=4000):
+    """Runs fn untimed until about `gpu_ms` of GPU time has passed (the
+    clocks ramp up under sustained load)."""
+    import torch
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    fn()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    one = max(t0.elapsed_time(t1), 1e-3)
+    for _ in range(min(max_calls, int(gpu_ms / one))):
+        fn()
+    torch.cuda.synchronize()
+
+
+def max_rel_err(got, ref64):
+    """max |got - ref| / max |ref| (f64 reference)."""
+    return float((got.double() - ref64).abs().max()) / max(float(ref64.abs().max()), 1e-300)
+
+
+def host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+GC_PAUSES_MS = []  # Python GC pauses inside timed regions (recorded with --keep-gc)
+KEEP_GC = False
+
+
+def _gc_callback(phase, info, _t={}):
+    if phase == "start":
+        _t["s"] = time.perf_counter()
+    elif "s" in _t:
+        GC_PAUSES_MS.append((time.perf_counter() - _t.pop("s")) * 1e3)
+
+
 def timed_loop(fn, steps, warmup, stream, flush=None):
     """Runs fn `warmup` times, then `steps` timed times; returns (total wall
-    seconds between synchronisations, per-launch event milliseconds)."""
+    seconds between synchronisations, per-launch event milliseconds).
+
+    Python's cyclic garbage collector is off inside the timed region (as in
+    the standard library's timeit): a full collection of this process takes
+    tens of milliseconds, and one that lands between the start event and a
+    rep's launches leaves the GPU idle for that long.  --keep-gc leaves it on
+    and records each collection's pause."""
+    import gc
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    evs = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        if flush is not None:
-            flush()
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        fn()
-        b.record(stream)
-        evs.append((a, b))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    gc.collect()
+    if not KEEP_GC:
+        gc.disable()
+    elif _gc_callback not in gc.callbacks:
+        gc.callbacks.append(_gc_callback)
+    try:
+        evs = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if flush is not None:
+                flush()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    finally:
+        gc.enable()
+        if _gc_callback in gc.callbacks:
+            gc.callbacks.remove(_gc_callback)
     return wall, [a.elapsed_time(b) for a, b in evs]
 
 
 def main():
     args = parse()
+    global KEEP_GC
+    KEEP_GC = args.keep_gc
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
                                    "of it (--gpus must be 1)"}))
@@ -119,6 +181,7 @@ def main():
     if args.copy_calibration:
         out.copy_(x)  # 512 MiB read + 512 MiB written by one copy kernel
         torch.cuda.synchronize()
+    settle(step, stream)
     wall, ms = timed_loop(step, args.steps, args.warmup, stream)
     ms_avg = float(np.mean(ms))
     gvox = n * args.steps / wall / 1e9
@@ -216,8 +279,22 @@ def main():
             extras["c5_fwd_adjoint_256"] = {"fwd_gvox_s": round(fwd_g, 3),
                                             "fwd_plus_adj_gvox_s": round(fa_g, 3),
                                             "ratio": round(fa_g / fwd_g, 3)}
+            # error of the timed f32 forward and adjoint against f64 evaluations
+            ref_f = itk.gaussian(x5.double(), s5, sp5)
+            ref_a = itk.gaussian_backward(g5.double(), s5, sp5)
+            extras["c5_max_rel_err_vs_f64"] = {"forward": max_rel_err(y5, ref_f),
+                                               "adjoint": max_rel_err(gi5, ref_a)}
+            del ref_f, ref_a
             del x5, g5, y5, gi5, im5, om5, go5, gin5, flushbuf
             torch.cuda.empty_cache()
+
+        def rep_stats(wall, ms, reps, voxels):
+            """Gvox/s from the wall clock and from the median rep, with the
+            mean and median rep times (a slow outlier shows as mean >> median)."""
+            return {"gvox_s": round(voxels * reps / wall / 1e9, 3),
+                    "gvox_s_median": round(voxels / (float(np.median(ms)) * 1e-3) / 1e9, 3),
+                    "ms_mean": round(float(np.mean(ms)), 4),
+                    "ms_median": round(float(np.median(ms)), 4)}
 
         if want("c4"):
             # batched: 64 uneven volumes vs a loop of plain launches
@@ -227,7 +304,7 @@ def main():
                                  direction=itk.BATCH_FORWARD, arith=arith)
             nb = plan.voxels
             plan.autotune(stream)  # z run per radius group, measured on these volumes
-            wb, _ = timed_loop(lambda: plan.run(stream), 5, 2, stream)
+            wb, msb4 = timed_loop(lambda: plan.run(stream), 10, 2, stream)
             imgs = [itk.Image(itk.Region([0, 0, 0], list(reversed(v.shape))), p[2], (0.0,) * 3, v)
                     for v, p in zip(vols, params)]
             oims = [itk.Image(i.region, i.spacing, i.origin, o) for i, o in zip(imgs, outs)]
@@ -238,10 +315,19 @@ def main():
             def loop():
                 for q, i, o in zip(fl, imgs, oims):
                     q.generate_into(i, o)
-            wl_, _ = timed_loop(loop, 5, 2, stream)
-            extras["c4_batched_64"] = {"voxels": nb, "batched_gvox_s": round(nb * 5 / wb / 1e9, 3),
-                                       "loop_gvox_s": round(nb * 5 / wl_ / 1e9, 3),
-                                       "speedup": round(wl_ / wb, 3)}
+            wl_, msl4 = timed_loop(loop, 10, 2, stream)
+            bs, ls = rep_stats(wb, msb4, 10, nb), rep_stats(wl_, msl4, 10, nb)
+            extras["c4_batched_64"] = {"voxels": nb, "batched_gvox_s": bs["gvox_s"],
+                                       "loop_gvox_s": ls["gvox_s"],
+                                       "speedup": round(wl_ / wb, 3),
+                                       "speedup_median": round(ls["ms_median"] / bs["ms_median"], 3),
+                                       "batched": bs, "loop": ls}
+            # error of the batched outputs against f64 evaluations, max over volumes
+            plan.run(stream)
+            err = 0.0
+            for v, o, p in zip(vols, outs, params):
+                err = max(err, max_rel_err(o, itk.gaussian(v.double(), p[1], p[2])))
+            extras["c4_max_rel_err_vs_f64"] = err
             plan.close()
             del vols, outs, imgs, oims, fl
             torch.cuda.empty_cache()
@@ -254,16 +340,24 @@ def main():
                                   direction=itk.BATCH_BACKWARD)
             ng = gplan.voxels
             gplan.autotune(stream)
-            wgb, _ = timed_loop(lambda: gplan.run(stream), 10, 2, stream)
+            wgb, msgb = timed_loop(lambda: gplan.run(stream), 10, 2, stream)
 
             def gloop():
                 for q, o, p in zip(grads, gouts, gparams):
                     itk.gaussian_backward(q, p[1], p[2], out=o)
-            wgl, _ = timed_loop(gloop, 10, 2, stream)
-            extras["c5_batched_adjoint_16"] = {"voxels": ng,
-                                               "batched_gvox_s": round(ng * 10 / wgb / 1e9, 3),
-                                               "loop_gvox_s": round(ng * 10 / wgl / 1e9, 3),
-                                               "speedup": round(wgl / wgb, 3)}
+            wgl, msgl = timed_loop(gloop, 10, 2, stream)
+            bs, ls = rep_stats(wgb, msgb, 10, ng), rep_stats(wgl, msgl, 10, ng)
+            extras["c5_batched_adjoint_16"] = {"voxels": ng, "batched_gvox_s": bs["gvox_s"],
+                                               "loop_gvox_s": ls["gvox_s"],
+                                               "speedup": round(wgl / wgb, 3),
+                                               "speedup_median": round(ls["ms_median"] /
+                                                                       bs["ms_median"], 3),
+                                               "batched": bs, "loop": ls}
+            gplan.run(stream)
+            err = 0.0
+            for q, o, p in zip(grads, gouts, gparams):
+                err = max(err, max_rel_err(o, itk.gaussian_backward(q.double(), p[1], p[2])))
+            extras["c5_batch_max_rel_err_vs_f64"] = err
             gplan.close()
             del grads, gouts
             torch.cuda.empty_cache()
@@ -271,16 +365,29 @@ def main():
         if want("c1_c3"):
             # 512^2 2-D and the VecPixel configuration
             x1, s1, sp1 = wl.c1(dev)
-            w1, _ = timed_loop(lambda: itk.gaussian(x1, s1, sp1, arith=arith), 20, 3, stream)
-            extras["c1_512sq"] = {"gvox_s": round(x1.numel() * 20 / w1 / 1e9, 3)}
+            w1, ms1 = timed_loop(lambda: itk.gaussian(x1, s1, sp1, arith=arith), 20, 3, stream)
+            extras["c1_512sq"] = {"gvox_s": round(x1.numel() * 20 / w1 / 1e9, 3),
+                                  "ms_median": round(float(np.median(ms1)), 4)}
+            extras["c1_max_rel_err_vs_f64"] = max_rel_err(itk.gaussian(x1, s1, sp1, arith=arith),
+                                                          itk.gaussian(x1.double(), s1, sp1))
             v3, s3, sp3 = wl.c3(dev)
-            w3, ms3 = timed_loop(lambda: itk.gaussian(v3, s3, sp3, ndim=3, arith=arith), 10, 2,
-                                 stream)
+            o3 = torch.empty_like(v3)
+            i3 = itk.Image(itk.Region([0, 0, 0], [256, 256, 160]), sp3, (0.0,) * 3, v3)
+            oi3 = itk.Image(i3.region, sp3, (0.0,) * 3, o3)
+            f3 = itk.GaussianFilter(i3, s3)
+            f3.arith = arith
+            # 252 MB in + out fits in the 256 MB MALL: flush it before every rep
+            flushbuf = torch.empty(512 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+            w3, ms3 = timed_loop(lambda: f3.generate_into(i3, oi3), 10, 2, stream,
+                                 lambda: flushbuf.fill_(1.0))
             n3 = v3.numel() // 3
-            extras["c3_vec3_256x256x160"] = {"gvox_s": round(n3 * 10 / w3 / 1e9, 3),
+            extras["c3_vec3_256x256x160"] = {"gvox_s": round(n3 / (np.mean(ms3) * 1e-3) / 1e9, 3),
+                                             "kernel_ms": round(float(np.mean(ms3)), 4),
                                              "hbm_frac_24B": round(24.0 * n3 / (np.mean(ms3) * 1e-3)
                                                                    / 1e9 / HBM_PEAK_GBS, 4)}
-            del v3
+            extras["c3_max_rel_err_vs_f64"] = max_rel_err(o3, itk.gaussian(v3.double(), s3, sp3,
+                                                                           ndim=3))
+            del v3, o3, i3, oi3, flushbuf
             torch.cuda.empty_cache()
 
         if want("filters"):
@@ -343,13 +450,17 @@ def main():
             del xc, oc, ic, disc, xx, yy
             torch.cuda.empty_cache()
 
+        if KEEP_GC:
+            extras["gc_pauses_ms"] = {"count": len(GC_PAUSES_MS),
+                                      "max": round(max(GC_PAUSES_MS, default=0.0), 3),
+                                      "total": round(sum(GC_PAUSES_MS), 3)}
         result["extras"] = extras
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         host = x.cpu().numpy()
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         times = []
         for _ in range(3):
             t0 = time.perf_counter()

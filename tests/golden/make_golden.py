@@ -1,13 +1,31 @@
 """Writes tests/golden/golden.json: oracle outputs of the full-size
 configurations, as hashes and sampled voxels, so that GPU tests can check the
-512^3 result without running the oracle on the GPU machine.
+full-size results without running the oracle on the GPU machine.
 
-The inputs come from itk-rs_amd/workloads.py (seeded, identical bits on any
-device); the outputs from the oracle's threaded restatement
+Configurations (inputs from itk-rs_amd/workloads.py, generated on the host):
+  c1    Image<f32,2> 512^2, sigma 1.5: output hash
+  c2    Image<f32,3> 512^3, sigma 2: output hash, 2048 sampled voxels
+  c3    Image<VecPixel<f32,3>,3> 256x256x160, spacing 0.7/0.7/2.5, sigma 3:
+        output hash, 2048 sampled (voxel, component) values
+  c4    the 64 batched volumes (extents 64..320, per-volume sigma and
+        spacing): per volume the input and output hashes and 64 sampled
+        voxels
+  c5    Image<f32,3> 256^3, sigma 1.8: forward output hash and 2048 sampled
+        voxels; the f64 adjoint of the N(0,1) upstream gradient at 2048
+        sampled voxels with its max |value|
+  c5b   the 16 batched adjoint gradients (extents 48..192, sigma 1.8): per
+        volume the input hash, the f64 adjoint's max |value| and 256
+        sampled voxels
+
+Forward outputs come from the oracle's threaded restatement
 (oracle/gauss_oracle.c, orc_ref_faithful_f32), which tests/test_oracle.py
-pins as bit-identical to the plain restatement.
+pins as bit-identical to the plain restatement; adjoints from the oracle's
+f64 transpose by scattered taps (orc_adjoint_f64), which tests/test_oracle.py
+pins against the dense transpose.  Inputs are hashed too, so a drift of the
+input generator shows up as its own failure.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [key ...]    (default: every key;
+                                                     others are kept)
 """
 import hashlib
 import json
@@ -24,42 +42,122 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import oracle  # noqa: E402
 from __graft_entry__ import load_package  # noqa: E402
 
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.json")
+
 
 def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def main():
+def sample(rng, n: int, k: int) -> np.ndarray:
+    return np.sort(rng.choice(n, min(k, n), replace=False))
+
+
+def f32_samples(y: np.ndarray, idx: np.ndarray) -> dict:
+    return {"sample_flat_index": idx.tolist(),
+            "sample_value_bits": y.reshape(-1)[idx].view(np.uint32).tolist()}
+
+
+def f64_samples(y: np.ndarray, idx: np.ndarray) -> dict:
+    # float repr round-trips f64 exactly through JSON
+    return {"sample_flat_index": idx.tolist(),
+            "sample_value": [float(v) for v in y.reshape(-1)[idx]],
+            "max_abs": float(np.abs(y).max())}
+
+
+def whole(shape_zyx):
+    return [0] * len(shape_zyx), list(reversed(shape_zyx))
+
+
+def make_c1(wl, threads):
+    x, sigma, sp = wl.c1("cpu")
+    a = x.numpy()
+    y = oracle.generate_region(a, [0, 0], sp, sigma, [0, 0], [512, 512])
+    return {"input_sha256": sha(a), "output_sha256": sha(y), "sigma": sigma}
+
+
+def make_c2(wl, threads):
+    x, sigma, sp = wl.c2("cpu")
+    a = x.numpy()
+    y = oracle.ref_faithful(a, [0, 0, 0], sp, sigma, [0, 0, 0], [512, 512, 512], 1, threads)
+    rng = np.random.default_rng(2024)
+    idx = sample(rng, y.size, 2048)
+    return dict({"input_sha256": sha(a), "output_sha256": sha(y), "sigma": sigma},
+                **f32_samples(y, idx))
+
+
+def make_c3(wl, threads):
+    v, sigma, sp = wl.c3("cpu")
+    a = v.numpy()
+    y = oracle.ref_faithful(a, [0, 0, 0], sp, sigma, [0, 0, 0], [256, 256, 160], 3, threads)
+    rng = np.random.default_rng(2025)
+    idx = sample(rng, y.size, 2048)
+    return dict({"input_sha256": sha(a), "output_sha256": sha(y), "sigma": sigma,
+                 "spacing": list(sp)}, **f32_samples(y, idx))
+
+
+def make_c4(wl, threads):
+    vols, params = wl.c4("cpu")
+    rng = np.random.default_rng(2026)
+    out = []
+    for v, (shape, sigma, sp) in zip(vols, params):
+        a = v.numpy()
+        idx0, size = whole(shape)
+        y = oracle.ref_faithful(a, idx0, sp, sigma, idx0, size, 1, threads)
+        radius = oracle.kernel_radius(sigma, sp[0])
+        out.append(dict({"shape_zyx": list(shape), "sigma": sigma, "spacing": sp[0],
+                         "radius": radius, "input_sha256": sha(a), "output_sha256": sha(y)},
+                        **f32_samples(y, sample(rng, y.size, 64))))
+    return {"volumes": out}
+
+
+def make_c5(wl, threads):
+    x, g, sigma, sp = wl.c5("cpu")
+    a, ga = x.numpy(), g.numpy()
+    idx0, size = whole(a.shape)
+    y = oracle.ref_faithful(a, idx0, sp, sigma, idx0, size, 1, threads)
+    gin, ii, isz = oracle.adjoint_f64(ga, idx0, size, sp, sigma, idx0, size)
+    assert ii == idx0 and isz == size
+    rng = np.random.default_rng(2027)
+    fwd = dict({"output_sha256": sha(y)}, **f32_samples(y, sample(rng, y.size, 2048)))
+    adj = f64_samples(gin, sample(rng, gin.size, 2048))
+    return {"input_sha256": sha(a), "grad_sha256": sha(ga), "sigma": sigma,
+            "forward": fwd, "adjoint_f64": adj}
+
+
+def make_c5b(wl, threads):
+    grads, params = wl.c5_batch("cpu")
+    rng = np.random.default_rng(2028)
+    out = []
+    for g, (shape, sigma, sp) in zip(grads, params):
+        ga = g.numpy()
+        idx0, size = whole(shape)
+        gin, _, _ = oracle.adjoint_f64(ga, idx0, size, sp, sigma, idx0, size)
+        out.append(dict({"shape_zyx": list(shape), "sigma": sigma, "grad_sha256": sha(ga)},
+                        **f64_samples(gin, sample(rng, gin.size, 256))))
+    return {"volumes": out}
+
+
+MAKERS = {"c1": make_c1, "c2": make_c2, "c3": make_c3, "c4": make_c4, "c5": make_c5,
+          "c5b": make_c5b}
+
+
+def main(keys):
     load_package()
     from itk_rs_amd import workloads as wl
 
     out = {}
+    if os.path.exists(PATH):
+        with open(PATH) as fh:
+            out = json.load(fh)
     threads = os.cpu_count() or 1
-
-    t = time.time()
-    x, sigma, sp = wl.c1("cpu")
-    a = x.numpy()
-    y = oracle.generate_region(a, [0, 0], sp, sigma, [0, 0], [512, 512])
-    out["c1"] = {"input_sha256": sha(a), "output_sha256": sha(y), "sigma": sigma}
-    print(f"c1 done {time.time() - t:.1f}s", flush=True)
-
-    t = time.time()
-    x, sigma, sp = wl.c2("cpu")
-    a = x.numpy()
-    print(f"c2 input {time.time() - t:.1f}s", flush=True)
-    y = oracle.ref_faithful(a, [0, 0, 0], sp, sigma, [0, 0, 0], [512, 512, 512], 1, threads)
-    rng = np.random.default_rng(2024)
-    idx = np.sort(rng.choice(y.size, 2048, replace=False))
-    out["c2"] = {
-        "input_sha256": sha(a), "output_sha256": sha(y), "sigma": sigma,
-        "sample_flat_index": idx.tolist(),
-        "sample_value_bits": y.reshape(-1)[idx].view(np.uint32).tolist(),
-    }
-    print(f"c2 done {time.time() - t:.1f}s", flush=True)
-
-    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden.json"), "w") as fh:
+    for k in keys or list(MAKERS):
+        t = time.time()
+        out[k] = MAKERS[k](wl, threads)
+        print(f"{k} done {time.time() - t:.1f}s", flush=True)
+    with open(PATH, "w") as fh:
         json.dump(out, fh)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

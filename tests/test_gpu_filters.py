@@ -231,6 +231,17 @@ def test_unsharp_streamed_and_vector_refused(itk, dev):
     ((200,), (1.0,), 0.3, -1, ((20,), (100,)), np.float32),
     ((30, 20, 10), (1.0, 1.0, 1.0), 2.5, 0, ((4, 4, 2), (20, 10, 6)), np.float64),
     ((1, 40, 12), (1.0, 1.0, 1.0), 1.0, -1, None, np.float32),   # an axis of one sample
+    # axis-0 rows of 32-sample chunks followed by a 16..31-sample tail, odd
+    # lengths (lanes of one wave on different paths), f64 chunked reads of
+    # the input, sub-requests; all axes and axis 0 alone
+    ((50, 33, 17), (1.0, 1.0, 1.0), 2.0, -1, None, np.float32),
+    ((50, 33, 17), (1.0, 1.0, 1.0), 2.0, 0, None, np.float32),
+    ((83, 37, 21), (1.0, 0.8, 1.3), 1.7, -1, ((3, 2, 1), (77, 33, 19)), np.float32),
+    ((83, 37, 21), (1.0, 1.0, 1.0), 2.4, 0, ((5, 0, 0), (61, 37, 21)), np.float32),
+    ((81, 40, 9), (1.0, 1.0, 1.0), 1.3, -1, ((7, 3, 2), (70, 31, 5)), np.float64),
+    ((81, 40, 9), (1.0, 1.0, 1.0), 2.2, 0, None, np.float64),
+    ((113, 7), (1.0, 1.0), 2.6, -1, None, np.float32),
+    ((95, 5, 3), (1.0, 1.0, 1.0), 1.1, 2, ((0, 1, 0), (95, 3, 3)), np.float32),
 ])
 def test_recursive_matches_oracle(itk, orc, dev, size, spacing, sigma, axis, req, dtype):
     D = len(size)

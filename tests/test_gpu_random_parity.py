@@ -55,6 +55,32 @@ def test_random_gaussian_parity(itk, orc, dev, seed):
             itk.set_path(itk.PATH_AUTO)
 
 
+@pytest.mark.parametrize("seed", range(16))
+def test_random_recursive_parity(itk, orc, dev, seed):
+    """RecursiveGaussianFilter / RecursiveGaussianAxisFilter on random shapes
+    (odd extents, rows with and without a chunk tail), spacings, sigmas,
+    axes, sub-requests and dtypes, bit for bit against the oracle."""
+    rng = np.random.default_rng(7000 + seed)
+    D = int(rng.integers(1, 4))
+    size = [int(rng.integers(1, 120 if D < 3 else 60)) for _ in range(D)]
+    spacing = [float(rng.choice([0.6, 1.0, 1.0, 1.7])) for _ in range(D)]
+    sigma = float(rng.choice([0.3, 0.9, 1.6, 2.4, 4.1]))
+    axis = int(rng.integers(-1, D))
+    dtype = np.float64 if rng.random() < 0.3 else np.float32
+    lo = [int(rng.integers(0, s)) for s in size]
+    reg = itk.Region(lo, [int(rng.integers(1, s - l + 1)) for s, l in zip(size, lo)])
+    if rng.random() < 0.3:
+        reg = itk.Region([0] * D, list(size))
+    x = rng.random(tuple(reversed(size))).astype(dtype)
+    src = itk.Image(itk.Region([0] * D, size), tuple(spacing), (0.0,) * D, torch.from_numpy(x).to(dev))
+    f = itk.RecursiveGaussianFilter(src, sigma) if axis < 0 else \
+        itk.RecursiveGaussianAxisFilter(src, sigma, axis)
+    got = f.generate_region(reg).data.cpu().numpy()
+    ref = orc.recursive_generate_region(x, [0] * D, spacing, sigma, axis, list(reg.index),
+                                        list(reg.size))
+    assert np.array_equal(got, ref), (seed, D, size, spacing, sigma, axis, dtype, reg)
+
+
 @pytest.mark.parametrize("seed", range(20))
 def test_random_filters_parity(itk, orc, dev, seed):
     D, size, _, _, req, ncomp, dtype = _case(seed + 500)

@@ -123,6 +123,21 @@ int itkg_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
                                   const itkg_view* input, const itkg_view* output,
                                   itkg_arith arith, void* stream);
 
+/* No counterpart in the reference: itkg_gaussian_generate_region for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results, bit for bit; it calls no hipMalloc, hipFree or allocating library
+ * routine.  The fused kernel needs no scratch; the per-axis path takes its two
+ * intermediates from a static device array of 64 MiB per GPU (held until the
+ * process ends), and a call whose intermediates would exceed it fails with
+ * ITKG_ERR_UNSUPPORTED.  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU race (serialise them, e.g. on one stream). */
+int itkg_gaussian_generate_region_noalloc(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                          double sigma, const double* spacing,
+                                          const itkg_region* largest, const itkg_view* input,
+                                          const itkg_view* output, itkg_arith arith,
+                                          void* stream);
+
 /* Same contract on dense HOST buffers (what an ImageSource whose Image owns
  * a host Vec<P> hands over): `input` holds input_region_for_output(requested)
  * densely, `output` receives `requested` densely.  Returns when the output is

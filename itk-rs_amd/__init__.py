@@ -34,7 +34,8 @@ BATCH_FORWARD, BATCH_BACKWARD = 0, 1
 # Every symbol include/itkg.h declares (checked by the test suite).
 EXPORTS = (
     "itkg_kernel_radius", "itkg_build_kernel", "itkg_input_region_for_output",
-    "itkg_split_region", "itkg_gaussian_generate_region", "itkg_gaussian_generate_region_host",
+    "itkg_split_region", "itkg_gaussian_generate_region", "itkg_gaussian_generate_region_noalloc",
+    "itkg_gaussian_generate_region_host",
     "itkg_gaussian_backward", "itkg_batch_create", "itkg_batch_run", "itkg_batch_autotune",
     "itkg_batch_voxels",
     "itkg_batch_destroy", "itkg_last_error", "itkg_set_path", "itkg_version",
@@ -85,6 +86,7 @@ def _load():
     L.itkg_gaussian_generate_region.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
         P(c_region), P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
+    L.itkg_gaussian_generate_region_noalloc.argtypes = L.itkg_gaussian_generate_region.argtypes
     L.itkg_gaussian_generate_region_host.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
         P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int]

@@ -781,6 +781,45 @@ bool args_ok(int ndim, itkg_dtype dt, int ncomp) {
 
 }  // namespace itkg
 
+// Scratch of itkg_gaussian_generate_region_noalloc: the per-axis path's two
+// ping-pong intermediates, in a file-scope device array instead of a pool
+// allocation.  Each GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` names a host-side stand-in, not
+// device memory: device code only ever receives the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch).
+// Every pass writes its whole output region before the next pass reads it,
+// so nothing has to start at zero; calls that overlap in time on one GPU
+// share the array and race.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace itkg {
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int noalloc_scratch(void** out) {
+    static std::mutex mu;
+    static void* addr[kPoolDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kPoolDevices) return fail(ITKG_ERR_UNSUPPORTED, "device %d", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(ITKG_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return ITKG_OK;
+}
+
+}  // namespace
+}  // namespace itkg
+
 using namespace itkg;
 
 extern "C" {
@@ -859,6 +898,34 @@ int itkg_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
     if ((rc = check_view(ndim, output, p.req, "output"))) return rc;
     return forward_impl(p, dtype, ncomp, *input, *output, arith,
                         static_cast<hipStream_t>(stream), nullptr);
+}
+
+int itkg_gaussian_generate_region_noalloc(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
+                                          double sigma, const double* spacing,
+                                          const itkg_region* largest, const itkg_view* input,
+                                          const itkg_view* output, itkg_arith arith,
+                                          void* stream) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (!output) return fail(ITKG_ERR_INVALID_ARG, "null output view");
+    Plan p;
+    int rc = make_plan(ndim, sigma, spacing, largest, output->region, &p);
+    if (rc) return rc;
+    if ((rc = check_view(ndim, input, p.in_reg, "input"))) return rc;
+    if ((rc = check_view(ndim, output, p.req, "output"))) return rc;
+    // the fused kernel and 1-D calls take no scratch; the per-axis path takes
+    // its two intermediates from the static array
+    const bool fused = use_fused(p, dtype, ncomp) && fused_src_ok(p, *input, ncomp, false);
+    const size_t need = (fused || ndim == 1) ? 0 : scratch_bytes(p, dtype, ncomp);
+    if (need > kNoAllocScratchBytes)
+        return fail(ITKG_ERR_UNSUPPORTED,
+                    "this call needs %zu bytes of per-axis scratch; the allocation-free entry "
+                    "point holds %zu",
+                    need, (size_t)kNoAllocScratchBytes);
+    void* scr = nullptr;
+    if ((rc = noalloc_scratch(&scr))) return rc;
+    return forward_impl(p, dtype, ncomp, *input, *output, arith, static_cast<hipStream_t>(stream),
+                        scr);
 }
 
 int itkg_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp,

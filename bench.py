@@ -28,9 +28,11 @@ against 8 TB/s.  cpu_baseline: the oracle's threaded restatement of the
 reference's loop structure on the host cores, over the same whole volume.
 
 Before the W warmup steps the headline launch runs untimed for about 200 ms
-of GPU time (`settle`): the GPU needs tens of milliseconds of sustained load to
-reach its steady clocks (after 3 launches the kernel still runs about 12%
-slower), and 1000 timed launches after the settle average the same as 100.
+of GPU time (`settle`; 50 ms before each extras leg): the GPU needs tens of
+milliseconds of sustained load to reach its steady clocks (after 3 launches
+the kernel still runs about 12% slower, and a gap of tens of milliseconds
+drops them again), and 1000 timed launches after the settle average the same
+as 100.
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 200] [--arith fast|exact]
 """
@@ -74,16 +76,19 @@ def settle(fn, stream, gpu_ms=200.0, max_calls=4000):
     """Runs fn untimed until about `gpu_ms` of GPU time has passed (the
     clocks ramp up under sustained load)."""
     import torch
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record(stream)
-    fn()
-    t1.record(stream)
-    torch.cuda.synchronize()
-    one = max(t0.elapsed_time(t1), 1e-3)
-    for _ in range(min(max_calls, int(gpu_ms / one))):
-        fn()
-    torch.cuda.synchronize()
+    done, calls = 0.0, 0
+    while done < gpu_ms and calls < max_calls:
+        # chunks of 20 calls, timed on the stream (the first call of a
+        # process also loads code objects, so no single call sizes the rest)
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(20):
+            fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        done += t0.elapsed_time(t1)
+        calls += 20
 
 
 def max_rel_err(got, ref64):
@@ -109,26 +114,31 @@ def _gc_callback(phase, info, _t={}):
         GC_PAUSES_MS.append((time.perf_counter() - _t.pop("s")) * 1e3)
 
 
-def timed_loop(fn, steps, warmup, stream, flush=None):
-    """Runs fn `warmup` times, then `steps` timed times; returns (total wall
-    seconds between synchronisations, per-launch event milliseconds).
+def timed_loop(fn, steps, warmup, stream, flush=None, settle_ms=50.0):
+    """Runs fn untimed for about `settle_ms` of GPU time and `warmup` times,
+    then `steps` timed times; returns (total wall seconds between
+    synchronisations, per-launch event milliseconds).
 
     Python's cyclic garbage collector is off inside the timed region (as in
     the standard library's timeit): a full collection of this process takes
     tens of milliseconds, and one that lands between the start event and a
     rep's launches leaves the GPU idle for that long.  --keep-gc leaves it on
-    and records each collection's pause."""
+    and records each collection's pause.  The collection before the region
+    idles the GPU too, and its clocks drop after so long a gap, so the
+    settle runs after it."""
     import gc
     import torch
-    for _ in range(warmup):
-        fn()
-    torch.cuda.synchronize()
     gc.collect()
     if not KEEP_GC:
         gc.disable()
     elif _gc_callback not in gc.callbacks:
         gc.callbacks.append(_gc_callback)
     try:
+        if settle_ms > 0:
+            settle(fn, stream, settle_ms)
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
         evs = []
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -181,8 +191,7 @@ def main():
     if args.copy_calibration:
         out.copy_(x)  # 512 MiB read + 512 MiB written by one copy kernel
         torch.cuda.synchronize()
-    settle(step, stream)
-    wall, ms = timed_loop(step, args.steps, args.warmup, stream)
+    wall, ms = timed_loop(step, args.steps, args.warmup, stream, settle_ms=200.0)
     ms_avg = float(np.mean(ms))
     gvox = n * args.steps / wall / 1e9
     achieved = 8.0 * n / (ms_avg * 1e-3) / 1e9

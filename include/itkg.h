@@ -330,6 +330,15 @@ int itkg_batch_destroy(itkg_batch* plan);
 const char* itkg_last_error(void);
 /* Process-wide kernel selection (ITKG_PATH_AUTO by default). */
 int itkg_set_path(itkg_path path);
+/* Scratch memory of the entry points (per-axis intermediates, staging,
+ * work images) comes from a stream-ordered pool per device that keeps up to
+ * 8 GiB of freed memory between calls (environment ITKG_POOL_KEEP_BYTES sets
+ * another limit).  This hands the current device's idle pool memory beyond
+ * keep_bytes back to the driver, e.g. before allocating through another
+ * allocator (torch's cannot see the pool).  No counterpart in the reference,
+ * which allocates host Vecs per call. */
+int itkg_trim_scratch(uint64_t keep_bytes);
+
 /* Library version string. */
 const char* itkg_version(void);
 

@@ -44,7 +44,7 @@ EXPORTS = (
     "itkg_index_gaussian_kernel", "itkg_unsharp_mask_generate_region",
     "itkg_unsharp_mask_generate_region_host", "itkg_yvv_coefficients",
     "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
-    "itkg_canny_generate_region", "itkg_canny_generate_region_host",
+    "itkg_canny_generate_region", "itkg_canny_generate_region_host", "itkg_trim_scratch",
 )
 
 
@@ -129,6 +129,7 @@ def _load():
     L.itkg_batch_destroy.argtypes = [ctypes.c_void_p]
     L.itkg_last_error.restype = ctypes.c_char_p
     L.itkg_set_path.argtypes = [ctypes.c_int]
+    L.itkg_trim_scratch.argtypes = [ctypes.c_uint64]
     L.itkg_version.restype = ctypes.c_char_p
     return L
 
@@ -144,6 +145,14 @@ def _check(rc: int) -> None:
 def set_path(path: int) -> None:
     """Kernel selection: PATH_AUTO, PATH_PER_AXIS or PATH_FUSED."""
     _check(lib.itkg_set_path(path))
+
+
+def trim_scratch(keep_bytes: int = 0, device=None) -> None:
+    """Hands the library's idle scratch-pool memory on `device` (default: the
+    current one) beyond `keep_bytes` back to the driver, e.g. before torch
+    allocates large tensors (torch's allocator cannot see that pool)."""
+    with torch.cuda.device(device if device is not None else torch.cuda.current_device()):
+        _check(lib.itkg_trim_scratch(int(keep_bytes)))
 
 
 # ---------------------------------------------------------------------------

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -47,40 +48,65 @@ void clear_error() { g_err.clear(); }
 
 // Scratch memory of the entry points comes from a stream-ordered pool per
 // device that the library owns and that keeps up to kPoolKeepBytes of freed
-// memory across synchronisations.  A device's default pool returns freed
-// memory to the driver at every synchronisation, and a caller that waits
-// between calls would then pay for mapping each call's scratch again.
+// memory across synchronisations (ITKG_POOL_KEEP_BYTES overrides it).  A
+// device's default pool returns freed memory to the driver at every
+// synchronisation, and a caller that waits between calls would then pay for
+// mapping each call's scratch again.  itkg_trim_scratch hands the idle part
+// back (memory other allocators, such as torch's, cannot see otherwise).
 constexpr uint64_t kPoolKeepBytes = 8ull << 30;
 constexpr int kPoolDevices = 64;
 
+namespace {
+std::mutex g_pool_mu;
+hipMemPool_t g_pools[kPoolDevices] = {};
+
+uint64_t pool_keep_bytes() {
+    const char* v = std::getenv("ITKG_POOL_KEEP_BYTES");
+    if (!v || !*v) return kPoolKeepBytes;
+    char* end = nullptr;
+    const unsigned long long k = std::strtoull(v, &end, 10);
+    return (end && *end == '\0') ? (uint64_t)k : kPoolKeepBytes;
+}
+}  // namespace
+
 hipError_t pool_alloc(void** ptr, size_t bytes, hipStream_t s) {
-    static std::mutex mu;
-    static hipMemPool_t pools[kPoolDevices] = {};
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (dev < 0 || dev >= kPoolDevices) return hipErrorInvalidDevice;
     hipMemPool_t pool;
     {
-        std::lock_guard<std::mutex> lk(mu);
-        if (!pools[dev]) {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (!g_pools[dev]) {
             hipMemPoolProps props{};
             props.allocType = hipMemAllocationTypePinned;
             props.location.type = hipMemLocationTypeDevice;
             props.location.id = dev;
             hipMemPool_t created = nullptr;
             if ((e = hipMemPoolCreate(&created, &props)) != hipSuccess) return e;
-            uint64_t keep = kPoolKeepBytes;
+            uint64_t keep = pool_keep_bytes();
             if ((e = hipMemPoolSetAttribute(created, hipMemPoolAttrReleaseThreshold, &keep)) !=
                 hipSuccess) {
                 (void)hipMemPoolDestroy(created);
                 return e;
             }
-            pools[dev] = created;
+            g_pools[dev] = created;
         }
-        pool = pools[dev];
+        pool = g_pools[dev];
     }
     return hipMallocFromPoolAsync(ptr, bytes, pool, s);
+}
+
+// Releases the current device's idle pool memory beyond keep_bytes.
+int trim_pool(uint64_t keep_bytes) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kPoolDevices) return fail(ITKG_ERR_UNSUPPORTED, "device %d", dev);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!g_pools[dev]) return ITKG_OK;
+    e = hipMemPoolTrimTo(g_pools[dev], (size_t)keep_bytes);
+    return e == hipSuccess ? ITKG_OK : hip_fail(e, "hipMemPoolTrimTo");
 }
 
 // ---- region algebra (src/image.rs:17-64) --------------------------------
@@ -389,13 +415,15 @@ const FusedEntry* fused_for(const Plan& p) {
 // Forward: source = input view at the input region, destination = output
 // (requested).  Adjoint: source = grad_output (requested), destination =
 // grad_input at the input region.  tz <= 0 picks the z run with fused_tz.
+// The caller keeps the radii within the argument block's MAXR.
+template <class ARGS>
 void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
-                     int mode, int accumulate, int tz, int comp, int ncomp, Fused3dArgs* out) {
+                     int mode, int accumulate, int tz, int comp, int ncomp, ARGS* out) {
     const bool adj = fused_is_adjoint(mode);
     const itkg_region& sreg = adj ? p.req : p.in_reg;
     const itkg_region& dreg = adj ? p.in_reg : p.req;
     const int D = p.D;
-    Fused3dArgs& A = *out;
+    ARGS& A = *out;
     std::memset(&A, 0, sizeof A);
     A.src = static_cast<const float*>(src_view.data) + view_offset(D, src_view, sreg.index) * ncomp +
             comp;
@@ -439,6 +467,10 @@ void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& 
     A.tz = tz > 0 ? tz : fused_tz(A.tiles_x * A.tiles_y * ncomp, A.n_dst[2], rz);
     A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
 }
+template void fill_fused_args<Fused3dArgs>(const Plan&, const itkg_view&, const itkg_view&, int,
+                                           int, int, int, int, Fused3dArgs*);
+template void fill_fused_args<Fused3dArgsB>(const Plan&, const itkg_view&, const itkg_view&, int,
+                                            int, int, int, int, Fused3dArgsB*);
 
 // Tile shape of a FAST or adjoint launch: 64-row tiles (fewer halo rows per
 // output) or 32-row tiles (twice the tiles, so longer z runs fill the GPU
@@ -497,6 +529,8 @@ bool fused_src_ok(const Plan& p, const itkg_view& src, int ncomp, bool adjoint) 
     }
     return extent < (INT64_C(1) << 30);
 }
+
+bool fused_path_allowed() { return g_path.load() != ITKG_PATH_PER_AXIS; }
 
 bool use_fused(const Plan& p, itkg_dtype dt, int ncomp) {
     const int path = g_path.load();
@@ -1167,20 +1201,27 @@ int recursive_impl(int ndim, itkg_dtype dtype, double sigma, const double* spaci
     if ((rc = check_view(ndim, &out, out.region, "output"))) return rc;
     const int64_t total = region_len(ndim, largest);
     if (region_len(ndim, out.region) == 0 || total == 0) return ITKG_OK;
-    // the widen / narrow grids take axis 1 and the plane count of axes 2-3
-    // as grid dimensions
-    const int64_t n1 = ndim > 1 ? largest.size[1] : 1;
-    const int64_t planes = (ndim > 2 ? largest.size[2] : 1) * (ndim > 3 ? largest.size[3] : 1);
-    if (n1 > 65535 || planes > 65535)
-        return fail(ITKG_ERR_UNSUPPORTED, "recursive Gaussian: axis 1 or axes 2-3 exceed 65535 rows");
     int axes[ITKG_MAX_DIM];
     double coef[ITKG_MAX_DIM][4];
     int naxes = 0;
+    bool any_line = false;  // a filtered axis with lines of more than one sample
     for (int d = 0; d < ndim; ++d) {
         if (axis >= 0 && d != axis) continue;
         axes[naxes] = d;
         yvv_coefficients(sigma / spacing[d], coef[naxes]);
         ++naxes;
+        any_line = any_line || largest.size[d] > 1;
+    }
+    if (!any_line) {
+        // the result is the input (every filtered line has one sample); the
+        // widen / narrow copy grids take axis 1 and the plane count of axes
+        // 2-3 as grid dimensions
+        const int64_t n1 = ndim > 1 ? largest.size[1] : 1;
+        const int64_t planes = (ndim > 2 ? largest.size[2] : 1) * (ndim > 3 ? largest.size[3] : 1);
+        if (n1 > 65535 || planes > 65535)
+            return fail(ITKG_ERR_UNSUPPORTED,
+                        "recursive Gaussian with no filtered line longer than one sample: axis 1 "
+                        "or axes 2-3 exceed 65535 rows");
     }
     void* work = nullptr;
     hipError_t e = pool_alloc(&work, (size_t)total * sizeof(double), s);
@@ -1407,6 +1448,11 @@ int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double
     if ((rc = check_view(ndim, grad_input, p.in_reg, "grad_input"))) return rc;
     return backward_impl(p, dtype, ncomp, *grad_output, *grad_input, accumulate,
                          static_cast<hipStream_t>(stream), nullptr);
+}
+
+int itkg_trim_scratch(uint64_t keep_bytes) {
+    clear_error();
+    return trim_pool(keep_bytes);
 }
 
 const char* itkg_last_error(void) { return g_err.c_str(); }

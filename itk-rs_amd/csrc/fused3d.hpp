@@ -93,9 +93,11 @@ struct Cfg {
     static constexpr int NTH = 64 * NW;
     static constexpr int TX = 64;
     static constexpr bool SMALL = MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S;
+    // radii from 18 up (FMA modes): one column per lane, so that the ring
+    // (2r + 1 registers a column) fits
     static constexpr int CPT = MODE == FUSED_FWD_EXACT
                                    ? (EXACT_MID && RZ <= 10 ? 4 : 2)
-                                   : (RZ <= 7 && !SMALL ? 4 : 2);
+                                   : (RZ >= 18 ? 1 : (RZ <= 7 && !SMALL ? 4 : 2));
     static constexpr int TY = NW * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows
@@ -153,6 +155,18 @@ struct Cfg {
     static_assert(WX - 64 <= 64, "halo wider than one wave");
     static_assert((2 * PLANE + 2 * XPLANE) * 4 <= 160 * 1024, "LDS budget of one workgroup");
 };
+
+// The tile's source and destination pointers in the global address space.
+// A pointer read from a device table (the batched kernels' argument blocks)
+// is otherwise generic, and generic (flat) loads also count against the LDS
+// wait counter: every wait for an LDS read would then wait for the window
+// loads in flight as well.
+template <class T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ global_ptr<T> to_global(T* p) {
+    return (global_ptr<T>)p;
+}
 
 // Compile-time loop: f(integral_constant<int, I>) for I = 0..N-1.
 template <class F, int... I>
@@ -228,8 +242,8 @@ __device__ __forceinline__ float tap_sum(const float* v, const Weights<N>& w) {
 // barrier).  PERSIST: called from the persistent batch kernel, whose work
 // loop holds more registers: there 64-row tiles keep two load sets, the
 // adjoint keeps the plain step order and all outputs take plain stores.
-template <int RXY, int RZ, int MODE, bool PERSIST>
-__device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int tile, float* sIn,
+template <int RXY, int RZ, int MODE, bool PERSIST, class ARGS = Fused3dArgs>
+__device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, float* sIn,
                                              float* sX) {
     using C = Cfg<RXY, RZ, MODE>;
     constexpr int CPT = C::CPT, P = C::P, LJ = C::LJ;
@@ -245,6 +259,12 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     constexpr bool PNT = !PERSIST || PFIT;   // non-temporal dense stores
     constexpr bool PASM = !PERSIST || PFIT;  // opaque load offsets (scalar-base loads)
     constexpr bool ADJOINT = fused_is_adjoint(MODE);
+    // Radii from 16 up (forward): the x and y passes take their inputs one
+    // LDS read at a time and add each into the outputs it belongs to, instead
+    // of holding all 2r + Q (x) or 2r + CPT (y) inputs in registers first.
+    // Every output still receives its taps k = 0..2r in order (the same FMA
+    // chain as tap_sum).
+    constexpr bool STREAM = !ADJOINT && MODE != FUSED_FWD_EXACT && (RXY >= 16 || (PERSIST && RXY >= 12));
     const int tx_i = tile % a.tiles_x;
     const int ty_i = (tile / a.tiles_x) % a.tiles_y;
     const int tz_i = tile / (a.tiles_x * a.tiles_y);
@@ -276,17 +296,18 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                   (uint32_t)sizeof(float);
     }
     const int zv0 = z0 + a.shift[2] - RZ;  // source plane of step u = 0
-    const char* src_bytes = reinterpret_cast<const char*>(a.src);
+    const global_ptr<const char> src_bytes = to_global(reinterpret_cast<const char*>(a.src));
     auto load_plane = [&](float* r, int u) {
         const int zr = clampi(zv0 + u, 0, nsz - 1);
-        const char* plane = src_bytes + (int64_t)zr * a.src_stride[2] * (int64_t)sizeof(float);
+        const global_ptr<const char> plane =
+            src_bytes + (int64_t)zr * a.src_stride[2] * (int64_t)sizeof(float);
 #pragma unroll
         for (int j = 0; j < LJ; ++j) {
             // the offset made opaque at every call, so that its widening is
             // not hoisted out of the loop: the load keeps the scalar-base +
             // 32-bit-offset form (no 64-bit address per lane and load)
             if constexpr (PASM) asm volatile("" : "+v"(gofs[j]));
-            r[j] = *reinterpret_cast<const float*>(plane + gofs[j]);
+            r[j] = *reinterpret_cast<const global_ptr<const float>>(plane + gofs[j]);
         }
     };
     float* const sIn_lane = sIn + wave * 64 + lane;
@@ -338,7 +359,31 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
             if (g < NG) {
                 const int row = Q * g + xr;
                 const vecq* srcq = reinterpret_cast<const vecq*>(&srcp[row * C::WX + Q * xq]);
-                float v[Q * C::XQ];
+                float v[STREAM ? 1 : Q * C::XQ];
+                float o[Q];
+                if constexpr (STREAM) {
+#pragma unroll
+                    for (int i = 0; i < C::XQ; ++i) {
+                        const vecq t = srcq[i];
+                        float tv[Q];
+                        if constexpr (Q == 4) {
+                            tv[0] = t.x, tv[1] = t.y, tv[2] = t.z, tv[3] = t.w;
+                        } else if constexpr (Q == 2) {
+                            tv[0] = t.x, tv[1] = t.y;
+                        } else {
+                            tv[0] = t;
+                        }
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) {
+#pragma unroll
+                            for (int j = 0; j < Q; ++j) {
+                                const int k = Q * i + q - j;  // tap of input Q*i+q in output j
+                                if (k == 0) o[j] = tv[q] * wx[0];
+                                else if (k > 0 && k <= 2 * RXY) o[j] = __builtin_fmaf(tv[q], wx[k], o[j]);
+                            }
+                        }
+                    }
+                } else {
 #pragma unroll
                 for (int i = 0; i < C::XQ; ++i) {
                     const vecq t = srcq[i];
@@ -354,9 +399,9 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                         v[i] = t;
                     }
                 }
-                float o[Q];
 #pragma unroll
                 for (int j = 0; j < Q; ++j) o[j] = tap_sum<RXY, MODE>(v + j, wx);
+                }
                 if constexpr (ADJOINT) {
                     const int xg = x0 + Q * xq;  // destination x of o[0]
                     const int jlo = -xg, jhi = ndx - 1 - xg;  // output index of x = 0 / ndx - 1
@@ -391,10 +436,16 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                 if (k < C::RL) {
                     const int row = C::RA + k;
                     const float* rp = srcp + row * C::WX + lane;
+                    float o;
+                    if constexpr (STREAM) {
+                        o = rp[0] * wx[0];
+#pragma unroll
+                        for (int i = 1; i <= 2 * RXY; ++i) o = __builtin_fmaf(rp[i], wx[i], o);
+                    } else {
                     float v[2 * RXY + 1];
 #pragma unroll
                     for (int i = 0; i <= 2 * RXY; ++i) v[i] = rp[i];
-                    float o = tap_sum<RXY, MODE>(v, wx);
+                    o = tap_sum<RXY, MODE>(v, wx);
                     if constexpr (ADJOINT) {
                         // one test; only the (at most two) edge lanes enter
                         if (x_edge && (x0 + lane == 0 || x0 + lane == ndx - 1)) {
@@ -407,6 +458,7 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                                 for (int m = 1; m <= RXY; ++m) o = __builtin_fmaf(a.px[m], v[m], o);
                             }
                         }
+                    }
                     }
                     dstp[row * C::SX + lane] = o;
                 }
@@ -516,6 +568,19 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     [[maybe_unused]] const bool y_edge = ybase <= 0 || ybase + CPT > ndy - 1;
     float val[CPT];
     auto y_pass = [&](const float* xp) {
+        if constexpr (STREAM) {
+#pragma unroll
+            for (int i = 0; i < CPT + 2 * RXY; ++i) {
+                const float t = xp[(wave * CPT + i) * C::SX + lane];
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const int k = i - c;  // tap of input row i in output c
+                    if (k == 0) val[c] = t * wy[0];
+                    else if (k > 0 && k <= 2 * RXY) val[c] = __builtin_fmaf(t, wy[k], val[c]);
+                }
+            }
+            return;
+        }
         float v[CPT + 2 * RXY];
 #pragma unroll
         for (int i = 0; i < CPT + 2 * RXY; ++i) v[i] = xp[(wave * CPT + i) * C::SX + lane];
@@ -553,15 +618,27 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
     const bool x_ok = ox < ndx;
     const int64_t dxs = a.dst_stride[0], dys = a.dst_stride[1], dzs = a.dst_stride[2];
     const bool store_nt = PNT && dxs == 1 && !(ADJOINT && a.accumulate);  // tile-uniform
-    float* const dwave = a.dst + x0 + (int64_t)ybase * dys;  // used when dxs == 1
+    const global_ptr<float> dwave = to_global(a.dst) + x0 + (int64_t)ybase * dys;  // used when dxs == 1
     uint32_t dlane = (uint32_t)lane * (uint32_t)sizeof(float);
 
     // ---- z ring: fixed slots, rotating weights (wz2 = wz repeated twice)
     static_assert(CPT <= 8, "at most 8 ring columns per lane");
-    float r0[P], r1[P], r2[P], r3[P], r4[P], r5[P], r6[P], r7[P];
+    // A column's ring is one array of up to 32 slots (the widest vector the
+    // compiler keeps in registers with indexed access) plus, for P > 32, a
+    // second array with slots 32..P-1; the slot that opens and the one that
+    // completes are each indexed within one of the two.
+    constexpr int PL = P > 32 ? 32 : P, PH = P > 32 ? P - 32 : 1;
+    static_assert(P <= 32 || !ADJOINT, "adjoint rings hold at most 32 slots");
+    float r0[PL], r1[PL], r2[PL], r3[PL], r4[PL], r5[PL], r6[PL], r7[PL];
+    [[maybe_unused]] float h0[PH], h1[PH], h2[PH], h3[PH], h4[PH], h5[PH], h6[PH], h7[PH];
 #pragma unroll
-    for (int p = 0; p < P; ++p)
+    for (int p = 0; p < PL; ++p)
         r0[p] = r1[p] = r2[p] = r3[p] = r4[p] = r5[p] = r6[p] = r7[p] = 0.0f;
+    if constexpr (P > 32) {
+#pragma unroll
+        for (int p = 0; p < PH; ++p)
+            h0[p] = h1[p] = h2[p] = h3[p] = h4[p] = h5[p] = h6[p] = h7[p] = 0.0f;
+    }
     const int tz_act = min(a.tz, ndz - z0);
     const int U = tz_act + 2 * RZ;
     const int t_hi = ndz - 1 - z0;  // step-relative index of the last destination plane
@@ -610,16 +687,28 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         static_for<CPT>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
             float* acc = ring_of<c>(r0, r1, r2, r3, r4, r5, r6, r7);
-            acc[s] = 0.0f;  // the output whose first tap is this plane
+            [[maybe_unused]] float* hac = ring_of<c>(h0, h1, h2, h3, h4, h5, h6, h7);
+            // the output whose first tap is this plane
+            if constexpr (P > 32) {
+                // each array indexed within its bounds, whichever holds slot s
+                // (a select between the two arrays' addresses would take both
+                // out of registers)
+                const int sl = min(s, PL - 1), sh = max(s - PL, 0);
+                acc[sl] = s < PL ? 0.0f : acc[sl];
+                hac[sh] = s >= PL ? 0.0f : hac[sh];
+            } else {
+                acc[s] = 0.0f;
+            }
 #pragma unroll
             for (int j = 0; j < P; ++j) {
+                float& aj = j < PL ? acc[j] : hac[j - PL];
                 if constexpr (MODE == FUSED_FWD_EXACT) {
                     const float prod = val[c] * w[-j];
-                    acc[j] = acc[j] + prod;
+                    aj = aj + prod;
                 } else if constexpr (EARLY) {
-                    acc[j] = __builtin_fmaf(val[c], wzs[j], acc[j]);
+                    aj = __builtin_fmaf(val[c], wzs[j], aj);
                 } else {
-                    acc[j] = __builtin_fmaf(val[c], w[-j], acc[j]);
+                    aj = __builtin_fmaf(val[c], w[-j], aj);
                 }
             }
         });
@@ -638,7 +727,13 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
         }
         static_for<CPT>([&](auto ci) {
             constexpr int c = decltype(ci)::value;
-            o[c] = ring_of<c>(r0, r1, r2, r3, r4, r5, r6, r7)[sn];  // tap 2r has just been added
+            // tap 2r has just been added
+            if constexpr (P > 32) {
+                const float lo = ring_of<c>(r0, r1, r2, r3, r4, r5, r6, r7)[min(sn, PL - 1)];
+                const float hi = ring_of<c>(h0, h1, h2, h3, h4, h5, h6, h7)[max(sn - PL, 0)];
+                o[c] = sn < PL ? lo : hi;
+            } else
+                o[c] = ring_of<c>(r0, r1, r2, r3, r4, r5, r6, r7)[sn];
         });
         s = sn;
         if (t >= 0 && t < tz_act) {
@@ -648,17 +743,20 @@ __device__ __forceinline__ void fused3d_tile(const Fused3dArgs& a, const int til
                 // at the stores, which take the scalar-base + 32-bit-offset
                 // form
                 asm volatile("" : "+v"(dlane));
-                char* const dplane = reinterpret_cast<char*>(dwave + (int64_t)(z0 + t) * dzs);
+                const global_ptr<char> dplane =
+                    reinterpret_cast<global_ptr<char>>(dwave + (int64_t)(z0 + t) * dzs);
 #pragma unroll
                 for (int c = 0; c < CPT; ++c)
                     if (x_ok && ybase + c < ndy)
-                        __builtin_nontemporal_store(o[c], reinterpret_cast<float*>(dplane + (int64_t)c * dys * 4 + dlane));
+                        __builtin_nontemporal_store(
+                            o[c], reinterpret_cast<global_ptr<float>>(dplane + (int64_t)c * dys * 4 + dlane));
             } else {
                 // one component of interleaved pixels (L2 merges the partial
                 // lines), or the adjoint's accumulation; its own address
                 // arithmetic, so that the compiler cannot merge these stores
                 // with the non-temporal ones (a merge drops the hint)
-                float* const dcol = a.dst + (int64_t)ox * dxs + (int64_t)ybase * dys + (int64_t)(z0 + t) * dzs;
+                const global_ptr<float> dcol =
+                    to_global(a.dst) + (int64_t)ox * dxs + (int64_t)ybase * dys + (int64_t)(z0 + t) * dzs;
 #pragma unroll
                 for (int c = 0; c < CPT; ++c) {
                     if (x_ok && ybase + c < ndy) {
@@ -781,6 +879,131 @@ void fused3d_batch_kernel(const Fused3dArgs* __restrict__ vols, const int2* __re
         if (it.x < 0) continue;  // padding of the interleaved list
         fused3d_tile<RXY, RZ, MODE, true>(vols[it.x], it.y, sIn, sX);
     }
+}
+
+// ---- the single persistent launch of a batched plan ---------------------
+// One kernel serves every fused volume of a plan: each work item names a
+// volume, a tile of it and a class -- the radius r (1..kMegaMaxR for FAST,
+// 1..15 for the adjoint) on the family's own tile shape (64 rows up to
+// radius 7, 32 rows beyond), or kMegaSmall + r for radius r (<= 7) on
+// 32-row tiles -- and the workgroup runs that class's tile.  Items sit in 8
+// queues, one per XCD (workgroup b runs on XCD b mod 8), each in decreasing
+// order of modelled cost; a workgroup takes the next item of its own queue
+// with one atomic add and, once that queue is empty, takes items from the
+// others.  The last workgroup to finish resets the 8 queue heads and the
+// finish count, so the next launch (or graph replay) starts from zero with
+// no memset.
+constexpr int kMegaSmall = 32;
+constexpr int kMegaMaxR = kFusedMaxRBatch;
+constexpr int kMegaQueues = 8;
+
+template <int FAMILY>
+constexpr int mega_mode(int cls) {
+    return cls > kMegaSmall ? (FAMILY == FUSED_FWD_FAST ? FUSED_FWD_FAST_S : FUSED_ADJ_S) : FAMILY;
+}
+template <int FAMILY>
+constexpr int mega_max_r() {
+    return FAMILY == FUSED_FWD_FAST ? kMegaMaxR : 15;
+}
+
+template <int FAMILY, int CLS>
+__device__ __forceinline__ void mega_case(const Fused3dArgsB& a, int tile, float* lds) {
+    constexpr int R = CLS > kMegaSmall ? CLS - kMegaSmall : CLS;
+    constexpr int MODE = mega_mode<FAMILY>(CLS);
+    fused3d_tile<R, R, MODE, true, Fused3dArgsB>(a, tile, lds, lds + 2 * Cfg<R, R, MODE>::PLANE);
+}
+
+template <int FAMILY, int... I>
+__device__ __forceinline__ void mega_dispatch(std::integer_sequence<int, I...>, int cls,
+                                              const Fused3dArgsB& a, int tile, float* lds) {
+    // classes 1..maxR, then kMegaSmall + 1..7
+    (void)((cls == (I < mega_max_r<FAMILY>() ? I + 1 : kMegaSmall + I - mega_max_r<FAMILY>() + 1)
+                ? (mega_case<FAMILY, (I < mega_max_r<FAMILY>() ? I + 1
+                                                               : kMegaSmall + I - mega_max_r<FAMILY>() + 1)>(
+                       a, tile, lds),
+                   true)
+                : false) ||
+           ...);
+}
+
+// `trace` (null unless the caller asks for a breakdown) receives, per work
+// item w, the start and end wall-clock ticks and the workgroup.
+template <int FAMILY>
+__global__ __launch_bounds__(1024) void fused3d_mega_kernel(const Fused3dArgsB* __restrict__ vols,
+                                                            const int4* __restrict__ work,
+                                                            const int* __restrict__ qstart,
+                                                            int* __restrict__ heads,
+                                                            unsigned long long* __restrict__ trace) {
+    extern __shared__ __attribute__((aligned(16))) float mega_lds[];
+    __shared__ int s_item;
+    const int home = (int)(blockIdx.x % kMegaQueues);
+    int tries = 0;  // queues found empty so far, from `home` on (they stay empty)
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int w = -1;
+            while (tries < kMegaQueues) {
+                const int q = (home + tries) % kMegaQueues;
+                const int n = qstart[q + 1] - qstart[q];
+                if (__hip_atomic_load(&heads[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n) {
+                    const int k = atomicAdd(&heads[q], 1);
+                    if (k < n) {
+                        w = qstart[q] + k;
+                        break;
+                    }
+                }
+                ++tries;
+            }
+            s_item = w;
+        }
+        __syncthreads();
+        // uniform: the item's table entries then come in with scalar loads
+        const int w = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();  // every wave has read s_item before lane 0 writes the next
+        if (w < 0) break;
+        const int4 it = work[w];
+        const unsigned long long t0 = trace ? wall_clock64() : 0ull;
+        mega_dispatch<FAMILY>(std::make_integer_sequence<int, mega_max_r<FAMILY>() + 7>{}, it.z,
+                              vols[it.x], it.y, mega_lds);
+        if (trace) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                trace[3 * w] = t0;
+                trace[3 * w + 1] = wall_clock64();
+                trace[3 * w + 2] = blockIdx.x;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int done = atomicAdd(&heads[kMegaQueues], 1);
+        if (done == (int)gridDim.x - 1) {
+            // every other workgroup has made its last queue access
+            for (int q = 0; q <= kMegaQueues; ++q) heads[q] = 0;
+            __threadfence();
+        }
+    }
+}
+
+// LDS bytes and tile rows of a class (host and device).
+template <int FAMILY, int CLS>
+constexpr int mega_lds_of() {
+    constexpr int R = CLS > kMegaSmall ? CLS - kMegaSmall : CLS;
+    using C = Cfg<R, R, mega_mode<FAMILY>(CLS)>;
+    return (2 * C::PLANE + 2 * C::XPLANE) * (int)sizeof(float);
+}
+template <int FAMILY, int CLS>
+constexpr int mega_ty_of() {
+    constexpr int R = CLS > kMegaSmall ? CLS - kMegaSmall : CLS;
+    return Cfg<R, R, mega_mode<FAMILY>(CLS)>::TY;
+}
+template <int FAMILY, int... I>
+void mega_table(std::integer_sequence<int, I...>, int* lds, int* ty) {
+    constexpr int M = mega_max_r<FAMILY>();
+    ((lds[(I < M ? I + 1 : kMegaSmall + I - M + 1)] =
+          mega_lds_of<FAMILY, (I < M ? I + 1 : kMegaSmall + I - M + 1)>(),
+      ty[(I < M ? I + 1 : kMegaSmall + I - M + 1)] =
+          mega_ty_of<FAMILY, (I < M ? I + 1 : kMegaSmall + I - M + 1)>()),
+     ...);
 }
 
 // Workgroup size of mode `mode` for (RXY, RZ).

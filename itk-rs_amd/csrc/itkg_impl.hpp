@@ -115,7 +115,8 @@ enum FusedMode {
 constexpr bool fused_is_adjoint(int mode) { return mode == FUSED_ADJ || mode == FUSED_ADJ_S; }
 constexpr int kFusedMaxR = 16;
 
-struct Fused3dArgs {
+template <int MAXR>
+struct Fused3dArgsT {
     const float* src;        // source view pixel at the source region index
     float* dst;              // destination view pixel at the destination region index
     int64_t src_stride[3];
@@ -126,10 +127,16 @@ struct Fused3dArgs {
     int32_t tz;              // destination planes per workgroup
     int32_t tiles_x, tiles_y, tiles_z;
     int32_t accumulate;      // adjoint: dst += result
-    float wx[2 * kFusedMaxR + 1], wy[2 * kFusedMaxR + 1], wz[2 * kFusedMaxR + 1];
-    float wz2[2 * (2 * kFusedMaxR + 1)];  // wz repeated twice: rotated-weight lookups
-    float px[kFusedMaxR + 1], py[kFusedMaxR + 1], pz[kFusedMaxR + 1];  // adjoint edge sums
+    float wx[2 * MAXR + 1], wy[2 * MAXR + 1], wz[2 * MAXR + 1];
+    float wz2[2 * (2 * MAXR + 1)];  // wz repeated twice: rotated-weight lookups
+    float px[MAXR + 1], py[MAXR + 1], pz[MAXR + 1];  // adjoint edge sums
 };
+// Arguments of a plain launch (they travel in the kernel-argument segment,
+// four of them per launch, so their size is bounded) ...
+using Fused3dArgs = Fused3dArgsT<kFusedMaxR>;
+// ... and of the persistent batched kernel (a device table, radii up to 24).
+constexpr int kFusedMaxRBatch = 24;
+using Fused3dArgsB = Fused3dArgsT<kFusedMaxRBatch>;
 
 // One compiled instantiation of the fused kernel: x/y radius rxy, z radius
 // rz (0 for 2-D images), its y tile height, and its launchers.  `batch` and
@@ -153,6 +160,30 @@ struct FusedEntry {
 constexpr int kFusedTileX = 64;
 // The instantiation for (rxy, rz), or null.
 const FusedEntry* fused_entry(int rxy, int rz);
+
+// The single persistent launch of a batched plan (fused3d_mega_kernel):
+// `fast` serves forward FAST, `adj` the adjoint.  Work items are int4
+// (volume, tile, class, unused), class = radius r on the family's own tile
+// shape or kMegaClassSmall + r (r <= 7) on 32-row tiles; qstart[0..8] bounds
+// the 8 per-XCD queues in `work`; heads[0..8] are the queue heads and the
+// finish count, zero before the first launch and reset by each launch.
+constexpr int kMegaClassSmall = 32;
+constexpr int kMegaClasses = 40;  // class index bound
+hipError_t mega_prepare_fast(int lds_bytes);
+hipError_t mega_prepare_adj(int lds_bytes);
+// trace: null, or 3 ticks-and-workgroup words per work item (breakdowns)
+hipError_t mega_launch_fast(const Fused3dArgsB* vols, const int4* work, const int* qstart,
+                            int* heads, int grid, int lds_bytes, hipStream_t s,
+                            unsigned long long* trace = nullptr);
+hipError_t mega_launch_adj(const Fused3dArgsB* vols, const int4* work, const int* qstart,
+                           int* heads, int grid, int lds_bytes, hipStream_t s,
+                           unsigned long long* trace = nullptr);
+int mega_occupancy_fast(int lds_bytes);
+int mega_occupancy_adj(int lds_bytes);
+// LDS bytes and tile rows of every class (arrays of kMegaClasses; classes
+// the family lacks are left untouched).
+void mega_classes_fast(int* lds, int* ty);
+void mega_classes_adj(int* lds, int* ty);
 
 // Per-call plan of GaussianFilter::generate_region (src/filters/gaussian.rs:
 // 59-90): radii, f64 weights and the region of every pass.
@@ -178,8 +209,13 @@ int fused_tz(int tiles_xy, int nz, int r);
 // True when the fused kernel can address the source view of this call (in-
 // plane offsets are 32-bit).
 bool fused_src_ok(const Plan& p, const itkg_view& src, int ncomp, bool adjoint);
+// Every extent fits the fused kernel's 32-bit coordinates.
+bool fits_i32(const Plan& p);
+// The path setting (itkg_set_path) allows the fused kernel.
+bool fused_path_allowed();
+template <class A>
 void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& dst_view,
-                     int mode, int accumulate, int tz, int comp, int ncomp, Fused3dArgs* out);
+                     int mode, int accumulate, int tz, int comp, int ncomp, A* out);
 // Enqueue one forward / adjoint; `scratch` (nullable) holds scratch_bytes().
 int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
                  const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch);

@@ -283,13 +283,15 @@ hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxe
                             const double (*coef)[4], hipStream_t s) {
     const int64_t total = a.n[0] * a.n[1] * a.n[2] * a.n[3];
     if (total == 0) return hipSuccess;
-    if (a.n[1] > 65535 || a.n[2] * a.n[3] > 65535 || a.m[1] > 65535 || a.m[2] * a.m[3] > 65535)
-        return hipErrorInvalidValue;  // grid rows / planes beyond the launch limit
     int eff[4], ne = 0;  // the listed axes with lines of more than one sample
     for (int k = 0; k < naxes; ++k)
         if (a.n[axes[k]] > 1) eff[ne++] = k;  // a line of one sample is left as it is
     const int64_t m = a.m[0] * a.m[1] * a.m[2] * a.m[3];
     if (ne == 0) {
+        // the copy grids: rows of axis 1 and planes of axes 2-3 as grid y / z
+        if (a.n[1] > 65535 || a.n[2] * a.n[3] > 65535 || a.m[1] > 65535 ||
+            a.m[2] * a.m[3] > 65535)
+            return hipErrorInvalidValue;
         hipLaunchKernelGGL(widen_kernel<T>,
                            dim3((unsigned)((a.n[0] + kRowBlock - 1) / kRowBlock), (unsigned)a.n[1],
                                 (unsigned)(a.n[2] * a.n[3])),

@@ -123,3 +123,64 @@ def test_batched_adjoint_uneven_gradients(itk, dev):
             torch.cuda.synchronize()
             assert torch.equal(o, ref), (shape, tune)
     plan.close()
+
+
+def _radius_volumes(dev, radii, seed, lo=8, hi=90):
+    """One volume per radius (spacing 1, sigma (r - 0.5) / 3 gives radius r),
+    extents drawn in [lo, hi): tiles past the volume edge, extents below the
+    tile and below 2r + 1."""
+    rng = np.random.default_rng(seed)
+    vols, sig = [], []
+    for r in radii:
+        shape = tuple(int(x) for x in rng.integers(lo, hi, size=3))
+        vols.append(torch.rand(shape, device=dev))
+        sig.append((r - 0.5) / 3.0)
+    return vols, sig
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_single_launch_every_radius(itk, dev, direction):
+    """FAST forward (radii 1..24, all in the one persistent launch) and the
+    adjoint (radii 1..15 in it, 16..20 on the per-axis path beside it): every
+    volume equals its plain result bit for bit, on the captured first run,
+    on replays, and after autotuning (each run starts from NaN-filled
+    outputs, so queue heads not reset between launches would show)."""
+    radii = list(range(1, 25)) if direction == 0 else list(range(1, 21))
+    vols, sig = _radius_volumes(dev, radii, 21 + direction)
+    for v, r, s in zip(vols, radii, sig):
+        assert itk.kernel_radius(s, 1.0) == r
+    sps = [(1.0, 1.0, 1.0)] * len(vols)
+    outs = [torch.full_like(v, float("nan")) for v in vols]
+    plan = itk.BatchPlan(vols, outs, sig, sps, direction=direction, arith=itk.FAST)
+    refs = [_plain(itk, v, s, sp, direction, itk.FAST) for v, s, sp in zip(vols, sig, sps)]
+    torch.cuda.synchronize()
+    st = torch.cuda.Stream()
+    for rep in range(5):
+        if rep == 3:
+            plan.autotune(st)
+        for o in outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.run(st)
+        st.synchronize()
+        for o, ref, r in zip(outs, refs, radii):
+            assert torch.equal(o, ref), (direction, rep, r, tuple(o.shape))
+    plan.close()
+
+
+def test_single_launch_large_volumes_and_tiny_ones(itk, dev):
+    """Volumes large enough to be split along z next to volumes smaller than
+    one tile, in one plan."""
+    shapes = [(200, 150, 130), (3, 5, 7), (1, 1, 1), (64, 64, 64), (65, 33, 129), (9, 300, 70)]
+    sig = [2.0, 1.2, 0.9, 3.7, 6.1, 4.4]
+    vols = [torch.rand(sh, device=dev) for sh in shapes]
+    sps = [(1.0, 1.0, 1.0)] * len(vols)
+    outs = [torch.full_like(v, float("nan")) for v in vols]
+    plan = itk.BatchPlan(vols, outs, sig, sps, direction=0, arith=itk.FAST)
+    st = torch.cuda.Stream()
+    for _ in range(2):
+        plan.run(st)
+        st.synchronize()
+        for v, o, s in zip(vols, outs, sig):
+            assert torch.equal(o, itk.gaussian(v, s, (1.0,) * 3, arith=itk.FAST)), (tuple(v.shape), s)
+    plan.close()

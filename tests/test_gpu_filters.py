@@ -257,6 +257,34 @@ def test_recursive_matches_oracle(itk, orc, dev, size, spacing, sigma, axis, req
     assert np.array_equal(out.data.cpu().numpy(), ref)
 
 
+def test_recursive_axis_1_beyond_65535(itk, orc, dev):
+    """A 16 x 100000 image: the line kernels take 1-D grids, so axis 1 may
+    exceed the 65535 rows of a grid dimension."""
+    x = rand((100000, 16))
+    src = itk.Image(itk.Region([0, 0], [16, 100000]), (1.0, 1.0), (0.0, 0.0), torch.from_numpy(x).to(dev))
+    reg = itk.Region([3, 99000], [10, 1000])
+    out = itk.RecursiveGaussianFilter(src, 1.5).generate_region(reg).data.cpu().numpy()
+    ref = orc.recursive_generate_region(x, [0, 0], [1.0, 1.0], 1.5, -1, [3, 99000], [10, 1000])
+    assert np.array_equal(out, ref)
+
+
+def test_trim_scratch_keeps_results(itk, dev):
+    """Trimming the scratch pool between calls changes nothing but the memory
+    the pool holds."""
+    x = torch.rand((40, 50, 60), device=dev)
+    itk.set_path(itk.PATH_PER_AXIS)  # the per-axis path allocates its intermediates
+    try:
+        a = itk.gaussian(x, 2.0)
+        torch.cuda.synchronize()
+        itk.trim_scratch(0)
+        b = itk.gaussian(x, 2.0)
+        torch.cuda.synchronize()
+        itk.trim_scratch(1 << 20)
+    finally:
+        itk.set_path(itk.PATH_AUTO)
+    assert torch.equal(a, b)
+
+
 def test_recursive_reference_tests_and_host_entry(itk, dev):
     # recursive_gaussian.rs:252-300 through the GPU filter
     imp = torch.zeros(51)

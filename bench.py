@@ -62,10 +62,16 @@ def parse():
                          "c2, c5, c4, c5_batch, c1_c3, filters")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--keep-gc", action="store_true",
-                    help="leave Python's garbage collector on in the timed regions and report "
-                         "its pauses (gc_pauses_ms)")
+                    help="leave Python's garbage collector as it is (no collection before a "
+                         "timed region, on inside it) and report its pauses (gc_pauses_ms)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprof-derived HBM bytes per launch, if measured")
+    ap.add_argument("--trace-loops", type=int, default=0, metavar="N",
+                    help="instead of the bench: N reps each of the BinomialBlurFilter call and "
+                         "the 16-gradient plain adjoint loop with the collector as it is, "
+                         "recording each rep's host and GPU time and every collector pause; "
+                         "prints the reps slower than twice the median and the pauses inside "
+                         "them")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device-to-device copy (a counter reference "
                          "of known size for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
@@ -122,14 +128,14 @@ def timed_loop(fn, steps, warmup, stream, flush=None, settle_ms=50.0):
     Python's cyclic garbage collector is off inside the timed region (as in
     the standard library's timeit): a full collection of this process takes
     tens of milliseconds, and one that lands between the start event and a
-    rep's launches leaves the GPU idle for that long.  --keep-gc leaves it on
-    and records each collection's pause.  The collection before the region
-    idles the GPU too, and its clocks drop after so long a gap, so the
-    settle runs after it."""
+    rep's launches leaves the GPU idle for that long.  A collection runs just
+    before the region instead; it idles the GPU too, and its clocks drop after
+    so long a gap, so the settle runs after it.  --keep-gc leaves the
+    collector as it is and records each collection's pause."""
     import gc
     import torch
-    gc.collect()
     if not KEEP_GC:
+        gc.collect()
         gc.disable()
     elif _gc_callback not in gc.callbacks:
         gc.callbacks.append(_gc_callback)
@@ -159,6 +165,69 @@ def timed_loop(fn, steps, warmup, stream, flush=None, settle_ms=50.0):
     return wall, [a.elapsed_time(b) for a, b in evs]
 
 
+def trace_loops(itk, wl, dev, stream, reps):
+    """Per-rep host and GPU times of two loop legs with Python's collector
+    left as it is, and the collector's pauses that overlap slow reps."""
+    import gc
+    import numpy as np
+    import torch
+    pauses = []
+    mark = {}
+
+    def cb(phase, info):
+        if phase == "start":
+            mark["t"] = time.perf_counter()
+        elif "t" in mark:
+            pauses.append((mark.pop("t"), time.perf_counter(), info.get("generation")))
+    gc.callbacks.append(cb)
+
+    def run(name, fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        pauses.clear()
+        recs = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            h0 = time.perf_counter()
+            a.record(stream)
+            fn()
+            b.record(stream)
+            recs.append((h0, time.perf_counter(), a, b))
+        torch.cuda.synchronize()
+        gpu = np.array([a.elapsed_time(b) for _, _, a, b in recs])
+        host = np.array([(h1 - h0) * 1e3 for h0, h1, _, _ in recs])
+        slow = [i for i in range(reps) if gpu[i] > 2 * np.median(gpu)]
+        print(json.dumps({
+            "leg": name, "reps": reps, "gpu_ms_median": float(np.median(gpu)),
+            "gpu_ms_mean": float(gpu.mean()), "gpu_ms_max": float(gpu.max()),
+            "host_ms_median": float(np.median(host)), "host_ms_max": float(host.max()),
+            "collections": len(pauses),
+            "longest_pause_ms": max([(e - s) * 1e3 for s, e, _ in pauses], default=0.0),
+            "slow_reps": [{"rep": i, "gpu_ms": float(gpu[i]), "host_ms": float(host[i]),
+                           "pauses_inside_ms": [round((e - s) * 1e3, 3) for s, e, _ in pauses
+                                                if s < recs[i][1] and e > recs[i][0]]}
+                          for i in slow[:8]]}), flush=True)
+
+    xm = torch.rand((512, 512, 512), device=dev)
+    im = itk.Image(itk.Region([0, 0, 0], [512] * 3), (1.0,) * 3, (0.0,) * 3, xm)
+    om = itk.Image(im.region, (1.0,) * 3, (0.0,) * 3, torch.empty_like(xm))
+    bf = itk.BinomialBlurFilter(im, 4)
+    bf.arith = itk.FAST
+    run("binomial_n4_512", lambda: bf.generate_into(im, om))
+    del xm, im, om
+    torch.cuda.empty_cache()
+    grads, gparams = wl.c5_batch(dev)
+    gouts = [torch.empty_like(q) for q in grads]
+
+    def gloop():
+        for q, o, p in zip(grads, gouts, gparams):
+            itk.gaussian_backward(q, p[1], p[2], out=o)
+    run("c5_plain_adjoint_loop_16", gloop)
+    gc.callbacks.remove(cb)
+
+
 def main():
     args = parse()
     global KEEP_GC
@@ -178,6 +247,9 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
     arith = itk.EXACT if args.arith == "exact" else itk.FAST
+    if args.trace_loops:
+        trace_loops(itk, wl, dev, stream, args.trace_loops)
+        return
 
     # ---- headline: 512^3 f32, sigma 2 -------------------------------------
     x, sigma, sp = wl.c2(dev)
@@ -189,7 +261,9 @@ def main():
     f.arith = arith
     step = lambda: f.generate_into(img, oimg)  # noqa: E731
     if args.copy_calibration:
-        out.copy_(x)  # 512 MiB read + 512 MiB written by one copy kernel
+        # 512 MiB read + 512 MiB written by one elementwise kernel (a plain
+        # copy_ may go to a DMA engine, which the counters do not see)
+        torch.mul(x, 1.0, out=out)
         torch.cuda.synchronize()
     wall, ms = timed_loop(step, args.steps, args.warmup, stream, settle_ms=200.0)
     ms_avg = float(np.mean(ms))

@@ -50,6 +50,22 @@ typedef enum {
 
 typedef enum { ITKG_F32 = 0, ITKG_F64 = 1 } itkg_dtype;
 
+/* Element types of the pixel conversions (itkg_convert_pixels). */
+typedef enum {
+    ITKG_PT_U8 = 0,
+    ITKG_PT_U16 = 1,
+    ITKG_PT_I16 = 2,
+    ITKG_PT_I32 = 3,
+    ITKG_PT_F32 = 4,
+    ITKG_PT_F64 = 5
+} itkg_ptype;
+
+/* ITKG_CONVERT_AS: Rust's `as` (integer -> integer truncates or extends;
+ * to float rounds to nearest even; float -> integer truncates toward zero,
+ * saturates, NaN -> 0).  ITKG_CONVERT_ROUND: a float is first rounded half
+ * away from zero (f32::round), then `as`. */
+typedef enum { ITKG_CONVERT_AS = 0, ITKG_CONVERT_ROUND = 1 } itkg_convert_mode;
+
 /* Arithmetic mode.
  *   ITKG_EXACT: the reference's arithmetic -- every tap fl(x * (T)w_k) added
  *     to an accumulator that starts at +0, taps k = 0..2r in order, no FMA,
@@ -330,6 +346,32 @@ int itkg_batch_destroy(itkg_batch* plan);
 const char* itkg_last_error(void);
 /* Process-wide kernel selection (ITKG_PATH_AUTO by default). */
 int itkg_set_path(itkg_path path);
+/* ---- pixel conversions (the UnaryFilter casts around a Gaussian) ------- */
+
+/* UnaryFilter::generate_region, src/filters/mod.rs:76-80, for the cast
+ * closures of examples/blur_png.rs:35-37 (`|p: u8| p as f32` is
+ * ITKG_CONVERT_AS to f32; `|p: f32| p.round().clamp(0.0, 255.0) as u8` is
+ * ITKG_CONVERT_ROUND to u8), on the device: every pixel of output->region
+ * (components interleaved, ncomp 1..4) is the conversion of the input pixel
+ * at the same index.  input->region must contain output->region; device
+ * views; strides in pixels.  Keeps a cast -> Gaussian -> cast chain
+ * resident on the GPU. */
+int itkg_convert_pixels(int32_t ndim, itkg_ptype in_type, itkg_ptype out_type, int32_t ncomp,
+                        const itkg_view* input, const itkg_view* output, itkg_convert_mode mode,
+                        void* stream);
+
+/* The chain UnaryFilter(cast to f32) -> GaussianFilter -> UnaryFilter(cast)
+ * on dense HOST buffers in one GPU round trip, so PCIe moves the pixels in
+ * their own type (1 byte each for u8) instead of f32: `input` (in_type)
+ * holds input_region_for_output(requested) of the f32 Gaussian, converted
+ * with ITKG_CONVERT_AS; `output` (out_type) receives `requested`, converted
+ * with out_mode.  Scalar pixels; arith as itkg_gaussian_generate_region.
+ * Returns when the output is written. */
+int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype out_type,
+                                 double sigma, const double* spacing, const itkg_region* largest,
+                                 const void* input, const itkg_region* requested, void* output,
+                                 itkg_arith arith, itkg_convert_mode out_mode);
+
 /* Scratch memory of the entry points (per-axis intermediates, staging,
  * work images) comes from a stream-ordered pool per device that keeps up to
  * 8 GiB of freed memory between calls (environment ITKG_POOL_KEEP_BYTES sets

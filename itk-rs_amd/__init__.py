@@ -17,6 +17,7 @@ import os
 from dataclasses import dataclass, field
 from typing import Iterator, List, Optional, Sequence, Tuple
 
+import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -30,6 +31,8 @@ F32, F64 = 0, 1
 EXACT, FAST = 0, 1
 PATH_AUTO, PATH_PER_AXIS, PATH_FUSED = 0, 1, 2
 BATCH_FORWARD, BATCH_BACKWARD = 0, 1
+PT_U8, PT_U16, PT_I16, PT_I32, PT_F32, PT_F64 = 0, 1, 2, 3, 4, 5
+CONVERT_AS, CONVERT_ROUND = 0, 1
 
 # Every symbol include/itkg.h declares (checked by the test suite).
 EXPORTS = (
@@ -45,6 +48,7 @@ EXPORTS = (
     "itkg_unsharp_mask_generate_region_host", "itkg_yvv_coefficients",
     "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
     "itkg_canny_generate_region", "itkg_canny_generate_region_host", "itkg_trim_scratch",
+    "itkg_convert_pixels", "itkg_cast_gaussian_cast_host",
 )
 
 
@@ -130,6 +134,11 @@ def _load():
     L.itkg_last_error.restype = ctypes.c_char_p
     L.itkg_set_path.argtypes = [ctypes.c_int]
     L.itkg_trim_scratch.argtypes = [ctypes.c_uint64]
+    L.itkg_convert_pixels.argtypes = [ctypes.c_int32, ctypes.c_int, ctypes.c_int, ctypes.c_int32,
+                                      P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
+    L.itkg_cast_gaussian_cast_host.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_int, ctypes.c_double, P(ctypes.c_double),
+        P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.itkg_version.restype = ctypes.c_char_p
     return L
 
@@ -293,7 +302,9 @@ class Image:
         """Strided view of this image's pixels (whole buffer by default)."""
         t = self.data
         D = self.ndim
-        _, nc = _pixel_info(t, D)
+        if t.dim() not in (D, D + 1):
+            raise ValueError(f"tensor of rank {t.dim()} does not hold a {D}-D image")
+        nc = 1 if t.dim() == D else int(t.shape[-1])
         if nc > 1 and t.stride(-1) != 1:
             raise ValueError("vector pixels must have contiguous components")
         v = c_view()
@@ -654,6 +665,86 @@ class BinomialBlurFilter(ImageSource):
         out = Image.empty(requested, self.spacing, self.origin, nc, inp.data.dtype, inp.data.device)
         self.generate_into(inp, out)
         return out
+
+
+# ---------------------------------------------------------------------------
+# UnaryFilter casts (src/filters/mod.rs:47-80) on the GPU
+# ---------------------------------------------------------------------------
+
+_PTYPES = {torch.uint8: PT_U8, torch.int16: PT_I16, torch.int32: PT_I32,
+           torch.float32: PT_F32, torch.float64: PT_F64}
+if hasattr(torch, "uint16"):
+    _PTYPES[torch.uint16] = PT_U16
+_NP_PTYPES = {np.dtype(np.uint8): PT_U8, np.dtype(np.uint16): PT_U16, np.dtype(np.int16): PT_I16,
+              np.dtype(np.int32): PT_I32, np.dtype(np.float32): PT_F32,
+              np.dtype(np.float64): PT_F64}
+
+
+def _ptype(dtype) -> int:
+    if dtype not in _PTYPES:
+        raise TypeError(f"no GPU conversion for {dtype}")
+    return _PTYPES[dtype]
+
+
+class CastFilter(ImageSource):
+    """UnaryFilter<S, F> (src/filters/mod.rs:47-80) whose closure is a cast,
+    on the GPU: `mode` CONVERT_AS is Rust's `p as T` (e.g. `|p: u8| p as
+    f32`), CONVERT_ROUND is `p.round() ... as T` (e.g. `|p: f32|
+    p.round().clamp(0.0, 255.0) as u8`, examples/blur_png.rs:35-37).  Same
+    region, spacing and origin as the upstream; a chain of these around a
+    GaussianFilter stays resident on the device."""
+
+    def __init__(self, source, dtype: torch.dtype, mode: int = CONVERT_AS):
+        self.source = source
+        self.dtype = dtype
+        self.mode = int(mode)
+
+    def largest_region(self) -> Region:
+        return self.source.largest_region()
+
+    @property
+    def spacing(self):
+        return tuple(self.source.spacing)
+
+    @property
+    def origin(self):
+        return tuple(self.source.origin)
+
+    def input_region_for_output(self, output_region: Region) -> Region:
+        return output_region
+
+    def generate_into(self, inp: Image, out: Image) -> None:
+        D = out.ndim
+        nc = 1 if inp.data.dim() == D else int(inp.data.shape[-1])
+        with torch.cuda.device(inp.data.device):
+            _check(lib.itkg_convert_pixels(D, _ptype(inp.data.dtype), _ptype(out.data.dtype), nc,
+                                           ctypes.byref(inp.view()), ctypes.byref(out.view()),
+                                           self.mode, _stream_ptr(inp.data.device)))
+
+    def generate_region(self, requested: Region) -> Image:
+        inp = self.source.generate_region(requested)
+        D = requested.ndim
+        nc = 1 if inp.data.dim() == D else int(inp.data.shape[-1])
+        out = Image.empty(inp.region, self.spacing, self.origin, nc, self.dtype, inp.data.device)
+        self.generate_into(inp, out)
+        return out
+
+
+def cast_gaussian_cast_host(x: np.ndarray, sigma: float, out_dtype, spacing=None,
+                            arith: int = EXACT, out_mode: int = CONVERT_ROUND) -> np.ndarray:
+    """UnaryFilter(x as f32) -> GaussianFilter(sigma) -> UnaryFilter(cast to
+    out_dtype) over a whole host image in one GPU round trip
+    (itkg_cast_gaussian_cast_host): PCIe carries the pixels in their own
+    types."""
+    x = np.ascontiguousarray(x)
+    D = x.ndim
+    sp = tuple(spacing) if spacing is not None else (1.0,) * D
+    reg = Region([0] * D, list(reversed(x.shape))).to_c()
+    out = np.empty(x.shape, dtype=out_dtype)
+    _check(lib.itkg_cast_gaussian_cast_host(
+        D, _NP_PTYPES[x.dtype], _NP_PTYPES[out.dtype], float(sigma), _doubles(sp),
+        ctypes.byref(reg), x.ctypes.data, ctypes.byref(reg), out.ctypes.data, arith, out_mode))
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -808,6 +808,41 @@ int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, 
     return rc;
 }
 
+bool ptype_ok(itkg_ptype t) { return t >= ITKG_PT_U8 && t <= ITKG_PT_F64; }
+
+size_t ptype_size(itkg_ptype t) {
+    switch (t) {
+        case ITKG_PT_U8: return 1;
+        case ITKG_PT_U16:
+        case ITKG_PT_I16: return 2;
+        case ITKG_PT_I32:
+        case ITKG_PT_F32: return 4;
+        default: return 8;
+    }
+}
+
+// Enqueues the conversion of region `reg` from view `in` to view `out`.
+int convert_views(int ndim, itkg_ptype in_type, itkg_ptype out_type, int ncomp,
+                  const itkg_view& in, const itkg_view& out, const itkg_region& reg,
+                  itkg_convert_mode mode, hipStream_t s) {
+    ConvertArgs a{};
+    int64_t at[ITKG_MAX_DIM] = {0, 0, 0, 0};
+    for (int d = 0; d < ndim; ++d) at[d] = reg.index[d];
+    a.in = static_cast<const char*>(in.data) + view_offset(ndim, in, at) * ncomp * (int64_t)ptype_size(in_type);
+    a.out = static_cast<char*>(out.data) + view_offset(ndim, out, at) * ncomp * (int64_t)ptype_size(out_type);
+    for (int d = 0; d < ITKG_MAX_DIM; ++d) {
+        a.n[d] = d < ndim ? reg.size[d] : 1;
+        a.in_stride[d] = d < ndim ? in.stride[d] : 0;
+        a.out_stride[d] = d < ndim ? out.stride[d] : 0;
+    }
+    a.ncomp = ncomp;
+    a.in_type = in_type;
+    a.out_type = out_type;
+    a.mode = mode;
+    const hipError_t e = launch_convert(a, s);
+    return e == hipSuccess ? ITKG_OK : hip_fail(e, "conversion launch");
+}
+
 bool args_ok(int ndim, itkg_dtype dt, int ncomp) {
     return ndim >= 1 && ndim <= ITKG_MAX_DIM && (dt == ITKG_F32 || dt == ITKG_F64) && ncomp >= 1 &&
            ncomp <= 4;
@@ -1448,6 +1483,72 @@ int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double
     if ((rc = check_view(ndim, grad_input, p.in_reg, "grad_input"))) return rc;
     return backward_impl(p, dtype, ncomp, *grad_output, *grad_input, accumulate,
                          static_cast<hipStream_t>(stream), nullptr);
+}
+
+int itkg_convert_pixels(int32_t ndim, itkg_ptype in_type, itkg_ptype out_type, int32_t ncomp,
+                        const itkg_view* input, const itkg_view* output, itkg_convert_mode mode,
+                        void* stream) {
+    clear_error();
+    if (ndim < 1 || ndim > ITKG_MAX_DIM || ncomp < 1 || ncomp > 4 || !ptype_ok(in_type) ||
+        !ptype_ok(out_type) || (mode != ITKG_CONVERT_AS && mode != ITKG_CONVERT_ROUND))
+        return fail(ITKG_ERR_INVALID_ARG, "bad ndim/type/ncomp/mode");
+    if (!output) return fail(ITKG_ERR_INVALID_ARG, "null output view");
+    int rc = check_view(ndim, output, output->region, "output");
+    if (!rc) rc = check_view(ndim, input, output->region, "input");
+    if (rc) return rc;
+    return convert_views(ndim, in_type, out_type, ncomp, *input, *output, output->region, mode,
+                         static_cast<hipStream_t>(stream));
+}
+
+int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype out_type,
+                                 double sigma, const double* spacing, const itkg_region* largest,
+                                 const void* input, const itkg_region* requested, void* output,
+                                 itkg_arith arith, itkg_convert_mode out_mode) {
+    clear_error();
+    if (!args_ok(ndim, ITKG_F32, 1) || !ptype_ok(in_type) || !ptype_ok(out_type) ||
+        (out_mode != ITKG_CONVERT_AS && out_mode != ITKG_CONVERT_ROUND))
+        return fail(ITKG_ERR_INVALID_ARG, "bad ndim/type/mode");
+    if (!requested) return fail(ITKG_ERR_INVALID_ARG, "null requested region");
+    Plan p;
+    int rc = make_plan(ndim, sigma, spacing, largest, *requested, &p);
+    if (rc) return rc;
+    const int64_t nin = region_len(ndim, p.in_reg), nout = region_len(ndim, p.req);
+    if (nout == 0) return ITKG_OK;
+    if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null buffer");
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    // one block: input pixels, f32 input, f32 output, output pixels
+    const size_t bi = (size_t)nin * ptype_size(in_type), bf = (size_t)nin * 4,
+                 bo = (size_t)nout * 4, bq = (size_t)nout * ptype_size(out_type);
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    void* block = nullptr;
+    e = pool_alloc(&block, up(bi) + up(bf) + up(bo) + up(bq), s);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return hip_fail(e, "pool_alloc(cast staging)");
+    }
+    char* base = static_cast<char*>(block);
+    void* din = base;
+    void* dfi = base + up(bi);
+    void* dfo = base + up(bi) + up(bf);
+    void* dq = base + up(bi) + up(bf) + up(bo);
+    const itkg_view vin = dense_view(ndim, din, p.in_reg), vfi = dense_view(ndim, dfi, p.in_reg);
+    const itkg_view vfo = dense_view(ndim, dfo, p.req), vq = dense_view(ndim, dq, p.req);
+    e = hipMemcpyAsync(din, input, bi, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync(input)");
+    if (!rc) rc = convert_views(ndim, in_type, ITKG_PT_F32, 1, vin, vfi, p.in_reg, ITKG_CONVERT_AS, s);
+    if (!rc) rc = forward_impl(p, ITKG_F32, 1, vfi, vfo, arith, s, nullptr);
+    if (!rc) rc = convert_views(ndim, ITKG_PT_F32, out_type, 1, vfo, vq, p.req, out_mode, s);
+    if (!rc) {
+        e = hipMemcpyAsync(output, dq, bq, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync(output)");
+    }
+    (void)hipFreeAsync(block, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
+    return rc;
 }
 
 int itkg_trim_scratch(uint64_t keep_bytes) {

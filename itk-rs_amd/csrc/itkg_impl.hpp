@@ -99,6 +99,20 @@ struct CannyArgs {
 template <typename T>
 hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s);
 
+// Pixel conversion over a region of sizes n (1 beyond ndim); strides in
+// pixels, ncomp interleaved components.
+struct ConvertArgs {
+    const void* in;
+    void* out;
+    int64_t n[4];
+    int64_t in_stride[4], out_stride[4];
+    int32_t ncomp;
+    itkg_ptype in_type, out_type;
+    itkg_convert_mode mode;
+};
+hipError_t launch_convert(const ConvertArgs& a, hipStream_t s);
+size_t ptype_size(itkg_ptype t);
+
 // Fused one-round-trip 3-D kernel for f32 scalar pixels (forward or adjoint).
 // Destination pixel j reads source pixels j + shift + k - r, k = 0..2r
 // (forward: clamped into the source; adjoint: zero outside it).

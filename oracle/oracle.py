@@ -500,3 +500,50 @@ def apply_dense(mats, x: np.ndarray, transpose=False) -> np.ndarray:
         np_axis = D - 1 - a
         y = np.moveaxis(np.tensordot(M, y, axes=([1], [np_axis])), 0, np_axis)
     return y
+
+
+# ---------------------------------------------------------------------------
+# UnaryFilter casts (src/filters/mod.rs:47-80; examples/blur_png.rs:35-37)
+# ---------------------------------------------------------------------------
+
+def rust_round(x: np.ndarray) -> np.ndarray:
+    """f32::round / f64::round: nearest integer, halves away from zero (NaN
+    and infinities unchanged).  x - trunc(x) is exact, so is the test."""
+    x = np.asarray(x)
+    t = np.trunc(x)
+    with np.errstate(invalid="ignore"):
+        d = x - t
+        # select rather than add, so a zero result keeps its sign
+        # (round(-0.3) is -0.0)
+        r = np.where(d >= 0.5, t + 1, np.where(d <= -0.5, t - 1, t))
+    return np.where(np.isfinite(x), r, x).astype(x.dtype)
+
+
+def rust_as(x: np.ndarray, dtype) -> np.ndarray:
+    """Rust's `x as T` for numeric x: integer -> integer keeps the low bits
+    (narrowing) or extends by the source's signedness (widening); integer
+    and float -> float round to nearest even; float -> integer truncates
+    toward zero, saturates at T's range, and maps NaN to 0."""
+    x = np.asarray(x)
+    dt = np.dtype(dtype)
+    if np.issubdtype(x.dtype, np.floating) and np.issubdtype(dt, np.integer):
+        info = np.iinfo(dt)
+        with np.errstate(invalid="ignore"):
+            v = np.trunc(x.astype(np.float64))
+            v = np.where(np.isnan(v), 0.0, np.clip(v, info.min, info.max))
+        return v.astype(dt)
+    return x.astype(dt)
+
+
+def blur_png_chain(img_u8: np.ndarray, sigma: float, spacing=None) -> np.ndarray:
+    """examples/blur_png.rs:35-40 on the C oracle: `p as f32`, the whole-image
+    GaussianFilter, `p.round().clamp(0.0, 255.0) as u8`."""
+    D = img_u8.ndim
+    sp = list(spacing) if spacing is not None else [1.0] * D
+    f = rust_as(img_u8, np.float32)
+    size = list(reversed(img_u8.shape))
+    g = generate_region(f, [0] * D, sp, sigma, [0] * D, size)
+    r = rust_round(g)
+    with np.errstate(invalid="ignore"):
+        c = np.where(np.isnan(r), r, np.clip(r, np.float32(0.0), np.float32(255.0)))
+    return rust_as(c.astype(np.float32), np.uint8)

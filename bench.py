@@ -20,7 +20,11 @@ reference, the accuracy the target is stated at).  Also reported (under
   - 512^2 2-D and the 256x256x160 VecPixel<f32,3> configuration,
   - MeanFilter (radius 2, FAST and EXACT), BinomialBlurFilter (4
     repetitions), UnsharpMaskFilter (sigma 2) and RecursiveGaussianFilter
-    (sigma 2) on a 512^3 volume; CannyEdgeDetectionFilter on a 2048^2 image.
+    (sigma 2) on a 512^3 volume; CannyEdgeDetectionFilter on a 2048^2 image,
+  - host buffers end to end: the host-buffer entry point on the 512^3
+    volume through the pinned staging ring and through pageable copies, and
+    the u8 -> f32 -> Gaussian -> u8 chain of examples/blur_png.rs on a 512^3
+    u8 volume.
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
 write) / the average kernel duration from HIP events on the launch stream,
@@ -59,7 +63,7 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="headline loop only (profiling)")
     ap.add_argument("--extras", default="",
                     help="comma list of extras groups to run (default all): "
-                         "c2, c5, c4, c5_batch, c1_c3, filters")
+                         "c2, c5, c4, c5_batch, c1_c3, filters, host")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--keep-gc", action="store_true",
                     help="leave Python's garbage collector as it is (no collection before a "
@@ -532,6 +536,60 @@ def main():
                                       "edge_pixels": int(oc.data.sum().item())}
             del xc, oc, ic, disc, xx, yy
             torch.cuda.empty_cache()
+
+        if want("host"):
+            # host buffers end to end (the drop-in's itkg_gaussian_generate_region_host
+            # on the whole 512^3 f32 volume: 512 MiB in over PCIe, the kernels,
+            # 512 MiB out), through the pinned staging ring and through pageable
+            # copies; then the examples/blur_png.rs chain (u8 -> f32, Gaussian,
+            # round -> u8) on a 512^3 u8 volume, host to host
+            import ctypes
+            hx = x.cpu().numpy()
+            hy = np.empty_like(hx)
+            reg_c = itk.Region([0, 0, 0], [512] * 3).to_c()
+            sp_c = (ctypes.c_double * 3)(*sp)
+
+            def host_call():
+                rc = itk.lib.itkg_gaussian_generate_region_host(
+                    3, itk.F32, 1, float(sigma), sp_c, ctypes.byref(reg_c), hx.ctypes.data,
+                    ctypes.byref(reg_c), hy.ctypes.data, arith)
+                if rc:
+                    raise RuntimeError(itk.lib.itkg_last_error())
+
+            def host_times(fn, reps):
+                fn()  # creates the staging ring / maps the pool on the first call
+                ts = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    fn()
+                    ts.append(time.perf_counter() - t0)
+                return ts
+
+            hres = {}
+            for mode in ("pinned", "pageable"):
+                if mode == "pageable":
+                    os.environ["ITKG_HOST_STAGING"] = "pageable"
+                else:
+                    os.environ.pop("ITKG_HOST_STAGING", None)
+                ts = host_times(host_call, 5)
+                hres[mode] = {"ms_median": round(float(np.median(ts)) * 1e3, 2),
+                              "ms_min": round(float(np.min(ts)) * 1e3, 2),
+                              "pcie_gb_s_both_ways": round(2 * 4.0 * n / float(np.median(ts))
+                                                           / 1e9, 1),
+                              "staged_pinned": itk.host_staging_pinned()}
+            os.environ.pop("ITKG_HOST_STAGING", None)
+            hres["bit_equal_to_device"] = bool(np.array_equal(
+                hy, itk.gaussian(x, sigma, sp, arith=arith).cpu().numpy()))
+            extras["host_to_host_512_f32"] = hres
+            del hx, hy
+            u8 = (torch.rand((512, 512, 512), device=dev) * 255).to(torch.uint8).cpu().numpy()
+            ts = host_times(lambda: itk.cast_gaussian_cast_host(u8, sigma, np.uint8, arith=arith), 5)
+            extras["host_to_host_512_u8_chain"] = {
+                "ms_median": round(float(np.median(ts)) * 1e3, 2),
+                "ms_min": round(float(np.min(ts)) * 1e3, 2),
+                "gvox_s": round(n / float(np.median(ts)) / 1e9, 3)}
+            del u8
+            itk.trim_scratch(0)
 
         if KEEP_GC:
             extras["gc_pauses_ms"] = {"count": len(GC_PAUSES_MS),

@@ -377,9 +377,24 @@ int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype ou
  * 8 GiB of freed memory between calls (environment ITKG_POOL_KEEP_BYTES sets
  * another limit).  This hands the current device's idle pool memory beyond
  * keep_bytes back to the driver, e.g. before allocating through another
- * allocator (torch's cannot see the pool).  No counterpart in the reference,
- * which allocates host Vecs per call. */
+ * allocator (torch's cannot see the pool), and frees the pinned host staging
+ * ring of the _host entry points (72 MiB) unless a call is using it; the
+ * next _host call creates it again.  No counterpart in the reference, which
+ * allocates host Vecs per call. */
 int itkg_trim_scratch(uint64_t keep_bytes);
+
+/* Host staging of the _host entry points: requests above 32 MiB are split
+ * into pieces along the slowest axis, and the input and output move in
+ * 12 MiB chunks, overlapped with each other and with the pieces' kernels,
+ * through a process-wide ring of pinned host slots (3 per direction) that
+ * 4 copy threads per direction fill and empty (ITKG_STAGE_CHUNK_MIB,
+ * ITKG_STAGE_SLOTS and ITKG_STAGE_THREADS set others).  One call at a time uses
+ * the ring; a call that finds it in use, or any call when the environment
+ * sets ITKG_HOST_STAGING=pageable, copies straight from and to the caller's
+ * pageable memory instead.  Results are the same either way.  This reports
+ * whether the last such call went through the ring (1) or not (0).  No
+ * counterpart in the reference (host Vecs, no device). */
+int itkg_host_staging_info(int32_t* last_pinned);
 
 /* Library version string. */
 const char* itkg_version(void);

@@ -48,7 +48,7 @@ EXPORTS = (
     "itkg_unsharp_mask_generate_region_host", "itkg_yvv_coefficients",
     "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
     "itkg_canny_generate_region", "itkg_canny_generate_region_host", "itkg_trim_scratch",
-    "itkg_convert_pixels", "itkg_cast_gaussian_cast_host",
+    "itkg_convert_pixels", "itkg_cast_gaussian_cast_host", "itkg_host_staging_info",
 )
 
 
@@ -134,6 +134,7 @@ def _load():
     L.itkg_last_error.restype = ctypes.c_char_p
     L.itkg_set_path.argtypes = [ctypes.c_int]
     L.itkg_trim_scratch.argtypes = [ctypes.c_uint64]
+    L.itkg_host_staging_info.argtypes = [P(ctypes.c_int32)]
     L.itkg_convert_pixels.argtypes = [ctypes.c_int32, ctypes.c_int, ctypes.c_int, ctypes.c_int32,
                                       P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
     L.itkg_cast_gaussian_cast_host.argtypes = [
@@ -162,6 +163,14 @@ def trim_scratch(keep_bytes: int = 0, device=None) -> None:
     allocates large tensors (torch's allocator cannot see that pool)."""
     with torch.cuda.device(device if device is not None else torch.cuda.current_device()):
         _check(lib.itkg_trim_scratch(int(keep_bytes)))
+
+
+def host_staging_pinned() -> bool:
+    """Whether the last staged *_host call went through the library's pinned
+    staging ring (False: it copied from and to pageable memory)."""
+    v = ctypes.c_int32()
+    _check(lib.itkg_host_staging_info(ctypes.byref(v)))
+    return bool(v.value)
 
 
 # ---------------------------------------------------------------------------

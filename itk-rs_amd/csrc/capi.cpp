@@ -46,6 +46,8 @@ int hip_fail(hipError_t e, const char* what) {
 
 void clear_error() { g_err.clear(); }
 
+std::string last_error_text() { return g_err; }
+
 // Scratch memory of the entry points comes from a stream-ordered pool per
 // device that the library owns and that keeps up to kPoolKeepBytes of freed
 // memory across synchronisations (ITKG_POOL_KEEP_BYTES overrides it).  A
@@ -636,20 +638,58 @@ int host_forward_serial(const Plan& p, itkg_dtype dtype, int ncomp, const void* 
 }
 
 // Host buffers, pipelined (src/source.rs:81-97 applied to the transfers):
-// the request is split into pieces along its slowest axis.  One helper
-// thread copies the input region in, chunk by chunk along that axis; the
-// calling thread launches each piece's kernels on a compute stream once the
-// input planes the piece reads have arrived; a second helper thread copies
-// each finished piece out.  A copy from or to pageable memory holds its
-// calling thread until it is done, so the two directions get threads of
-// their own, and the inbound and outbound copies overlap.  The hand-offs are
-// ordered on the host: a chunk's copy has completed before the chunk is
-// announced, and a piece's kernels have completed before its copy out
-// starts (free here: a pageable copy holds its thread until it is done).  Each piece is
-// computed exactly as a streamed piece, so the output is bit-identical to
-// the serial form.  Pieces hold about kHostPieceBytes of output.
+// the request is split into pieces along its slowest axis, and the staging
+// of host_stage.cpp moves the input in and the output out in chunks while
+// the pieces run.  Piece k's kernels wait for the input planes the piece
+// reads; each piece is computed exactly as a streamed piece, so the output
+// is bit-identical to the serial form.  Pieces hold about kHostPieceBytes of
+// the larger of the two transfers.
 constexpr int64_t kHostPieceBytes = 16ll << 20;
 constexpr int kHostMaxPieces = 32;
+
+struct HostPieces {
+    std::vector<Plan> piece;
+    std::vector<int64_t> in_planes_end;   // input planes (from p.in_reg) piece k reads: [0, end)
+    std::vector<int64_t> out_planes_end;  // output planes (from p.req) done after piece k
+    int64_t in_plane = 0, out_plane = 0;  // pixels per plane
+};
+
+// Splits p along its slowest axis into pieces of about kHostPieceBytes of
+// the larger transfer (in_px / out_px bytes per pixel); 1 piece when the
+// request is small.
+int host_pieces(const Plan& p, size_t in_px, size_t out_px, HostPieces* hp) {
+    const int D = p.D, ax = D - 1;
+    const int64_t nin = region_len(D, p.in_reg), nout = region_len(D, p.req);
+    const int64_t in_planes = p.in_reg.size[ax], out_planes = p.req.size[ax];
+    const int64_t big = std::max<int64_t>(nin * (int64_t)in_px, nout * (int64_t)out_px);
+    const int K = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)kHostMaxPieces, out_planes, in_planes,
+                              big / kHostPieceBytes}));
+    hp->in_plane = nin / in_planes;
+    hp->out_plane = nout / out_planes;
+    hp->piece.assign(K, p);
+    hp->in_planes_end.resize(K);
+    hp->out_planes_end.resize(K);
+    for (int k = 0; k < K; ++k) {
+        const int64_t o0 = out_planes * k / K, o1 = out_planes * (k + 1) / K;
+        itkg_region r = p.req;
+        r.index[ax] = p.req.index[ax] + o0;
+        r.size[ax] = o1 - o0;
+        if (int rc = plan_regions(D, &p.bounds, r, &hp->piece[k])) return rc;
+        const Plan& q = hp->piece[k];
+        hp->in_planes_end[k] = q.in_reg.index[ax] + q.in_reg.size[ax] - p.in_reg.index[ax];
+        hp->out_planes_end[k] = o1;
+    }
+    return ITKG_OK;
+}
+
+StageStats g_last_stage;  // of the last staged call (read by itkg_host_staging_info)
+std::mutex g_last_stage_mu;
+
+void note_stage(const StageStats& st) {
+    std::lock_guard<std::mutex> lk(g_last_stage_mu);
+    g_last_stage = st;
+}
 
 int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, void* output,
                  itkg_arith arith) {
@@ -658,153 +698,50 @@ int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, 
     if (nout == 0) return ITKG_OK;
     if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
     const size_t px = dtype_size(dtype) * ncomp;
-    const int ax = D - 1;
-    const int64_t in_planes = p.in_reg.size[ax], out_planes = p.req.size[ax];
-    const int K = (int)std::min<int64_t>(
-        {(int64_t)kHostMaxPieces, out_planes, in_planes,
-         std::max<int64_t>(1, (nout * (int64_t)px) / kHostPieceBytes)});
+    HostPieces hp;
+    if (int rc = host_pieces(p, px, px, &hp)) return rc;
+    const int K = (int)hp.piece.size();
     if (K <= 1) return host_forward_serial(p, dtype, ncomp, input, output, arith);
-    const int64_t in_plane = nin / in_planes, out_plane = nout / out_planes;  // pixels per plane
 
-    // chunk j of the input planes: [ib[j], ib[j+1]); piece k of the request:
-    // output planes [ob[k], ob[k+1])
-    std::vector<int64_t> ib(K + 1), ob(K + 1);
-    for (int j = 0; j <= K; ++j) {
-        ib[j] = in_planes * j / K;
-        ob[j] = out_planes * j / K;
-    }
-    std::vector<Plan> piece(K);
-    std::vector<int> need(K);  // last input chunk a piece reads
-    for (int k = 0; k < K; ++k) {
-        itkg_region r = p.req;
-        r.index[ax] = p.req.index[ax] + ob[k];
-        r.size[ax] = ob[k + 1] - ob[k];
-        piece[k] = p;
-        if (int rc = plan_regions(D, &p.bounds, r, &piece[k])) return rc;
-        const int64_t last = piece[k].in_reg.index[ax] + piece[k].in_reg.size[ax] - 1 - p.in_reg.index[ax];
-        int j = 0;
-        while (j + 1 < K && ib[j + 1] <= last) ++j;
-        need[k] = j;
-    }
-
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-    hipStream_t s_in = nullptr, s_comp = nullptr, s_out = nullptr;
-    std::vector<hipEvent_t> ev_comp(K, nullptr);
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     void *din = nullptr, *dout = nullptr;
+    e = pool_alloc(&din, (size_t)nin * px, s);
+    if (e == hipSuccess) e = pool_alloc(&dout, (size_t)nout * px, s);
+    // the copy streams use the buffers: in place before they start
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     int rc = ITKG_OK;
-    auto cleanup = [&]() {
-        for (hipStream_t st : {s_in, s_comp, s_out})
-            if (st) (void)hipStreamSynchronize(st);
-        if (din) (void)hipFree(din);
-        if (dout) (void)hipFree(dout);
-        for (auto ev : ev_comp)
-            if (ev) (void)hipEventDestroy(ev);
-        for (hipStream_t st : {s_in, s_comp, s_out})
-            if (st) (void)hipStreamDestroy(st);
-    };
-    for (hipStream_t* st : {&s_in, &s_comp, &s_out})
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
-    for (int k = 0; k < K && e == hipSuccess; ++k)
-        e = hipEventCreateWithFlags(&ev_comp[k], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMalloc(&din, (size_t)nin * px);
-    if (e == hipSuccess) e = hipMalloc(&dout, (size_t)nout * px);
     if (e != hipSuccess) {
-        rc = hip_fail(e, "host staging set-up");
-        cleanup();
-        return rc;
+        rc = hip_fail(e, "pool_alloc(host staging)");
+    } else {
+        StagedJob job;
+        job.host_in = input;
+        job.in_bytes = (size_t)nin * px;
+        job.host_out = output;
+        job.out_bytes = (size_t)nout * px;
+        job.dev_in = din;
+        job.dev_out = dout;
+        job.pieces = K;
+        for (int k = 0; k < K; ++k) {
+            job.in_need.push_back((size_t)(hp.in_planes_end[k] * hp.in_plane) * px);
+            job.out_end.push_back((size_t)(hp.out_planes_end[k] * hp.out_plane) * px);
+        }
+        const itkg_view vin = dense_view(D, din, p.in_reg);
+        const itkg_view vout = dense_view(D, dout, p.req);
+        job.compute = [&](int k, hipStream_t st) {
+            return forward_impl(hp.piece[k], dtype, ncomp, vin, vout, arith, st, nullptr);
+        };
+        StageStats stats;
+        job.stats = &stats;
+        rc = run_staged(job, s);
+        note_stage(stats);
     }
-
-    // progress shared with the helper threads
-    std::mutex mu;
-    std::condition_variable cv;
-    int in_done = 0, comp_done = 0;
-    bool abort = false;
-    int rc_in = ITKG_OK, rc_out = ITKG_OK;
-    std::string err_in, err_out;
-    auto publish = [&](int& counter, int value) {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            counter = value;
-        }
-        cv.notify_all();
-    };
-    auto stop = [&]() {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            abort = true;
-        }
-        cv.notify_all();
-    };
-    // waits until counter > idx or abort; returns false on abort
-    auto wait_for = [&](const int& counter, int idx) {
-        std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [&] { return abort || counter > idx; });
-        return !abort;
-    };
-    const char* src = static_cast<const char*>(input);
-    char* dst = static_cast<char*>(output);
-
-    std::thread t_in([&]() {
-        hipError_t te = hipSetDevice(dev);
-        for (int j = 0; j < K && te == hipSuccess; ++j) {
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                if (abort) return;
-            }
-            const size_t off = (size_t)(ib[j] * in_plane) * px;
-            const size_t bytes = (size_t)((ib[j + 1] - ib[j]) * in_plane) * px;
-            te = hipMemcpyAsync(static_cast<char*>(din) + off, src + off, bytes,
-                                hipMemcpyHostToDevice, s_in);
-            // complete before it is announced; the kernels that read it are
-            // launched after that
-            if (te == hipSuccess) te = hipStreamSynchronize(s_in);
-            if (te == hipSuccess) publish(in_done, j + 1);
-        }
-        if (te != hipSuccess) {
-            rc_in = hip_fail(te, "copy in");
-            err_in = g_err;
-            stop();
-        }
-    });
-    std::thread t_out([&]() {
-        hipError_t te = hipSetDevice(dev);
-        for (int k = 0; k < K && te == hipSuccess; ++k) {
-            if (!wait_for(comp_done, k)) return;
-            const size_t off = (size_t)(ob[k] * out_plane) * px;
-            const size_t bytes = (size_t)((ob[k + 1] - ob[k]) * out_plane) * px;
-            // the piece's kernels have finished before its copy is issued
-            te = hipEventSynchronize(ev_comp[k]);
-            if (te == hipSuccess)
-                te = hipMemcpyAsync(dst + off, static_cast<char*>(dout) + off, bytes,
-                                    hipMemcpyDeviceToHost, s_out);
-            if (te == hipSuccess) te = hipStreamSynchronize(s_out);
-        }
-        if (te == hipSuccess) te = hipStreamSynchronize(s_out);
-        if (te != hipSuccess) {
-            rc_out = hip_fail(te, "copy out");
-            err_out = g_err;
-            stop();
-        }
-    });
-    const itkg_view vin = dense_view(D, din, p.in_reg);
-    const itkg_view vout = dense_view(D, dout, p.req);
-    for (int k = 0; k < K && !rc; ++k) {
-        if (!wait_for(in_done, need[k])) break;
-        rc = forward_impl(piece[k], dtype, ncomp, vin, vout, arith, s_comp, nullptr);
-        if (!rc) {
-            e = hipEventRecord(ev_comp[k], s_comp);
-            if (e != hipSuccess) rc = hip_fail(e, "hipEventRecord");
-        }
-        if (!rc) publish(comp_done, k + 1);
-    }
-    if (rc) stop();
-    t_in.join();
-    t_out.join();
-    if (!rc && rc_in) rc = fail(rc_in, "%s", err_in.c_str());
-    if (!rc && rc_out) rc = fail(rc_out, "%s", err_out.c_str());
-    cleanup();
+    if (din) (void)hipFreeAsync(din, s);
+    if (dout) (void)hipFreeAsync(dout, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
     return rc;
 }
 
@@ -1515,16 +1452,23 @@ int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype ou
     const int64_t nin = region_len(ndim, p.in_reg), nout = region_len(ndim, p.req);
     if (nout == 0) return ITKG_OK;
     if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null buffer");
+    const size_t pin = ptype_size(in_type), pout = ptype_size(out_type);
+    HostPieces hp;
+    if ((rc = host_pieces(p, pin, pout, &hp))) return rc;
+    const int K = (int)hp.piece.size();
     hipStream_t s = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     // one block: input pixels, f32 input, f32 output, output pixels
-    const size_t bi = (size_t)nin * ptype_size(in_type), bf = (size_t)nin * 4,
-                 bo = (size_t)nout * 4, bq = (size_t)nout * ptype_size(out_type);
+    const size_t bi = (size_t)nin * pin, bf = (size_t)nin * 4, bo = (size_t)nout * 4,
+                 bq = (size_t)nout * pout;
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     void* block = nullptr;
     e = pool_alloc(&block, up(bi) + up(bf) + up(bo) + up(bq), s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
+        if (block) (void)hipFreeAsync(block, s);
+        (void)hipStreamSynchronize(s);
         (void)hipStreamDestroy(s);
         return hip_fail(e, "pool_alloc(cast staging)");
     }
@@ -1535,15 +1479,39 @@ int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype ou
     void* dq = base + up(bi) + up(bf) + up(bo);
     const itkg_view vin = dense_view(ndim, din, p.in_reg), vfi = dense_view(ndim, dfi, p.in_reg);
     const itkg_view vfo = dense_view(ndim, dfo, p.req), vq = dense_view(ndim, dq, p.req);
-    e = hipMemcpyAsync(din, input, bi, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync(input)");
-    if (!rc) rc = convert_views(ndim, in_type, ITKG_PT_F32, 1, vin, vfi, p.in_reg, ITKG_CONVERT_AS, s);
-    if (!rc) rc = forward_impl(p, ITKG_F32, 1, vfi, vfo, arith, s, nullptr);
-    if (!rc) rc = convert_views(ndim, ITKG_PT_F32, out_type, 1, vfo, vq, p.req, out_mode, s);
-    if (!rc) {
-        e = hipMemcpyAsync(output, dq, bq, hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync(output)");
+    StagedJob job;
+    job.host_in = input;
+    job.in_bytes = bi;
+    job.host_out = output;
+    job.out_bytes = bq;
+    job.dev_in = din;
+    job.dev_out = dq;
+    job.pieces = K;
+    for (int k = 0; k < K; ++k) {
+        job.in_need.push_back((size_t)(hp.in_planes_end[k] * hp.in_plane) * pin);
+        job.out_end.push_back((size_t)(hp.out_planes_end[k] * hp.out_plane) * pout);
     }
+    const int ax = ndim - 1;
+    int64_t converted = 0;  // input planes already converted to f32
+    job.compute = [&](int k, hipStream_t st) {
+        const Plan& q = hp.piece[k];
+        // the f32 input planes this piece reads that no earlier piece converted
+        itkg_region cr = p.in_reg;
+        cr.index[ax] = p.in_reg.index[ax] + converted;
+        cr.size[ax] = hp.in_planes_end[k] - converted;
+        int r = ITKG_OK;
+        if (cr.size[ax] > 0) {
+            r = convert_views(ndim, in_type, ITKG_PT_F32, 1, vin, vfi, cr, ITKG_CONVERT_AS, st);
+            converted = hp.in_planes_end[k];
+        }
+        if (!r) r = forward_impl(q, ITKG_F32, 1, vfi, vfo, arith, st, nullptr);
+        if (!r) r = convert_views(ndim, ITKG_PT_F32, out_type, 1, vfo, vq, q.req, out_mode, st);
+        return r;
+    };
+    StageStats stats;
+    job.stats = &stats;
+    rc = run_staged(job, s);
+    note_stage(stats);
     (void)hipFreeAsync(block, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
@@ -1553,7 +1521,17 @@ int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype ou
 
 int itkg_trim_scratch(uint64_t keep_bytes) {
     clear_error();
-    return trim_pool(keep_bytes);
+    int rc = trim_pool(keep_bytes);
+    if (!rc) rc = stage_trim();
+    return rc;
+}
+
+int itkg_host_staging_info(int32_t* last_pinned) {
+    clear_error();
+    if (!last_pinned) return fail(ITKG_ERR_INVALID_ARG, "null output");
+    std::lock_guard<std::mutex> lk(g_last_stage_mu);
+    *last_pinned = g_last_stage.pinned ? 1 : 0;
+    return ITKG_OK;
 }
 
 const char* itkg_last_error(void) { return g_err.c_str(); }

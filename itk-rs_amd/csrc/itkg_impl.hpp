@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -17,6 +18,8 @@ constexpr int kMaxTaps = 2 * ITKG_MAX_RADIUS + 1;
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 void clear_error();
+// The calling thread's current error text.
+std::string last_error_text();
 
 // Arguments of one clamp-to-edge pass along `axis` (the convolve_axis of
 // src/filters/conv.rs:11-41) from a strided source view to a strided
@@ -235,5 +238,30 @@ int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
                  const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch);
 int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout,
                   const itkg_view& gin, int accumulate, hipStream_t s, void* scratch);
+
+// A piecewise computation between dense host buffers and dense device
+// buffers of the same byte layout (host_stage.cpp).  Piece k reads the input
+// bytes [0, in_need[k]) and completes the output bytes [0, out_end[k])
+// (both non-decreasing in k); compute(k, s) enqueues it on s.
+struct StageStats {
+    bool pinned = false;  // staged through the pinned ring (else pageable)
+};
+struct StagedJob {
+    const void* host_in = nullptr;
+    size_t in_bytes = 0;
+    void* host_out = nullptr;
+    size_t out_bytes = 0;
+    void* dev_in = nullptr;
+    const void* dev_out = nullptr;
+    int pieces = 0;
+    std::vector<size_t> in_need, out_end;
+    std::function<int(int, hipStream_t)> compute;
+    StageStats* stats = nullptr;
+};
+// Runs the job with the compute on s_comp (a stream the caller owns) and
+// the copies on streams of its own; returns when the output is in host_out.
+int run_staged(const StagedJob& job, hipStream_t s_comp);
+// Frees the pinned staging ring if no call is using it.
+int stage_trim();
 
 }  // namespace itkg

@@ -299,6 +299,18 @@ int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double
                            const itkg_view* grad_output, const itkg_view* grad_input,
                            int32_t accumulate, void* stream);
 
+/* itkg_gaussian_backward on dense host buffers (axis 0 fastest, ncomp
+ * interleaved): grad_output covers `requested`, grad_input covers
+ * *grad_input_region, which must contain input_region_for_output(
+ * requested).  Only that subregion changes; with accumulate != 0 the result
+ * is added to grad_input's contents.  For a host-Vec pipeline node's
+ * backward; no counterpart in the reference. */
+int itkg_gaussian_backward_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                                const double* spacing, const itkg_region* largest,
+                                const void* grad_output, const itkg_region* requested,
+                                void* grad_input, const itkg_region* grad_input_region,
+                                int32_t accumulate);
+
 /* ---- batched many-volume form (no counterpart in the reference) -------- */
 
 /* One whole 3-D f32 scalar volume: output = GaussianFilter{source: input,

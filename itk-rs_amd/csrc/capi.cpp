@@ -153,6 +153,12 @@ bool region_ok(int D, const itkg_region& r) {
     return true;
 }
 
+bool same_region(int D, const itkg_region& a, const itkg_region& b) {
+    for (int d = 0; d < D; ++d)
+        if (a.index[d] != b.index[d] || a.size[d] != b.size[d]) return false;
+    return true;
+}
+
 // ---- kernel (src/filters/gaussian.rs:24-40) ------------------------------
 
 uint64_t radius_of(double sigma, double spacing) {
@@ -1420,6 +1426,62 @@ int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double
     if ((rc = check_view(ndim, grad_input, p.in_reg, "grad_input"))) return rc;
     return backward_impl(p, dtype, ncomp, *grad_output, *grad_input, accumulate,
                          static_cast<hipStream_t>(stream), nullptr);
+}
+
+int itkg_gaussian_backward_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                                const double* spacing, const itkg_region* largest,
+                                const void* grad_output, const itkg_region* requested,
+                                void* grad_input, const itkg_region* grad_input_region,
+                                int32_t accumulate) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (!requested || !grad_input_region) return fail(ITKG_ERR_INVALID_ARG, "null region");
+    Plan p;
+    int rc = make_plan(ndim, sigma, spacing, largest, *requested, &p);
+    if (rc) return rc;
+    itkg_region greg = *grad_input_region;
+    for (int d = ndim; d < ITKG_MAX_DIM; ++d) {
+        greg.index[d] = 0;
+        greg.size[d] = 1;
+    }
+    if (!region_ok(ndim, greg)) return fail(ITKG_ERR_INVALID_ARG, "grad_input region: negative size");
+    const int64_t nout = region_len(ndim, p.req), ngin = region_len(ndim, greg);
+    if (nout == 0) return ITKG_OK;
+    if (!contains(ndim, greg, p.in_reg))
+        return fail(ITKG_ERR_INVALID_ARG,
+                    "grad_input region does not cover input_region_for_output(requested)");
+    if (!grad_output || !grad_input) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
+    const size_t px = dtype_size(dtype) * ncomp;
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    void *dgo = nullptr, *dgi = nullptr;
+    e = pool_alloc(&dgo, (size_t)nout * px, s);
+    if (e == hipSuccess) e = pool_alloc(&dgi, (size_t)ngin * px, s);
+    if (e != hipSuccess) rc = hip_fail(e, "pool_alloc(backward staging)");
+    // the grad_input buffer goes in whole when the kernels add to it or when
+    // it is larger than the part they write (the rest comes back unchanged)
+    const bool whole = accumulate || !same_region(ndim, greg, p.in_reg);
+    if (!rc) {
+        e = hipMemcpyAsync(dgo, grad_output, (size_t)nout * px, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && whole)
+            e = hipMemcpyAsync(dgi, grad_input, (size_t)ngin * px, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy in");
+    }
+    if (!rc) {
+        const itkg_view vgo = dense_view(ndim, dgo, p.req), vgi = dense_view(ndim, dgi, greg);
+        rc = backward_impl(p, dtype, ncomp, vgo, vgi, accumulate ? 1 : 0, s, nullptr);
+    }
+    if (!rc) {
+        e = hipMemcpyAsync(grad_input, dgi, (size_t)ngin * px, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = hip_fail(e, "copy out");
+    }
+    if (dgo) (void)hipFreeAsync(dgo, s);
+    if (dgi) (void)hipFreeAsync(dgi, s);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
+    (void)hipStreamDestroy(s);
+    return rc;
 }
 
 int itkg_convert_pixels(int32_t ndim, itkg_ptype in_type, itkg_ptype out_type, int32_t ncomp,

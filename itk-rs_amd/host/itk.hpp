@@ -276,6 +276,31 @@ struct GaussianFilter {
             input.data.data(), &cr, out.data.data(), arith));
         return out;
     }
+
+    // Adjoint of generate_region(grad_output.region) (no counterpart in the
+    // reference): the gradient w.r.t. the source's pixels, on
+    // input_region_for_output(grad_output.region).
+    Image<P, D> backward(const Image<P, D>& grad_output) const {
+        Image<P, D> grad_input = Image<P, D>::allocate(
+            input_region_for_output(grad_output.region), spacing(), origin(), P{});
+        backward_into(grad_output, grad_input, false);
+        return grad_input;
+    }
+
+    // Writes (accumulate false) or adds (true) that gradient into
+    // grad_input, whose region must contain
+    // input_region_for_output(grad_output.region); the rest of grad_input
+    // is left as it is.  Streamed pieces accumulate into one image.
+    void backward_into(const Image<P, D>& grad_output, Image<P, D>& grad_input,
+                       bool accumulate) const {
+        const auto sp = spacing();
+        const itkg_region cb = source.largest_region().to_c(), cr = grad_output.region.to_c(),
+                          cg = grad_input.region.to_c();
+        check(itkg_gaussian_backward_host((int32_t)D, PixelTraits<P>::dtype,
+                                          PixelTraits<P>::ncomp, sigma, sp.data(), &cb,
+                                          grad_output.data.data(), &cr, grad_input.data.data(),
+                                          &cg, accumulate ? 1 : 0));
+    }
 };
 
 template <class S>

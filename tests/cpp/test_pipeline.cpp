@@ -283,6 +283,73 @@ static void test_recursive_gaussian() {
     CHECK(std::fabs(sum(oa.data) - 1.0f) < 0.02f, "single axis energy=%g", sum(oa.data));
 }
 
+// GaussianFilter::backward is the transpose of generate_region: column j of
+// the forward operator is the forward response to the impulse at source
+// pixel j, so (G^T y)_j = <G e_j, y>, built here from forward calls only.
+template <class T>
+static void check_backward_transpose(double sigma, Region<2> req) {
+    const std::array<uint64_t, 2> size{11, 9};
+    std::array<double, 2> one{1.0, 1.0}, zero{};
+    const auto largest = Region<2>::make({0, 0}, size);
+    Image<T, 2> y = Image<T, 2>::allocate(req, one, zero, T{});
+    uint32_t st = 12345u;
+    for (auto& v : y.data) {
+        st = st * 1664525u + 1013904223u;
+        v = (T)((st >> 8) * (1.0 / 16777216.0) - 0.5);
+    }
+    auto src = Image<T, 2>::allocate(largest, one, zero, T{});
+    GaussianFilter<Image<T, 2>> f(src, sigma);
+    const Image<T, 2> g = f.backward(y);
+    CHECK(g.region == f.input_region_for_output(req), "gradient region");
+    Image<T, 2>::for_each_index(largest, [&](const Index<2>& j) {
+        Image<T, 2> e = Image<T, 2>::allocate(largest, one, zero, T{});
+        e.set_pixel(j, (T)1);
+        GaussianFilter<Image<T, 2>> fe(e, sigma);
+        const Image<T, 2> col = fe.generate_region(req);
+        double want = 0.0, mag = 0.0;
+        for (size_t i = 0; i < col.data.size(); ++i) {
+            want += (double)col.data[i] * (double)y.data[i];
+            mag += std::fabs((double)col.data[i] * (double)y.data[i]);
+        }
+        const bool inside = g.region.contains(j);
+        const double got = inside ? (double)g.get_pixel(j) : 0.0;
+        const double tol = (sizeof(T) == 4 ? 2e-6 : 1e-14) * std::max(1.0, mag);
+        CHECK(std::fabs(got - want) <= tol, "(%lld,%lld): backward %.9g vs transpose %.9g",
+              (long long)j.v[0], (long long)j.v[1], got, want);
+    });
+}
+
+static void test_gaussian_backward_transpose() {
+    check_backward_transpose<float>(1.3, Region<2>::make({2, 1}, {6, 5}));
+    check_backward_transpose<float>(0.8, Region<2>::make({0, 0}, {11, 9}));
+    check_backward_transpose<double>(1.6, Region<2>::make({5, 3}, {6, 6}));
+}
+
+// backward_into with accumulate adds into a larger image and leaves the
+// rest of it as it was
+static void test_gaussian_backward_accumulate() {
+    std::array<double, 2> one{1.0, 1.0}, zero{};
+    const auto largest = Region<2>::make({0, 0}, {40, 30});
+    auto src = Image<float, 2>::allocate(largest, one, zero, 0.0f);
+    GaussianFilter<Image<float, 2>> f(src, 1.5);
+    const auto req = Region<2>::make({10, 8}, {12, 9});
+    Image<float, 2> y = Image<float, 2>::allocate(req, one, zero, 0.5f);
+    const Image<float, 2> g = f.backward(y);
+    Image<float, 2> acc = Image<float, 2>::allocate(largest, one, zero, 2.0f);
+    f.backward_into(y, acc, true);
+    f.backward_into(y, acc, true);
+    Image<float, 2>::for_each_index(largest, [&](const Index<2>& i) {
+        const float want = g.region.contains(i) ? 2.0f + 2.0f * g.get_pixel(i) : 2.0f;
+        CHECK(std::fabs(acc.get_pixel(i) - want) <= 1e-6f, "(%lld,%lld): %g vs %g",
+              (long long)i.v[0], (long long)i.v[1], acc.get_pixel(i), want);
+    });
+    // a constant gradient spreads with unit mass: the adjoint of a
+    // normalised clamp-to-edge blur preserves the sum
+    double s = 0.0;
+    for (float v : g.data) s += v;
+    CHECK(std::fabs(s - 0.5 * 12 * 9) < 1e-3, "gradient mass %g", s);
+}
+
 int main() {
 #define RUN(t)                  \
     t();                        \
@@ -307,6 +374,8 @@ int main() {
     RUN(test_unsharp_mask);
     RUN(test_canny);
     RUN(test_recursive_gaussian);
+    RUN(test_gaussian_backward_transpose);
+    RUN(test_gaussian_backward_accumulate);
     std::printf("all passed\n");
     return 0;
 }

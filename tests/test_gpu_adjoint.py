@@ -119,3 +119,60 @@ def test_forward_adjoint_dot_product_256cube(itk, dev):
     lhs = float((y.double() * g.double()).sum())
     rhs = float((x.double() * gi.double()).sum())
     assert abs(lhs - rhs) <= 1e-5 * (y.double().abs() * g.double().abs()).sum().item()
+
+
+def _host_backward(itk, g, ii, size, sp, sigma, ri, rs, gin, gin_index, accumulate, ncomp=1):
+    """itkg_gaussian_backward_host on numpy buffers; gin is updated in place."""
+    import ctypes
+    D = len(sp)
+    dt = itk.F64 if g.dtype == np.float64 else itk.F32
+    g = np.ascontiguousarray(g)
+    gsize = list(reversed(gin.shape[:D]))
+    rc = itk.lib.itkg_gaussian_backward_host(
+        D, dt, ncomp, float(sigma), (ctypes.c_double * D)(*sp),
+        ctypes.byref(itk.Region(ii, size).to_c()), g.ctypes.data,
+        ctypes.byref(itk.Region(ri, rs).to_c()), gin.ctypes.data,
+        ctypes.byref(itk.Region(gin_index, gsize).to_c()), int(accumulate))
+    assert rc == 0, itk.lib.itkg_last_error()
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("ncomp", [1, 3])
+def test_host_backward_f64_equals_transpose(itk, orc, dev, case, ncomp):
+    shape, ii, sp, sigma, ri, rs = case
+    g = np.random.default_rng(21).standard_normal(
+        tuple(reversed(rs)) + ((ncomp,) if ncomp > 1 else ()))
+    size = list(reversed(shape))
+    ref, in_i, in_s = orc.adjoint_f64(g, ii, size, sp, sigma, ri, rs, ncomp)
+    gin = np.full(ref.shape, np.nan)  # every pixel of the region is written
+    _host_backward(itk, g, ii, size, sp, sigma, ri, rs, gin, in_i, False, ncomp)
+    assert np.abs(gin - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_host_backward_f32_matches_device_and_accumulates(itk, orc, dev):
+    shape, ii, sp, sigma, ri, rs = CASES[5]
+    size = list(reversed(shape))
+    g = np.random.default_rng(22).standard_normal(tuple(reversed(rs))).astype(np.float32)
+    dev_out = _run_backward(itk, dev, shape, ii, sp, sigma, ri, rs, g)
+    in_i, in_s = dev_out.region.index, dev_out.region.size
+    gin = np.empty(tuple(reversed(in_s)), dtype=np.float32)
+    _host_backward(itk, g, ii, size, sp, sigma, ri, rs, gin, list(in_i), False)
+    assert np.array_equal(gin, dev_out.data.cpu().numpy())
+    # accumulate twice into the whole image (larger than the written region):
+    # the region gains 2x the gradient, the rest keeps its values
+    base = np.random.default_rng(23).standard_normal(tuple(reversed(size))).astype(np.float32)
+    acc = base.copy()
+    _host_backward(itk, g, ii, size, sp, sigma, ri, rs, acc, ii, True)
+    _host_backward(itk, g, ii, size, sp, sigma, ri, rs, acc, ii, True)
+    sl = tuple(slice(in_i[d] - ii[d], in_i[d] - ii[d] + in_s[d]) for d in reversed(range(3)))
+    mask = np.ones(base.shape, dtype=bool)
+    mask[sl] = False
+    assert np.array_equal(acc[mask], base[mask])
+    want = base[sl].astype(np.float64) + 2.0 * gin
+    assert np.abs(acc[sl] - want).max() <= 1e-6 * max(1.0, np.abs(want).max())
+    # without accumulate into the larger image: the region is overwritten, the rest kept
+    over = base.copy()
+    _host_backward(itk, g, ii, size, sp, sigma, ri, rs, over, ii, False)
+    want = base.copy()
+    want[sl] = gin
+    assert np.array_equal(over, want)

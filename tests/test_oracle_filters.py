@@ -244,12 +244,78 @@ def test_recursive_reference_tests(orc):
 
 
 @pytest.mark.parametrize("sigma", [0.1, 0.5, 1.0, 2.4, 2.5, 3.7, 10.0])
-def test_yvv_coefficients(orc, sigma):
+def test_yvv_coefficients_clamp(orc, sigma):
     c = orc.yvv_coeffs(sigma)
-    assert abs(float(c.sum()) - 1.0) < 1e-14         # unit DC gain: B + a1 + a2 + a3 = 1
     if sigma < 0.5:
         assert np.array_equal(c, orc.yvv_coeffs(0.5))  # clamped below at 0.5
     assert np.array_equal(orc.yvv_coeffs(float("nan")), orc.yvv_coeffs(0.5))  # f64::max drops NaN
+
+
+# Young & van Vliet (Signal Processing 44, 1995) design the recursion from
+# three poles of the scaled Gaussian, published as m0 = 1.16680 (real) and
+# m1 +- i m2 = 1.10783 +- 1.40586 i; the denominator 1 - a1 x - a2 x^2 - a3 x^3
+# (x = z^-1) vanishes where q (1 - x) = -m0 and -m1 +- i m2, with q the scale of
+# the q(sigma) rule (q = 0.98711 sigma - 0.96330 from sigma 2.5 up, else
+# 3.97156 - 4.14554 sqrt(1 - 0.26891 sigma)).  The poles recovered from the
+# oracle's a_k therefore pin every constant of the b polynomials against the
+# published pole constants, and the scale pins the q branch.  The b
+# constants are rounded to about six digits, so the agreement loosens in
+# proportion to q.
+YVV_M0, YVV_M1, YVV_M2 = 1.16680, 1.10783, 1.40586
+
+
+@pytest.mark.parametrize("sigma", [0.3, 0.6, 1.0, 1.7, 2.4, 2.5, 3.3, 5.0, 8.0])
+def test_yvv_poles_are_the_published_ones(orc, sigma):
+    B, a1, a2, a3 = orc.yvv_coeffs(sigma)
+    v = 1.0 - np.roots([-a3, -a2, -a1, 1.0])          # q (1 - x) / q at the poles
+    real = v[np.abs(v.imag) < 1e-9].real
+    cplx = v[v.imag > 1e-9]
+    assert real.size == 1 and cplx.size == 1
+    q = -YVV_M0 / real[0]
+    tol = 5e-5 + 7e-5 * q
+    assert abs(cplx[0].real * q + YVV_M1) < tol
+    assert abs(abs(cplx[0].imag) * q - YVV_M2) < tol
+    s = max(sigma, 0.5)
+    q_rule = 0.98711 * s - 0.96330 if s >= 2.5 else 3.97156 - 4.14554 * math.sqrt(1 - 0.26891 * s)
+    assert abs(q - q_rule) < tol * q_rule
+    # the gain B makes the response sum to one: B = 1 - a1 - a2 - a3 follows
+    # from the b polynomials only up to their rounding (6e-6 of b0 q^2)
+    assert abs(B - (1.0 - a1 - a2 - a3)) < 1e-12
+
+
+def _yvv_closed_forms(c):
+    """Variance and centre value of the zero-phase (causal then anticausal)
+    impulse response on an unbounded line, from the coefficients' transfer
+    function H(w) = B^2 / |1 - sum_k a_k e^{-ikw}|^2: the causal factor
+    G(x) = B / (1 - sum a_k x^k) has mean m = sum k a_k / B and variance
+    sum k^2 a_k / B + m^2 (its first two derivatives at x = 1), the two
+    passes double the variance, and the centre value is the mean of H over a
+    period (trapezoid rule, exact to rounding for this smooth periodic H)."""
+    B, a = c[0], np.asarray(c[1:])
+    k = np.arange(1, 4)
+    m = (k * a).sum() / B
+    var = 2.0 * ((k * k * a).sum() / B + m * m)
+    w = 2.0 * np.pi * np.arange(1 << 16) / (1 << 16)
+    A = 1.0 - (a[None, :] * np.exp(-1j * np.outer(w, k))).sum(axis=1)
+    return var, float((B * B / np.abs(A) ** 2).mean())
+
+
+@pytest.mark.parametrize("sigma", [0.6, 1.0, 1.7, 2.4, 2.5, 3.3, 5.0, 8.0])
+def test_recursive_impulse_matches_transfer_function(orc, sigma):
+    # an impulse in the middle of a line long enough that the boundary
+    # extension never reaches it (the slowest pole decays by 1 / (1 + m0 / q)
+    # per sample): the oracle's recursion has the closed-form variance and
+    # centre value of its coefficients, and unit sum
+    n = 2001
+    x = np.zeros(n)
+    x[n // 2] = 1.0
+    h = orc.recursive_generate_region(x, [0], [1.0], sigma, -1, [0], [n])
+    i = np.arange(n) - n // 2
+    var, peak = _yvv_closed_forms(orc.yvv_coeffs(sigma))
+    assert abs(h.sum() - 1.0) < 1e-12
+    assert abs((h * i * i).sum() - var) < 1e-11 * var
+    assert abs(h[n // 2] - peak) < 1e-11 * peak
+    assert abs((h * i).sum()) < 1e-10                  # zero phase: no shift
 
 
 @pytest.mark.parametrize("sigma,lo,hi", [(1.0, 1.08, 1.30), (1.5, 1.08, 1.30), (2.0, 1.08, 1.30),

@@ -299,6 +299,15 @@ int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double
                            const itkg_view* grad_output, const itkg_view* grad_input,
                            int32_t accumulate, void* stream);
 
+/* The direction bin CannyEdgeDetectionFilter's non-maximum suppression
+ * uses for each of n gradients (src/filters/edges.rs:602, 612-620: atan2 in
+ * degrees modulo 180; 0 below 22.5 or from 157.5, 1 below 67.5, 2 below
+ * 112.5, else 3), from device arrays gx, gy (f64) into device bins, exactly
+ * as the Canny kernels bin them: next to a bin edge the bin is the one a
+ * correctly rounded atan2 gives, independent of the device's atan2. */
+int itkg_canny_direction_bins(const double* gx, const double* gy, int64_t n, uint8_t* bins,
+                              void* stream);
+
 /* itkg_gaussian_backward on dense host buffers (axis 0 fastest, ncomp
  * interleaved): grad_output covers `requested`, grad_input covers
  * *grad_input_region, which must contain input_region_for_output(

@@ -49,7 +49,7 @@ EXPORTS = (
     "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
     "itkg_canny_generate_region", "itkg_canny_generate_region_host", "itkg_trim_scratch",
     "itkg_convert_pixels", "itkg_cast_gaussian_cast_host", "itkg_host_staging_info",
-    "itkg_gaussian_backward_host",
+    "itkg_gaussian_backward_host", "itkg_canny_direction_bins",
 )
 
 
@@ -136,6 +136,8 @@ def _load():
     L.itkg_set_path.argtypes = [ctypes.c_int]
     L.itkg_trim_scratch.argtypes = [ctypes.c_uint64]
     L.itkg_host_staging_info.argtypes = [P(ctypes.c_int32)]
+    L.itkg_canny_direction_bins.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                            ctypes.c_void_p, ctypes.c_void_p]
     L.itkg_gaussian_backward_host.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
         P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_int32]

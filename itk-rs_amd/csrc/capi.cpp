@@ -1428,6 +1428,16 @@ int itkg_gaussian_backward(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double
                          static_cast<hipStream_t>(stream), nullptr);
 }
 
+int itkg_canny_direction_bins(const double* gx, const double* gy, int64_t n, uint8_t* bins,
+                              void* stream) {
+    clear_error();
+    if (n < 0) return fail(ITKG_ERR_INVALID_ARG, "negative count");
+    if (n == 0) return ITKG_OK;
+    if (!gx || !gy || !bins) return fail(ITKG_ERR_INVALID_ARG, "null buffer");
+    const hipError_t e = launch_canny_bins(gx, gy, bins, n, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ITKG_OK : hip_fail(e, "canny bins launch");
+}
+
 int itkg_gaussian_backward_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
                                 const double* spacing, const itkg_region* largest,
                                 const void* grad_output, const itkg_region* requested,

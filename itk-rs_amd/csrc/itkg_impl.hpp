@@ -101,6 +101,9 @@ struct CannyArgs {
 };
 template <typename T>
 hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s);
+// The direction bins of n gradients (device arrays), as the Canny kernels bin them.
+hipError_t launch_canny_bins(const double* gx, const double* gy, uint8_t* bins, int64_t n,
+                             hipStream_t s);
 
 // Pixel conversion over a region of sizes n (1 beyond ndim); strides in
 // pixels, ncomp interleaved components.

@@ -6,11 +6,52 @@
 // clamped neighbours along the bin; hysteresis (strong = nms >= upper, then
 // every pixel with nms >= lower 8-connected to an edge).  Compiled with
 // -ffp-contract=off: the products and sums round as the reference's do.
+#include <algorithm>
+
 #include "itkg_impl.hpp"
 
 namespace itkg {
 
 namespace {
+
+#include "canny_edges.inc"
+
+// The direction bin (0: 0, 1: 45, 2: 90, 3: 135 degrees) of a gradient as
+// the reference computes it with the host libm's atan2 (edges.rs:602,
+// 612-620).  The device atan2 may differ from the host's in the last bit,
+// which only matters next to a bin edge: within 1e-9 degrees of one, the bin
+// is decided by where the true angle lies against the edge's exact
+// threshold m (the midpoint below the smallest double direction that
+// to_degrees and rem_euclid take onto the edge; a correctly rounded atan2
+// lands on the edge side exactly when the angle is above m), from the sign
+// of gy cos(m) - gx sin(m) in double-double arithmetic.  A sign too close to
+// zero to trust (below 2^-100 of the gradient, never seen for double
+// inputs) keeps the ordinary bin.
+__device__ __forceinline__ int canny_bin(const double gx, const double gy) {
+    // to_degrees multiplies by the f64 nearest 180/pi; rem_euclid(180)
+    const double deg = atan2(gy, gx) * (180.0 / 3.14159265358979323846);
+    double angle = fmod(deg, 180.0);
+    if (angle < 0.0) angle += 180.0;
+    int b = angle < 22.5 || angle >= 157.5 ? 0 : (angle < 67.5 ? 1 : (angle < 112.5 ? 2 : 3));
+    const double k = rint((angle - 22.5) * (1.0 / 45.0));
+    if (k >= 0.0 && k <= 3.0 && fabs(angle - (22.5 + 45.0 * k)) < 1e-9) {
+        const int e = (int)k;
+        const double* t = kCannyEdgeTrig[deg < 0.0 ? 1 : 0][e];
+        const double p1 = gy * t[0], e1 = fma(gy, t[0], -p1);
+        const double p2 = gx * t[2], e2 = fma(gx, t[2], -p2);
+        const double side = (p1 - p2) + ((e1 - e2) + (gy * t[1] - gx * t[3]));
+        if (fabs(side) > 0x1p-100 * (fabs(gx) + fabs(gy)))
+            b = side > 0.0 ? ((e + 1) & 3) : e;  // above edge e: the next bin
+    }
+    return b;
+}
+
+__global__ __launch_bounds__(256) void canny_bins_kernel(const double* gx, const double* gy,
+                                                         uint8_t* bins, const int64_t n) {
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256)
+        bins[k] = (uint8_t)canny_bin(gx[k], gy[k]);
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void canny_grad_kernel(const T* sm, double* mag, uint8_t* bin,
@@ -22,11 +63,7 @@ __global__ __launch_bounds__(256) void canny_grad_kernel(const T* sm, double* ma
     const double gx = (double)sm[ip + j * nx] - (double)sm[im + j * nx];
     const double gy = (double)sm[i + jp * nx] - (double)sm[i + jm * nx];
     mag[i + j * nx] = sqrt(gx * gx + gy * gy);
-    // to_degrees multiplies by the f64 nearest 180/pi; rem_euclid(180)
-    const double deg = atan2(gy, gx) * (180.0 / 3.14159265358979323846);
-    double angle = fmod(deg, 180.0);
-    if (angle < 0.0) angle += 180.0;
-    bin[i + j * nx] = angle < 22.5 || angle >= 157.5 ? 0 : (angle < 67.5 ? 1 : (angle < 112.5 ? 2 : 3));
+    bin[i + j * nx] = (uint8_t)canny_bin(gx, gy);
 }
 
 __global__ __launch_bounds__(256) void canny_nms_kernel(const double* mag, const uint8_t* bin,
@@ -112,6 +149,14 @@ __global__ __launch_bounds__(256) void canny_out_kernel(const uint8_t* edges, T*
 
 }  // namespace
 
+hipError_t launch_canny_bins(const double* gx, const double* gy, uint8_t* bins, int64_t n,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(canny_bins_kernel, dim3(grid), dim3(256), 0, s, gx, gy, bins, n);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     const int64_t nx = a.nx, ny = a.ny;
@@ -124,7 +169,7 @@ hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     const dim3 gt((unsigned)((nx + kTile - 1) / kTile), (unsigned)((ny + kTile - 1) / kTile));
     // passes until one adds nothing (each pass grows every tile to its own
     // fixed point; a chain crossing k tiles needs about k passes)
-    for (int pass = 0; e == hipSuccess; ++pass) {
+    while (e == hipSuccess) {
         int host = 0;
         e = hipMemsetAsync(a.changed, 0, sizeof(int), s);
         if (e != hipSuccess) break;

@@ -501,6 +501,26 @@ int NAME(int D, const T* img, const int64_t* img_index, const int64_t* img_size,
 DEFINE_RECURSIVE(orc_recursive_generate_region_f32, float)
 DEFINE_RECURSIVE(orc_recursive_generate_region_f64, double)
 
+/* Direction bin of the suppression step (src/filters/edges.rs:612-620) for
+ * a direction r = atan2(gy, gx) in radians: to_degrees (one multiply by the
+ * double nearest 180/pi), rem_euclid(180), then 0 below 22.5 or from 157.5,
+ * 1 below 67.5, 2 below 112.5, else 3. */
+static int canny_bin_of(double r) {
+    double deg = r * (180.0 / 3.14159265358979323846);
+    double angle = fmod(deg, 180.0);
+    if (angle < 0) angle += 180.0;
+    if (angle < 22.5 || angle >= 157.5) return 0;
+    if (angle < 67.5) return 1;
+    if (angle < 112.5) return 2;
+    return 3;
+}
+
+/* The bin of each gradient (gx[k], gy[k]), with the host libm's atan2 as
+ * the reference's f64::atan2 (src/filters/edges.rs:602). */
+void orc_canny_bins(int64_t n, const double* gx, const double* gy, unsigned char* out) {
+    for (int64_t k = 0; k < n; ++k) out[k] = (unsigned char)canny_bin_of(atan2(gy[k], gx[k]));
+}
+
 /* ---- CannyEdgeDetectionFilter: src/filters/edges.rs:548-664, loop for
  * loop (2-D): smooth the whole image with the index-space kernel (every
  * pass over the whole buffer), gradient by clamped central differences in
@@ -551,14 +571,9 @@ int NAME(const T* img, const int64_t* img_index, const int64_t* img_size,       
         for (int64_t i = 0; i < nx; ++i) {                                       \
             int64_t f = i + j * nx;                                              \
             double m = mag[f];                                                   \
-            double deg = dir[f] * (180.0 / 3.14159265358979323846);              \
-            double angle = fmod(deg, 180.0);                                     \
-            if (angle < 0) angle += 180.0;                                       \
-            int64_t di, dj;                                                      \
-            if (angle < 22.5 || angle >= 157.5) { di = 1; dj = 0; }              \
-            else if (angle < 67.5) { di = 1; dj = 1; }                           \
-            else if (angle < 112.5) { di = 0; dj = 1; }                          \
-            else { di = -1; dj = 1; }                                            \
+            const int bin = canny_bin_of(dir[f]);                                \
+            const int64_t di = bin == 0 ? 1 : (bin == 1 ? 1 : (bin == 2 ? 0 : -1)); \
+            const int64_t dj = bin == 0 ? 0 : 1;                                 \
             int64_t a_i = i + di, a_j = j + dj, b_i = i - di, b_j = j - dj;      \
             a_i = a_i < 0 ? 0 : (a_i > nx - 1 ? nx - 1 : a_i);                   \
             a_j = a_j < 0 ? 0 : (a_j > ny - 1 ? ny - 1 : a_j);                   \

@@ -96,6 +96,9 @@ def lib():
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_void_p, i64p, i64p, ctypes.c_double, ctypes.c_double,
                            ctypes.c_double, i64p, i64p, ctypes.c_void_p]
+        L.orc_canny_bins.restype = None
+        L.orc_canny_bins.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
         L.orc_adjoint_f64.restype = ctypes.c_int
         L.orc_adjoint_f64.argtypes = [ctypes.c_int, ctypes.c_int, i64p, i64p, f64p, ctypes.c_double,
                                       i64p, i64p, ctypes.c_void_p, ctypes.c_void_p, i64p, i64p]
@@ -302,6 +305,17 @@ def canny_generate_region(img: np.ndarray, img_index, sigma, lower, upper, req_i
             float(upper), _i64(req_index), _i64(req_size), out.ctypes.data)
     if rc != 0:
         raise ValueError("oracle rejected arguments")
+    return out
+
+
+def canny_bins(gx: np.ndarray, gy: np.ndarray) -> np.ndarray:
+    """Direction bins (0: 0, 1: 45, 2: 90, 3: 135 degrees) of the suppression
+    step of CannyEdgeDetectionFilter (src/filters/edges.rs:602, 612-620) for
+    f64 gradients, with the host libm's atan2 as in the reference."""
+    gx = np.ascontiguousarray(gx, dtype=np.float64)
+    gy = np.ascontiguousarray(gy, dtype=np.float64)
+    out = np.empty(gx.shape, dtype=np.uint8)
+    lib().orc_canny_bins(gx.size, gx.ctypes.data, gy.ctypes.data, out.ctypes.data)
     return out
 
 

@@ -245,10 +245,11 @@ int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, doubl
  * suppression, hysteresis between lower and upper; the request as 1 / 0.
  * input covers `largest` (the filter's input_region_for_output).  The
  * hysteresis repeats until a pass adds nothing, waiting on the stream after
- * each pass, so the call returns when the output is written.  Results equal
- * the reference's wherever the device's atan2 bins a direction as glibc's
- * does: they differ only for a gradient angle within a few ulps of 22.5 +
- * 45 k degrees. */
+ * each pass, so the call synchronises the stream, returns when the output
+ * is written, and cannot be captured into a graph.  The direction bins are
+ * those of a correctly rounded atan2 (itkg_canny_direction_bins): they equal
+ * the reference's except where the reference's libm atan2 is off by more
+ * than half an ulp for an angle within an ulp of 22.5 + 45 k degrees. */
 int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, double lower,
                                double upper, const itkg_region* largest, const itkg_view* input,
                                const itkg_view* output, itkg_arith arith, void* stream);

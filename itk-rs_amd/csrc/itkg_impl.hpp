@@ -84,7 +84,7 @@ hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxe
                             const double (*coef)[4], hipStream_t s);
 
 // Canny after smoothing: `smoothed` is the dense nx x ny smoothed image;
-// mag, nms (f64), bin, edges (u8) are nx x ny scratch and `changed` one int;
+// mag, nms (f64), bin, edges (u8) are nx x ny scratch and `changed` two ints;
 // out (strides in pixels) receives the mx x my request at offset (ox, oy).
 template <typename T>
 struct CannyArgs {

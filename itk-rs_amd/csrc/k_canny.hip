@@ -168,8 +168,17 @@ hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     hipError_t e = hipGetLastError();
     const dim3 gt((unsigned)((nx + kTile - 1) / kTile), (unsigned)((ny + kTile - 1) / kTile));
     // passes until one adds nothing (each pass grows every tile to its own
-    // fixed point; a chain crossing k tiles needs about k passes)
+    // fixed point; a chain crossing k tiles needs about k passes).  Passes
+    // go in batches of 1, 2, 4, ... 64, and only a batch's last pass raises
+    // the flag that is read back: a pass after the fixed point changes
+    // nothing, so a chain across k tiles costs about log2(k) waits on the
+    // host instead of k, for at most twice the passes.  The batch's other
+    // passes raise a second flag nobody reads.
+    int batch = 1;
     while (e == hipSuccess) {
+        for (int i = 0; i + 1 < batch; ++i)
+            hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.nms, a.edges, nx,
+                               ny, a.lower, a.changed + 1);
         int host = 0;
         e = hipMemsetAsync(a.changed, 0, sizeof(int), s);
         if (e != hipSuccess) break;
@@ -178,6 +187,7 @@ hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
         e = hipMemcpyAsync(&host, a.changed, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess || !host) break;
+        batch = batch < 64 ? 2 * batch : 64;
     }
     if (e != hipSuccess) return e;
     if (a.mx > 0 && a.my > 0)

@@ -327,9 +327,19 @@ def _canny_images():
     sy = (100 + 2.8 * t * np.sin(t)).astype(int)
     spiral[np.clip(sy, 0, 199), np.clip(sx, 0, 199)] = 1.0   # one long chain for hysteresis
     yield "spiral", spiral, 0.8, 0.02, 0.5
+    # a step edge that is strong only at its left end and weak along 1170
+    # more pixels: hysteresis must carry it across 38 tiles of 32 pixels
+    # (several batches of passes between flag read-backs); weak bars that
+    # touch no strong edge stay out, a separate strong block stays in
+    step = np.zeros((64, 1216), np.float32)
+    step[32:, :] = np.interp(np.arange(1216), [0, 12, 42], [1.0, 1.0, 0.1])[None, :]
+    step[8:20, 1100:1110] = 1.0
+    for k in range(0, 1000, 80):
+        step[4:28, 100 + k:104 + k] += 0.08
+    yield "long chain", step, 1.0, 0.01, 0.3
 
 
-@pytest.mark.parametrize("case", list(range(4)))
+@pytest.mark.parametrize("case", list(range(5)))
 def test_canny_matches_oracle(itk, orc, dev, case):
     name, x, sigma, lower, upper = list(_canny_images())[case]
     ny, nx = x.shape

@@ -24,7 +24,7 @@ reference, the accuracy the target is stated at).  Also reported (under
   - host buffers end to end: the host-buffer entry point on the 512^3
     volume through the pinned staging ring and through pageable copies, and
     the u8 -> f32 -> Gaussian -> u8 chain of examples/blur_png.rs on a 512^3
-    u8 volume.
+    u8 volume and on one 512 x 512 image (the example's own use).
 
 roofline: algorithmic bytes = 8 B per output voxel (one f32 read, one f32
 write) / the average kernel duration from HIP events on the launch stream,
@@ -589,6 +589,14 @@ def main():
                 "ms_min": round(float(np.min(ts)) * 1e3, 2),
                 "gvox_s": round(n / float(np.median(ts)) / 1e9, 3)}
             del u8
+            # the reference's CLI case (examples/blur_png.rs): one 512 x 512 u8
+            # image, host to host, one piece (no helper threads)
+            png = (torch.rand((512, 512), device=dev) * 255).to(torch.uint8).cpu().numpy()
+            ts = host_times(lambda: itk.cast_gaussian_cast_host(png, sigma, np.uint8, arith=arith),
+                            50)
+            extras["host_to_host_512sq_u8_chain"] = {
+                "us_median": round(float(np.median(ts)) * 1e6, 1),
+                "us_min": round(float(np.min(ts)) * 1e6, 1)}
             itk.trim_scratch(0)
 
         if KEEP_GC:

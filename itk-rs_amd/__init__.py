@@ -475,9 +475,14 @@ def stream_full(source, num_pieces: int) -> Image:
     of the output (same values as generate_region + copy_from)."""
     full = source.largest_region()
     first = source.generate_region(Region(full.index, [0] * full.ndim))
-    out = Image.allocate(full, source.spacing, source.origin, 0.0,
-                         first.data.shape[-1] if first.data.dim() > full.ndim else 1,
-                         first.data.dtype, first.data.device)
+    nc = first.data.shape[-1] if first.data.dim() > full.ndim else 1
+    if isinstance(source, GaussianFilter):
+        # the pieces cover the region and each writes its whole slab
+        out = Image.empty(full, source.spacing, source.origin, nc, first.data.dtype,
+                          first.data.device)
+    else:
+        out = Image.allocate(full, source.spacing, source.origin, 0.0, nc, first.data.dtype,
+                             first.data.device)
     for piece in split_region(full, num_pieces):
         if isinstance(source, GaussianFilter):
             in_reg = source.input_region_for_output(piece)

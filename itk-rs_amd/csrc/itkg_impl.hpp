@@ -263,6 +263,7 @@ struct StagedJob {
 };
 // Runs the job with the compute on s_comp (a stream the caller owns) and
 // the copies on streams of its own; returns when the output is in host_out.
+// One piece of at most one chunk each way runs serially on s_comp.
 int run_staged(const StagedJob& job, hipStream_t s_comp);
 // Frees the pinned staging ring if no call is using it.
 int stage_trim();

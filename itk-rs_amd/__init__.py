@@ -12,6 +12,7 @@ raises.  Build it with ``make -C itk-rs_amd`` (or ``__graft_entry__.build()``).
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 from dataclasses import dataclass, field
@@ -167,7 +168,7 @@ def trim_scratch(keep_bytes: int = 0, device=None) -> None:
     """Hands the library's idle scratch-pool memory on `device` (default: the
     current one) beyond `keep_bytes` back to the driver, e.g. before torch
     allocates large tensors (torch's allocator cannot see that pool)."""
-    with torch.cuda.device(device if device is not None else torch.cuda.current_device()):
+    with _on_device(device if device is not None else torch.cuda.current_device()):
         _check(lib.itkg_trim_scratch(int(keep_bytes)))
 
 
@@ -314,7 +315,20 @@ class Image:
                      for d in reversed(range(D)))
 
     def view(self, r: Optional[Region] = None) -> c_view:
-        """Strided view of this image's pixels (whole buffer by default)."""
+        """Strided view of this image's pixels (whole buffer by default); the
+        last one built is kept while the buffer, its layout and the region
+        stay the same."""
+        t = self.data
+        key = (t.data_ptr(), t.shape, t.stride(), self.region) if t is not None else None
+        cached = self.__dict__.get("_view_cache")
+        if cached is not None and key is not None and cached[0] == key:
+            return cached[1]
+        v = self._make_view()
+        if key is not None:
+            self.__dict__["_view_cache"] = (key, v)
+        return v
+
+    def _make_view(self) -> c_view:
         t = self.data
         D = self.ndim
         if t.dim() not in (D, D + 1):
@@ -379,8 +393,27 @@ class ImageSource:
         raise NotImplementedError
 
 
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_NO_SWITCH = contextlib.nullcontext()
+
+
+def _device_index(device) -> int:
+    d = torch.device(device) if not isinstance(device, torch.device) else device
+    return d.index if d.index is not None else torch.cuda.current_device()
+
+
 def _stream_ptr(device) -> ctypes.c_void_p:
+    """torch's current stream on `device` (the raw handle, without building a
+    Stream object where torch offers that)."""
+    if _RAW_STREAM is not None:
+        return ctypes.c_void_p(_RAW_STREAM(_device_index(device)))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _on_device(device):
+    """Makes `device` current for the block (no switch when it already is)."""
+    idx = _device_index(device)
+    return _NO_SWITCH if idx == torch.cuda.current_device() else torch.cuda.device(idx)
 
 
 class GaussianFilter(ImageSource):
@@ -434,7 +467,7 @@ class GaussianFilter(ImageSource):
         `inp` that covers input_region_for_output(out.region)."""
         D = out.ndim
         dt, nc = _pixel_info(inp.data, D)
-        with torch.cuda.device(inp.data.device):
+        with _on_device(inp.data.device):
             _check(lib.itkg_gaussian_generate_region(
                 D, dt, nc, self.sigma, _doubles(self.spacing),
                 ctypes.byref(self.largest_region().to_c()), ctypes.byref(inp.view()),
@@ -461,7 +494,7 @@ class GaussianFilter(ImageSource):
         into `grad_input`, whose region must contain input_region_for_output."""
         D = grad_output.ndim
         dt, nc = _pixel_info(grad_output.data, D)
-        with torch.cuda.device(grad_output.data.device):
+        with _on_device(grad_output.data.device):
             _check(lib.itkg_gaussian_backward(
                 D, dt, nc, self.sigma, _doubles(self.spacing),
                 ctypes.byref(self.largest_region().to_c()), ctypes.byref(grad_output.view()),
@@ -525,7 +558,7 @@ def _separable_into(radii: Sequence[int], weights: Sequence[Sequence[float]], la
     D = out.ndim
     dt, nc = _pixel_info(inp.data, D)
     flat = [float(w) for ws in weights for w in ws]
-    with torch.cuda.device(inp.data.device):
+    with _on_device(inp.data.device):
         _check(lib.itkg_separable_generate_region(
             D, dt, nc, (ctypes.c_uint64 * D)(*[int(r) for r in radii]),
             (ctypes.c_double * len(flat))(*flat), ctypes.byref(largest.to_c()),
@@ -736,7 +769,7 @@ class CastFilter(ImageSource):
     def generate_into(self, inp: Image, out: Image) -> None:
         D = out.ndim
         nc = 1 if inp.data.dim() == D else int(inp.data.shape[-1])
-        with torch.cuda.device(inp.data.device):
+        with _on_device(inp.data.device):
             _check(lib.itkg_convert_pixels(D, _ptype(inp.data.dtype), _ptype(out.data.dtype), nc,
                                            ctypes.byref(inp.view()), ctypes.byref(out.view()),
                                            self.mode, _stream_ptr(inp.data.device)))
@@ -820,7 +853,7 @@ class UnsharpMaskFilter(ImageSource):
         dt, nc = _pixel_info(inp.data, D)
         if nc != 1:
             raise ItkgError(ERR_UNSUPPORTED, "unsharp mask takes scalar pixels")
-        with torch.cuda.device(inp.data.device):
+        with _on_device(inp.data.device):
             _check(lib.itkg_unsharp_mask_generate_region(
                 D, dt, self.sigma, self.amount, self.threshold,
                 ctypes.byref(self.largest_region().to_c()), ctypes.byref(inp.view()),
@@ -868,7 +901,7 @@ class CannyEdgeDetectionFilter(ImageSource):
         dt, nc = _pixel_info(inp.data, D)
         if nc != 1:
             raise ItkgError(ERR_UNSUPPORTED, "Canny takes scalar pixels")
-        with torch.cuda.device(inp.data.device):
+        with _on_device(inp.data.device):
             _check(lib.itkg_canny_generate_region(
                 D, dt, self.sigma, self.lower_threshold, self.upper_threshold,
                 ctypes.byref(self.largest_region().to_c()), ctypes.byref(inp.view()),
@@ -924,7 +957,7 @@ class RecursiveGaussianFilter(ImageSource):
         dt, nc = _pixel_info(inp.data, D)
         if nc != 1:
             raise ItkgError(ERR_UNSUPPORTED, "the recursive Gaussian takes scalar pixels")
-        with torch.cuda.device(inp.data.device):
+        with _on_device(inp.data.device):
             _check(lib.itkg_recursive_gaussian_generate_region(
                 D, dt, self.sigma, _doubles(self.spacing), self._axis,
                 ctypes.byref(self.largest_region().to_c()), ctypes.byref(inp.view()),
@@ -1065,17 +1098,29 @@ def gaussian(x: torch.Tensor, sigma: float, spacing=None, ndim: Optional[int] = 
     """Whole-image GaussianFilter{source: x, sigma}.generate_region(largest)."""
     nd = ndim if ndim is not None else x.dim()
     sp = tuple(spacing) if spacing is not None else (1.0,) * nd
-    src = _full_image(x.contiguous() if not x.is_contiguous() else x, nd, sp)
-    f = GaussianFilter(src, sigma)
-    out = Image.empty(src.region, sp, src.origin,
-                      x.shape[-1] if x.dim() > nd else 1, x.dtype, x.device)
-    with torch.cuda.device(x.device):
+    if not x.is_contiguous():
+        x = x.contiguous()
+    dt, nc = _pixel_info(x, nd)
+    out = torch.empty_like(x)
+    vin, vout = _whole_view(x, nd, nc), _whole_view(out, nd, nc)
+    with _on_device(x.device):
         _check(lib.itkg_gaussian_generate_region(
-            nd, _pixel_info(x, nd)[0], _pixel_info(x, nd)[1], float(sigma), _doubles(sp),
-            ctypes.byref(src.region.to_c()), ctypes.byref(src.view()),
-            ctypes.byref(out.view()), arith, _stream_ptr(x.device)))
-    del f
-    return out.data
+            nd, dt, nc, float(sigma), _doubles(sp), ctypes.byref(vin.region), ctypes.byref(vin),
+            ctypes.byref(vout), arith, _stream_ptr(x.device)))
+    return out
+
+
+def _whole_view(t: torch.Tensor, nd: int, nc: int) -> c_view:
+    """The view of a contiguous tensor as a whole image at index 0 (what
+    Image(Region([0] * nd, reversed shape), ..., t).view() gives)."""
+    v = c_view()
+    v.data = t.data_ptr() if t.numel() else None
+    st = t.stride()
+    for d in range(nd):
+        v.region.index[d] = 0
+        v.region.size[d] = t.shape[nd - 1 - d]
+        v.stride[d] = st[nd - 1 - d] // nc
+    return v
 
 
 def gaussian_backward(g: torch.Tensor, sigma: float, spacing=None, ndim: Optional[int] = None,
@@ -1090,7 +1135,7 @@ def gaussian_backward(g: torch.Tensor, sigma: float, spacing=None, ndim: Optiona
         accumulate = False
     oimg = _full_image(out, nd, sp)
     dt, nc = _pixel_info(g, nd)
-    with torch.cuda.device(g.device):
+    with _on_device(g.device):
         _check(lib.itkg_gaussian_backward(
             nd, dt, nc, float(sigma), _doubles(sp), ctypes.byref(gimg.region.to_c()),
             ctypes.byref(gimg.view()), ctypes.byref(oimg.view()), int(accumulate),

@@ -99,6 +99,43 @@ hipError_t pool_alloc(void** ptr, size_t bytes, hipStream_t s) {
     return hipMallocFromPoolAsync(ptr, bytes, pool, s);
 }
 
+// Streams of the _host entry points, reused across calls: creating and
+// destroying a stream costs every call, which matters for small images.  A
+// free list per device; a stream in use belongs to one call, which has
+// synchronised it before handing it back; the streams live until the
+// process ends.
+namespace {
+std::mutex g_hs_mu;
+std::vector<hipStream_t> g_hs_free[kPoolDevices];
+}  // namespace
+
+hipError_t acquire_host_stream(hipStream_t* s) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kPoolDevices) return hipErrorInvalidDevice;
+    {
+        std::lock_guard<std::mutex> lk(g_hs_mu);
+        if (!g_hs_free[dev].empty()) {
+            *s = g_hs_free[dev].back();
+            g_hs_free[dev].pop_back();
+            return hipSuccess;
+        }
+    }
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+void release_host_stream(hipStream_t s) {
+    int dev = 0;
+    if (!s) return;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kPoolDevices) {
+        (void)hipStreamDestroy(s);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_hs_mu);
+    g_hs_free[dev].push_back(s);
+}
+
 // Releases the current device's idle pool memory beyond keep_bytes.
 int trim_pool(uint64_t keep_bytes) {
     int dev = 0;
@@ -615,7 +652,7 @@ int host_forward_serial(const Plan& p, itkg_dtype dtype, int ncomp, const void* 
     int rc = ITKG_OK;
     const size_t px = dtype_size(dtype) * ncomp;
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = acquire_host_stream(&s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     void *din = nullptr, *dout = nullptr;
     e = pool_alloc(&din, (size_t)nin * px, s);
@@ -639,7 +676,7 @@ int host_forward_serial(const Plan& p, itkg_dtype dtype, int ncomp, const void* 
     if (dout) (void)hipFreeAsync(dout, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
+    release_host_stream(s);
     return rc;
 }
 
@@ -710,7 +747,7 @@ int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, 
     if (K <= 1) return host_forward_serial(p, dtype, ncomp, input, output, arith);
 
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = acquire_host_stream(&s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     void *din = nullptr, *dout = nullptr;
     e = pool_alloc(&din, (size_t)nin * px, s);
@@ -747,7 +784,7 @@ int host_forward(const Plan& p, itkg_dtype dtype, int ncomp, const void* input, 
     if (dout) (void)hipFreeAsync(dout, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
+    release_host_stream(s);
     return rc;
 }
 
@@ -1114,7 +1151,7 @@ int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, doubl
     if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
     const size_t px = dtype_size(dtype);
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = acquire_host_stream(&s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     int rc = ITKG_OK;
     void *din = nullptr, *dout = nullptr;
@@ -1135,7 +1172,7 @@ int itkg_unsharp_mask_generate_region_host(int32_t ndim, itkg_dtype dtype, doubl
     if (dout) (void)hipFreeAsync(dout, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
+    release_host_stream(s);
     return rc;
 }
 
@@ -1262,7 +1299,7 @@ int itkg_recursive_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype,
     if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
     const size_t px = dtype_size(dtype);
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = acquire_host_stream(&s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     int rc = ITKG_OK;
     void *din = nullptr, *dout = nullptr;
@@ -1283,7 +1320,7 @@ int itkg_recursive_gaussian_generate_region_host(int32_t ndim, itkg_dtype dtype,
     if (dout) (void)hipFreeAsync(dout, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
+    release_host_stream(s);
     return rc;
 }
 
@@ -1386,7 +1423,7 @@ int itkg_canny_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma
     if (!input || !output) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
     const size_t px = dtype_size(dtype);
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = acquire_host_stream(&s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     int rc = ITKG_OK;
     void* block = nullptr;  // input and output staging in one allocation
@@ -1408,7 +1445,7 @@ int itkg_canny_generate_region_host(int32_t ndim, itkg_dtype dtype, double sigma
     if (block) (void)hipFreeAsync(block, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
+    release_host_stream(s);
     return rc;
 }
 
@@ -1463,7 +1500,7 @@ int itkg_gaussian_backward_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp, d
     if (!grad_output || !grad_input) return fail(ITKG_ERR_INVALID_ARG, "null host buffer");
     const size_t px = dtype_size(dtype) * ncomp;
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = acquire_host_stream(&s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     void *dgo = nullptr, *dgi = nullptr;
     e = pool_alloc(&dgo, (size_t)nout * px, s);
@@ -1490,7 +1527,7 @@ int itkg_gaussian_backward_host(int32_t ndim, itkg_dtype dtype, int32_t ncomp, d
     if (dgi) (void)hipFreeAsync(dgi, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
+    release_host_stream(s);
     return rc;
 }
 
@@ -1529,7 +1566,7 @@ int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype ou
     if ((rc = host_pieces(p, pin, pout, &hp))) return rc;
     const int K = (int)hp.piece.size();
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = acquire_host_stream(&s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     // one block: input pixels, f32 input, f32 output, output pixels
     const size_t bi = (size_t)nin * pin, bf = (size_t)nin * 4, bo = (size_t)nout * 4,
@@ -1541,7 +1578,7 @@ int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype ou
     if (e != hipSuccess) {
         if (block) (void)hipFreeAsync(block, s);
         (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
+        release_host_stream(s);
         return hip_fail(e, "pool_alloc(cast staging)");
     }
     char* base = static_cast<char*>(block);
@@ -1587,7 +1624,7 @@ int itkg_cast_gaussian_cast_host(int32_t ndim, itkg_ptype in_type, itkg_ptype ou
     (void)hipFreeAsync(block, s);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess && !rc) rc = hip_fail(e, "hipStreamSynchronize");
-    (void)hipStreamDestroy(s);
+    release_host_stream(s);
     return rc;
 }
 

@@ -589,6 +589,22 @@ def main():
                 "ms_min": round(float(np.min(ts)) * 1e3, 2),
                 "gvox_s": round(n / float(np.median(ts)) / 1e9, 3)}
             del u8
+            # the casts of that chain on device-resident 512^3 images
+            # (CastFilter, itkg_convert_pixels): 1 + 4 bytes per voxel
+            reg3 = itk.Region([0, 0, 0], [512] * 3)
+            xu = torch.randint(0, 256, (512, 512, 512), dtype=torch.uint8, device=dev)
+            yf = torch.empty((512, 512, 512), dtype=torch.float32, device=dev)
+            iu = itk.Image(reg3, (1.0,) * 3, (0.0,) * 3, xu)
+            of = itk.Image(reg3, (1.0,) * 3, (0.0,) * 3, yf)
+            to_f = itk.CastFilter(iu, torch.float32)
+            to_u = itk.CastFilter(of, torch.uint8, itk.CONVERT_ROUND)
+            _, msf = timed_loop(lambda: to_f.generate_into(iu, of), 20, 3, stream)
+            _, msu = timed_loop(lambda: to_u.generate_into(of, iu), 20, 3, stream)
+            extras["cast_device_512"] = {
+                "u8_to_f32_ms": round(float(np.mean(msf)), 4),
+                "f32_to_u8_round_ms": round(float(np.mean(msu)), 4),
+                "hbm_frac_5B": round(5.0 * n / (float(np.mean(msf)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            del xu, yf, iu, of
             # the reference's CLI case (examples/blur_png.rs): one 512 x 512 u8
             # image, host to host, one piece (no helper threads)
             png = (torch.rand((512, 512), device=dev) * 255).to(torch.uint8).cpu().numpy()

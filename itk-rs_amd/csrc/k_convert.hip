@@ -50,32 +50,145 @@ __device__ __forceinline__ D rust_as(S v) {
 constexpr int kConvBlock = 256;
 
 // One block per (row, chunk of kConvBlock elements of the row); rows
-// enumerate axes 1..3 (axis 1 fastest), elements run over x * ncomp.
+// enumerate axes 1..3 (axis 1 fastest), elements run over x * ncomp.  The
+// row and chunk of an item are uniform across the block; per element only
+// the component split remains, none for scalar pixels and in 32 bits when
+// a row has fewer than 2^32 elements.
 template <class S, class D, bool ROUND>
 __global__ __launch_bounds__(kConvBlock) void convert_kernel(const ConvertArgs a) {
     const int64_t row_elems = a.n[0] * a.ncomp;
     const int64_t chunks = (row_elems + kConvBlock - 1) / kConvBlock;
     const int64_t rows = a.n[1] * a.n[2] * a.n[3];
+    const bool narrow = row_elems <= (int64_t)UINT32_MAX;
+    const uint32_t nc = (uint32_t)a.ncomp;
     for (int64_t item = blockIdx.x; item < rows * chunks; item += gridDim.x) {
         const int64_t row = item / chunks, c = item - row * chunks;
         const int64_t y = row % a.n[1], zw = row / a.n[1];
         const int64_t z = zw % a.n[2], w = zw / a.n[2];
         const int64_t j = c * kConvBlock + threadIdx.x;
         if (j >= row_elems) continue;
-        const int64_t x = j / a.ncomp, e = j - x * a.ncomp;
+        int64_t x, e;
+        if (nc == 1) {
+            x = j;
+            e = 0;
+        } else if (narrow) {
+            const uint32_t q = (uint32_t)j / nc;
+            x = q;
+            e = (int64_t)((uint32_t)j - q * nc);
+        } else {
+            x = j / nc;
+            e = j - x * nc;
+        }
         const int64_t si = (x * a.in_stride[0] + y * a.in_stride[1] + z * a.in_stride[2] +
-                            w * a.in_stride[3]) * a.ncomp + e;
+                            w * a.in_stride[3]) * nc + e;
         const int64_t di = (x * a.out_stride[0] + y * a.out_stride[1] + z * a.out_stride[2] +
-                            w * a.out_stride[3]) * a.ncomp + e;
+                            w * a.out_stride[3]) * nc + e;
         S v = static_cast<const S*>(a.in)[si];
         if constexpr (ROUND && std::is_floating_point_v<S>) v = round_half_away(v);
         static_cast<D*>(a.out)[di] = rust_as<D>(v);
     }
 }
 
+// The common case: blockIdx.y is the row's axis-1 index, blockIdx.z its
+// axis-2/3 plane, blockIdx.x a span of U * kConvBlock elements of the row
+// (U = 1..4 chosen so that short rows do not leave threads idle), so no
+// index needs a division beyond the component split.  Needs n[1] and
+// n[2] * n[3] within the grid limits and rows under 2^31 elements.
+constexpr int kConvMaxU = 4;
+
+template <class S, class D, bool ROUND>
+__global__ __launch_bounds__(kConvBlock) void convert_rows_kernel(const ConvertArgs a,
+                                                                  const int u) {
+    const uint32_t nc = (uint32_t)a.ncomp;
+    const uint32_t n2 = (uint32_t)a.n[2];
+    const int64_t y = blockIdx.y, z = blockIdx.z % n2, w = blockIdx.z / n2;
+    const int64_t in_row = y * a.in_stride[1] + z * a.in_stride[2] + w * a.in_stride[3];
+    const int64_t out_row = y * a.out_stride[1] + z * a.out_stride[2] + w * a.out_stride[3];
+    const uint32_t row_elems = (uint32_t)(a.n[0] * nc);
+    const uint32_t j0 = blockIdx.x * (uint32_t)(u * kConvBlock) + threadIdx.x;
+    const S* in = static_cast<const S*>(a.in);
+    D* out = static_cast<D*>(a.out);
+#pragma unroll
+    for (int k = 0; k < kConvMaxU; ++k) {
+        const uint32_t j = j0 + (uint32_t)(k * kConvBlock);
+        if (k >= u || j >= row_elems) break;
+        const uint32_t x = nc == 1 ? j : j / nc;
+        const uint32_t e = j - x * nc;
+        S v = in[((int64_t)x * a.in_stride[0] + in_row) * nc + e];
+        if constexpr (ROUND && std::is_floating_point_v<S>) v = round_half_away(v);
+        out[((int64_t)x * a.out_stride[0] + out_row) * nc + e] = rust_as<D>(v);
+    }
+}
+
+// Four consecutive elements per lane and iteration, as one load and one
+// store each: dense scalar rows whose starts are all aligned to four
+// elements (the host checks the pointers and strides).
+template <class T>
+struct alignas(4 * sizeof(T)) Quad {
+    T v[4];
+};
+
+template <class S, class D, bool ROUND>
+__global__ __launch_bounds__(kConvBlock) void convert_rows4_kernel(const ConvertArgs a,
+                                                                   const int u) {
+    const uint32_t n2 = (uint32_t)a.n[2];
+    const int64_t y = blockIdx.y, z = blockIdx.z % n2, w = blockIdx.z / n2;
+    const int64_t in_row = y * a.in_stride[1] + z * a.in_stride[2] + w * a.in_stride[3];
+    const int64_t out_row = y * a.out_stride[1] + z * a.out_stride[2] + w * a.out_stride[3];
+    const uint32_t quads = (uint32_t)(a.n[0] / 4);
+    const uint32_t q0 = blockIdx.x * (uint32_t)(u * kConvBlock) + threadIdx.x;
+    const Quad<S>* in = reinterpret_cast<const Quad<S>*>(static_cast<const S*>(a.in) + in_row);
+    Quad<D>* out = reinterpret_cast<Quad<D>*>(static_cast<D*>(a.out) + out_row);
+#pragma unroll
+    for (int k = 0; k < kConvMaxU; ++k) {
+        const uint32_t q = q0 + (uint32_t)(k * kConvBlock);
+        if (k >= u || q >= quads) break;
+        const Quad<S> v = in[q];
+        Quad<D> r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            S t = v.v[i];
+            if constexpr (ROUND && std::is_floating_point_v<S>) t = round_half_away(t);
+            r.v[i] = rust_as<D>(t);
+        }
+        out[q] = r;
+    }
+}
+
+template <class S, class D>
+bool quad_ok(const ConvertArgs& a) {
+    if (a.ncomp != 1 || a.in_stride[0] != 1 || a.out_stride[0] != 1 || a.n[0] % 4) return false;
+    if ((uintptr_t)a.in % (4 * sizeof(S)) || (uintptr_t)a.out % (4 * sizeof(D))) return false;
+    for (int d = 1; d < ITKG_MAX_DIM; ++d)
+        if ((a.n[d] > 1) && (a.in_stride[d] % 4 || a.out_stride[d] % 4)) return false;
+    return true;
+}
+
 template <class S, class D>
 hipError_t launch_sd(const ConvertArgs& a, bool round, hipStream_t s) {
     const int64_t row_elems = a.n[0] * a.ncomp;
+    if (row_elems < (INT64_C(1) << 31) && a.n[1] <= 65535 && a.n[2] * a.n[3] <= 65535 &&
+        quad_ok<S, D>(a)) {
+        const int64_t quads = a.n[0] / 4;
+        const int u = (int)std::min<int64_t>(kConvMaxU, (quads + kConvBlock - 1) / kConvBlock);
+        const dim3 g((unsigned)((quads + (int64_t)u * kConvBlock - 1) / ((int64_t)u * kConvBlock)),
+                     (unsigned)a.n[1], (unsigned)(a.n[2] * a.n[3]));
+        if (round)
+            hipLaunchKernelGGL((convert_rows4_kernel<S, D, true>), g, dim3(kConvBlock), 0, s, a, u);
+        else
+            hipLaunchKernelGGL((convert_rows4_kernel<S, D, false>), g, dim3(kConvBlock), 0, s, a, u);
+        return hipGetLastError();
+    }
+    if (row_elems < (INT64_C(1) << 31) && a.n[1] <= 65535 && a.n[2] * a.n[3] <= 65535) {
+        const int u = (int)std::min<int64_t>(kConvMaxU, (row_elems + kConvBlock - 1) / kConvBlock);
+        const dim3 g((unsigned)((row_elems + (int64_t)u * kConvBlock - 1) / ((int64_t)u * kConvBlock)),
+                     (unsigned)a.n[1], (unsigned)(a.n[2] * a.n[3]));
+        if (round)
+            hipLaunchKernelGGL((convert_rows_kernel<S, D, true>), g, dim3(kConvBlock), 0, s, a, u);
+        else
+            hipLaunchKernelGGL((convert_rows_kernel<S, D, false>), g, dim3(kConvBlock), 0, s, a, u);
+        return hipGetLastError();
+    }
     const int64_t items = ((row_elems + kConvBlock - 1) / kConvBlock) * a.n[1] * a.n[2] * a.n[3];
     const unsigned grid = (unsigned)std::min<int64_t>(items, 1 << 20);
     if (round)

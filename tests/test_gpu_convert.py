@@ -44,15 +44,18 @@ def _same(a, b):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_every_type_pair_matches_rust_casts(itk, orc, dev, src, mode):
     rng = np.random.default_rng(2 * TYPES.index(src) + mode)
-    x = _values(src, rng)
-    xin = itk.Image(itk.Region([0], [x.size]), (1.0,), (0.0,), torch.from_numpy(x).to(dev))
-    for dst in TYPES:
-        out = itk.Image.empty(xin.region, (1.0,), (0.0,), 1, TORCH[dst], dev)
-        itk.CastFilter(xin, TORCH[dst], mode).generate_into(xin, out)
-        torch.cuda.synchronize()
-        v = orc.rust_round(x) if (mode == 1 and np.issubdtype(np.dtype(src), np.floating)) else x
-        ref = orc.rust_as(v, dst)
-        assert _same(out.data.cpu().numpy(), ref), (np.dtype(src), np.dtype(dst), mode)
+    full = _values(src, rng)
+    # an odd length (one element per lane) and a multiple of four (dense
+    # aligned rows: four elements per lane)
+    for x in (full, np.ascontiguousarray(full[: full.size // 4 * 4])):
+        xin = itk.Image(itk.Region([0], [x.size]), (1.0,), (0.0,), torch.from_numpy(x).to(dev))
+        for dst in TYPES:
+            out = itk.Image.empty(xin.region, (1.0,), (0.0,), 1, TORCH[dst], dev)
+            itk.CastFilter(xin, TORCH[dst], mode).generate_into(xin, out)
+            torch.cuda.synchronize()
+            v = orc.rust_round(x) if (mode == 1 and np.issubdtype(np.dtype(src), np.floating)) else x
+            ref = orc.rust_as(v, dst)
+            assert _same(out.data.cpu().numpy(), ref), (np.dtype(src), np.dtype(dst), mode, x.size)
 
 
 def test_strided_subregion_and_components(itk, orc, dev):
@@ -109,3 +112,15 @@ def test_host_chain_u16_to_f32(itk, orc):
     f = orc.rust_as(img, np.float32)
     ref = orc.generate_region(f, [0, 0, 0], [1.0] * 3, 1.7, [0, 0, 0], [29, 47, 33])
     assert _same(got, ref)
+
+
+def test_rows_beyond_the_grid_limit(itk, orc, dev):
+    # axis 1 longer than 65535 rows: the generic conversion kernel
+    rng = np.random.default_rng(4)
+    x = (rng.standard_normal((70001, 5)) * 200).astype(np.float32)
+    img = itk.Image(itk.Region([0, 0], [5, 70001]), (1.0, 1.0), (0.0, 0.0), torch.from_numpy(x).to(dev))
+    for dst, mode in ((torch.uint8, itk.CONVERT_ROUND), (torch.int16, itk.CONVERT_AS)):
+        out = itk.CastFilter(img, dst, mode).generate_region(img.region)
+        v = orc.rust_round(x) if mode == itk.CONVERT_ROUND else x
+        ref = orc.rust_as(v, np.uint8 if dst == torch.uint8 else np.int16)
+        assert _same(out.data.cpu().numpy(), ref), dst

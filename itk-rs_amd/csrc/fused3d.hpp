@@ -84,8 +84,11 @@ __device__ __forceinline__ void lds_barrier_tracked() {
 //   FAST and adjoint: 16 waves; 4 columns (64-row tiles) up to rz 7, then 2;
 //   their _S variants: 16 waves x 2 columns (32-row tiles) at every radius;
 //   EXACT (a larger register need: the product and the sum round
-//   separately): 16 waves x 2 columns up to rz 8 and at rz 15; 8 x 4
-//   (rz 9, 10) or 8 x 2 (rz 11..14), where 16 waves would spill.
+//   separately): 16 waves x 4 columns (64-row tiles) up to rz 5, 16 x 2
+//   at rz 6..8 and 15 (4 columns spill there: 20 VGPRs at rz 6); 8 x 4
+//   (rz 9, 10) or 8 x 2 (rz 11..14), where 16 waves would spill.  The
+//   64-row tiles measured 8-10% faster at 512^3 (rz 3, 5) and 22% on the
+//   VecPixel radii (13, 13, 4), bit-identical.
 template <int RXY, int RZ, int MODE>
 struct Cfg {
     static constexpr bool EXACT_MID = MODE == FUSED_FWD_EXACT && RZ >= 9 && RZ <= 14;
@@ -96,7 +99,7 @@ struct Cfg {
     // radii from 18 up (FMA modes): one column per lane, so that the ring
     // (2r + 1 registers a column) fits
     static constexpr int CPT = MODE == FUSED_FWD_EXACT
-                                   ? (EXACT_MID && RZ <= 10 ? 4 : 2)
+                                   ? ((EXACT_MID && RZ <= 10) || RZ <= 5 ? 4 : 2)
                                    : (RZ >= 18 ? 1 : (RZ <= 7 && !SMALL ? 4 : 2));
     static constexpr int TY = NW * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride

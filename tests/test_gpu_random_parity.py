@@ -5,7 +5,9 @@ pixel to radii past the fused set, anisotropic spacings, sub-region
 requests (inside and touching the border), 1 or 3 components, f32 and f64,
 and both kernel paths; then MeanFilter / BinomialBlurFilter the same way;
 the adjoint against the oracle's f64 transpose; random batched plans
-against plain launches, forward and adjoint."""
+against plain launches, forward and adjoint; the host-buffer entry point
+at staged sizes (pinned or pageable) against the oracle; random casts over
+strided windows against the oracle's Rust casts."""
 import numpy as np
 import pytest
 import torch
@@ -155,3 +157,78 @@ def test_random_batch_equals_plain(itk, dev, seed, direction):
             torch.cuda.synchronize()
             assert torch.equal(o, ref), (seed, tune, tuple(v.shape), sg, sp)
     plan.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_host_staging_parity(itk, orc, dev, seed):
+    # host buffers large enough to run as staged pieces (over 32 MiB of f32),
+    # odd extents, sub-requests: EXACT against the oracle bit for bit,
+    # through the pinned ring or pageable copies at random
+    import ctypes
+    import os
+    rng = np.random.default_rng(3000 + seed)
+    size = [int(rng.integers(150, 330)), int(rng.integers(150, 330)), int(rng.integers(120, 260))]
+    sigma = float(rng.choice([0.8, 1.3, 2.0, 3.1]))
+    lo = [int(rng.integers(0, s // 3)) for s in size]
+    req = (lo, [int(rng.integers(s // 2, s - l + 1)) for s, l in zip(size, lo)])
+    x = rng.standard_normal(tuple(reversed(size))).astype(np.float32)
+    largest = itk.Region([0, 0, 0], size)
+    r = itk.Region(*req)
+    in_reg = itk.GaussianFilter(itk.Image(largest, (1.0,) * 3, (0.0,) * 3, None), sigma) \
+        .input_region_for_output(r)
+    sl = tuple(slice(in_reg.index[d], in_reg.index[d] + in_reg.size[d]) for d in reversed(range(3)))
+    xin = np.ascontiguousarray(x[sl])
+    out = np.empty(tuple(reversed(r.size)), dtype=np.float32)
+    mode = "pageable" if rng.random() < 0.4 else None
+    old = os.environ.pop("ITKG_HOST_STAGING", None)
+    if mode:
+        os.environ["ITKG_HOST_STAGING"] = mode
+    try:
+        rc = itk.lib.itkg_gaussian_generate_region_host(
+            3, itk.F32, 1, sigma, (ctypes.c_double * 3)(1.0, 1.0, 1.0),
+            ctypes.byref(largest.to_c()), xin.ctypes.data, ctypes.byref(r.to_c()),
+            out.ctypes.data, itk.EXACT)
+    finally:
+        os.environ.pop("ITKG_HOST_STAGING", None)
+        if old is not None:
+            os.environ["ITKG_HOST_STAGING"] = old
+    assert rc == 0, itk.lib.itkg_last_error()
+    ref = orc.generate_region(x, [0, 0, 0], [1.0] * 3, sigma, list(r.index), list(r.size))
+    assert np.array_equal(out, ref), (seed, size, req, sigma, mode)
+
+
+_CAST_TYPES = [(np.uint8, torch.uint8), (np.int16, torch.int16), (np.int32, torch.int32),
+               (np.float32, torch.float32), (np.float64, torch.float64)]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_cast_parity(itk, orc, dev, seed):
+    # random type pairs, modes, shapes, components and windows of larger
+    # buffers (strided views) against the oracle's Rust casts, bit for bit
+    rng = np.random.default_rng(4000 + seed)
+    D = int(rng.integers(1, 4))
+    ncomp = int(rng.choice([1, 1, 2, 3]))
+    (snp, st), (dnp, dt) = [_CAST_TYPES[i] for i in rng.choice(len(_CAST_TYPES), 2)]
+    mode = int(rng.integers(0, 2))
+    big = [int(rng.integers(4, 90 if D < 3 else 40)) for _ in range(D)]
+    shape = tuple(reversed(big)) + ((ncomp,) if ncomp > 1 else ())
+    if np.issubdtype(snp, np.floating):
+        x = (rng.standard_normal(shape) * rng.choice([1.0, 300.0, 1e6])).astype(snp)
+    else:
+        info = np.iinfo(snp)
+        x = rng.integers(info.min, int(info.max) + 1, shape, dtype=np.int64).astype(snp)
+    img = itk.Image(itk.Region([0] * D, big), (1.0,) * D, (0.0,) * D, torch.from_numpy(x).to(dev))
+    lo = [int(rng.integers(0, b)) for b in big]
+    sub = itk.Region(lo, [int(rng.integers(1, b - l + 1)) for b, l in zip(big, lo)])
+    # the output is the same window of a second buffer
+    ob = torch.zeros(shape, dtype=dt, device=dev)
+    obig = itk.Image(img.region, img.spacing, img.origin, ob)
+    win = itk.Image(sub, img.spacing, img.origin, obig.data[obig._slices(sub)])
+    itk.CastFilter(img, dt, mode).generate_into(img, win)
+    torch.cuda.synchronize()
+    v = orc.rust_round(x) if (mode == 1 and np.issubdtype(snp, np.floating)) else x
+    ref = np.zeros(shape, dtype=dnp)
+    sl = tuple(slice(sub.index[d], sub.index[d] + sub.size[d]) for d in reversed(range(D)))
+    ref[sl] = orc.rust_as(v[sl], dnp)
+    got = ob.cpu().numpy()
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (seed, snp, dnp, mode, big, ncomp)

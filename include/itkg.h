@@ -409,7 +409,8 @@ int itkg_trim_scratch(uint64_t keep_bytes);
  * into pieces along the slowest axis, and the input and output move in
  * 12 MiB chunks, overlapped with each other and with the pieces' kernels,
  * through a process-wide ring of pinned host slots (3 per direction) that
- * 4 copy threads per direction fill and empty (ITKG_STAGE_CHUNK_MIB,
+ * 6 copy threads per direction (at most half the host's cores) fill and
+ * empty (ITKG_STAGE_CHUNK_MIB,
  * ITKG_STAGE_SLOTS and ITKG_STAGE_THREADS set others).  One call at a time uses
  * the ring; a call that finds it in use, or any call when the environment
  * sets ITKG_HOST_STAGING=pageable, copies straight from and to the caller's

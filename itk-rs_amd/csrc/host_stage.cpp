@@ -44,11 +44,13 @@ namespace {
 // Measured on 512^3 f32 host to host (1 GiB over the link): 3 slots of
 // 12 MiB per direction 13.4 ms, 4 of 8 MiB 14.3 ms, 4 of 16 MiB 15.3 ms, 8
 // or 16 slots 19-21 ms (a deeper inbound queue holds the outbound DMA back),
-// 2 slots 15.8-16.9 ms; 4 copy threads per direction as fast as 6 or 8.
+// 2 slots 15.8-16.9 ms.  Copy threads per direction: f32 the same with 4,
+// 6 or 8 (the DMA is the limit); the 512^3 u8 chain 6.1 ms with 4, 5.2 with
+// 6 or 8 (copying out into fresh pages is the limit there).
 constexpr size_t kStageChunk = size_t(12) << 20;  // bytes per chunk and per slot
 constexpr int kStageSlots = 3;                     // slots per direction
 constexpr int kStageMaxSlots = 16;
-constexpr int kStageThreads = 4;                   // copy threads per direction
+constexpr int kStageThreads = 6;                   // copy threads per direction
 
 // ITKG_STAGE_THREADS / ITKG_STAGE_CHUNK_MIB / ITKG_STAGE_SLOTS override the
 // copy threads per direction (1-16), the chunk size (1-256 MiB) and the

@@ -609,4 +609,78 @@ UnaryFilter<S, F, Q> make_unary(S src, F f) {
     return UnaryFilter<S, F, Q>{std::move(src), std::move(f)};
 }
 
+// ---- the casts around a Gaussian (examples/blur_png.rs:35-40) ------------
+
+template <class T>
+struct PTypeOf;
+template <>
+struct PTypeOf<uint8_t> {
+    static constexpr itkg_ptype value = ITKG_PT_U8;
+};
+template <>
+struct PTypeOf<uint16_t> {
+    static constexpr itkg_ptype value = ITKG_PT_U16;
+};
+template <>
+struct PTypeOf<int16_t> {
+    static constexpr itkg_ptype value = ITKG_PT_I16;
+};
+template <>
+struct PTypeOf<int32_t> {
+    static constexpr itkg_ptype value = ITKG_PT_I32;
+};
+template <>
+struct PTypeOf<float> {
+    static constexpr itkg_ptype value = ITKG_PT_F32;
+};
+template <>
+struct PTypeOf<double> {
+    static constexpr itkg_ptype value = ITKG_PT_F64;
+};
+
+// UnaryFilter(p as f32) -> GaussianFilter{sigma} -> UnaryFilter(cast to Q)
+// as one node on host images (itkg_cast_gaussian_cast_host): the pixels
+// cross PCIe in their own types and the f32 intermediates stay on the GPU.
+// out_mode ITKG_CONVERT_ROUND is `p.round() ... as Q` (for u8 the
+// reference's clamp(0, 255) changes nothing the saturating cast does not),
+// ITKG_CONVERT_AS is `p as Q`.
+template <class Q, class S>
+struct CastGaussianCast {
+    using A = typename S::pixel_type;
+    using pixel_type = Q;
+    static constexpr size_t dim = S::dim;
+    static constexpr size_t D = dim;
+    S source;
+    double sigma;
+    itkg_convert_mode out_mode = ITKG_CONVERT_ROUND;
+    // Arithmetic of the Gaussian inside (EXACT reproduces the reference).
+    static inline itkg_arith arith = ITKG_EXACT;
+
+    Region<D> largest_region() const { return source.largest_region(); }
+    std::array<double, D> spacing() const { return source.spacing(); }
+    std::array<double, D> origin() const { return source.origin(); }
+    Region<D> input_region_for_output(const Region<D>& output) const {
+        std::array<uint64_t, D> r{};
+        const auto sp = spacing();
+        for (size_t d = 0; d < D; ++d) r[d] = kernel_radius(sigma, sp[d]);
+        return output.padded_per_axis(r).clipped_to(source.largest_region());
+    }
+    Image<Q, D> generate_region(Region<D> requested) const {
+        const Image<A, D> input = source.generate_region(input_region_for_output(requested));
+        Image<Q, D> out = Image<Q, D>::allocate(requested, spacing(), origin(), Q{});
+        const auto sp = spacing();
+        const itkg_region cb = source.largest_region().to_c(), cr = requested.to_c();
+        check(itkg_cast_gaussian_cast_host((int32_t)D, PTypeOf<A>::value, PTypeOf<Q>::value, sigma,
+                                           sp.data(), &cb, input.data.data(), &cr, out.data.data(),
+                                           arith, out_mode));
+        return out;
+    }
+};
+
+template <class Q, class S>
+CastGaussianCast<Q, S> make_cast_gaussian_cast(S src, double sigma,
+                                               itkg_convert_mode out_mode = ITKG_CONVERT_ROUND) {
+    return CastGaussianCast<Q, S>{std::move(src), sigma, out_mode};
+}
+
 }  // namespace itk

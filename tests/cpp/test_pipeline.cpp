@@ -350,6 +350,35 @@ static void test_gaussian_backward_accumulate() {
     CHECK(std::fabs(s - 0.5 * 12 * 9) < 1e-3, "gradient mass %g", s);
 }
 
+// examples/blur_png.rs:35-40: u8 -> f32, GaussianFilter, round and clamp
+// back to u8.  The one-node GPU chain equals the three-node chain (CPU casts
+// around the GPU Gaussian) bit for bit, on the whole image and on a
+// sub-request.
+static void test_blur_png_chain() {
+    std::array<double, 2> one{1.0, 1.0}, zero{};
+    const auto largest = Region<2>::make({0, 0}, {67, 45});
+    auto img = Image<uint8_t, 2>::allocate(largest, one, zero, 0);
+    uint32_t st = 7u;
+    for (auto& v : img.data) {
+        st = st * 1664525u + 1013904223u;
+        v = (uint8_t)(st >> 24);
+    }
+    img.set_pixel(Index<2>{{20, 10}}, 255);
+    auto as_f32 = make_unary<float>(img, [](uint8_t p) { return (float)p; });
+    auto blurred = make_gaussian(as_f32, 1.7);
+    auto as_u8 = make_unary<uint8_t>(blurred, [](float p) {
+        return (uint8_t)std::fmin(std::fmax(std::round(p), 0.0f), 255.0f);
+    });
+    auto fused = make_cast_gaussian_cast<uint8_t>(img, 1.7);
+    for (const auto& req : {largest, Region<2>::make({9, 4}, {30, 21})}) {
+        const auto want = as_u8.generate_region(req);
+        const auto got = fused.generate_region(req);
+        CHECK(got.region == want.region, "region");
+        for (size_t i = 0; i < got.data.size(); ++i)
+            CHECK(got.data[i] == want.data[i], "pixel %zu: %d vs %d", i, got.data[i], want.data[i]);
+    }
+}
+
 int main() {
 #define RUN(t)                  \
     t();                        \
@@ -376,6 +405,7 @@ int main() {
     RUN(test_recursive_gaussian);
     RUN(test_gaussian_backward_transpose);
     RUN(test_gaussian_backward_accumulate);
+    RUN(test_blur_png_chain);
     std::printf("all passed\n");
     return 0;
 }

@@ -139,6 +139,20 @@ int itkg_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
                                   const itkg_view* input, const itkg_view* output,
                                   itkg_arith arith, void* stream);
 
+/* Several requests of one GaussianFilter over one input view: output k
+ * receives generate_region(outputs[k].region), exactly as n calls of
+ * itkg_gaussian_generate_region with the same input would give it (input
+ * must cover every request's input_region_for_output).  Where the fused
+ * kernel serves them all (3-D, f32 scalar pixels, isotropic radii) the
+ * pieces run as one persistent launch over all their tiles; otherwise one
+ * call per piece.  For stream_full's pieces over a resident volume (each
+ * piece still reads its own halo).  No counterpart in the reference, whose
+ * stream_full calls generate_region piece by piece. */
+int itkg_gaussian_generate_regions(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                                   const double* spacing, const itkg_region* largest,
+                                   const itkg_view* input, int32_t n, const itkg_view* outputs,
+                                   itkg_arith arith, void* stream);
+
 /* No counterpart in the reference: itkg_gaussian_generate_region for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same arguments and

@@ -50,7 +50,7 @@ EXPORTS = (
     "itkg_recursive_gaussian_generate_region", "itkg_recursive_gaussian_generate_region_host",
     "itkg_canny_generate_region", "itkg_canny_generate_region_host", "itkg_trim_scratch",
     "itkg_convert_pixels", "itkg_cast_gaussian_cast_host", "itkg_host_staging_info",
-    "itkg_gaussian_backward_host", "itkg_canny_direction_bins",
+    "itkg_gaussian_backward_host", "itkg_canny_direction_bins", "itkg_gaussian_generate_regions",
 )
 
 
@@ -93,6 +93,9 @@ def _load():
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
         P(c_region), P(c_view), P(c_view), ctypes.c_int, ctypes.c_void_p]
     L.itkg_gaussian_generate_region_noalloc.argtypes = L.itkg_gaussian_generate_region.argtypes
+    L.itkg_gaussian_generate_regions.argtypes = [
+        ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
+        P(c_region), P(c_view), ctypes.c_int32, P(c_view), ctypes.c_int, ctypes.c_void_p]
     L.itkg_gaussian_generate_region_host.argtypes = [
         ctypes.c_int32, ctypes.c_int, ctypes.c_int32, ctypes.c_double, P(ctypes.c_double),
         P(c_region), ctypes.c_void_p, P(c_region), ctypes.c_void_p, ctypes.c_int]
@@ -516,6 +519,23 @@ def stream_full(source, num_pieces: int) -> Image:
     else:
         out = Image.allocate(full, source.spacing, source.origin, 0.0, nc, first.data.dtype,
                              first.data.device)
+    if isinstance(source, GaussianFilter) and isinstance(source.source, Image) and \
+            source.source.data.is_cuda:
+        # every piece from the resident leaf in one call (one persistent
+        # launch over all pieces' tiles where the fused kernel serves them;
+        # each piece still reads its own halo, with the same results)
+        leaf = source.source
+        slabs = [Image(piece, out.spacing, out.origin, out.data[out._slices(piece)])
+                 for piece in split_region(full, num_pieces)]
+        views = (c_view * len(slabs))(*[sl.view() for sl in slabs])
+        D = full.ndim
+        dt, nc = _pixel_info(leaf.data, D)
+        with _on_device(leaf.data.device):
+            _check(lib.itkg_gaussian_generate_regions(
+                D, dt, nc, source.sigma, _doubles(source.spacing),
+                ctypes.byref(source.largest_region().to_c()), ctypes.byref(leaf.view()),
+                len(slabs), views, source.arith, _stream_ptr(leaf.data.device)))
+        return out
     for piece in split_region(full, num_pieces):
         if isinstance(source, GaussianFilter):
             in_reg = source.input_region_for_output(piece)

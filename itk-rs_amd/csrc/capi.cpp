@@ -643,6 +643,93 @@ int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout
                           : backward_per_axis<float>(p, ncomp, gout, gin, accumulate, s, scratch);
 }
 
+// Several requests over one input view (stream_full's pieces): one
+// persistent launch of the fused batch kernel over every piece's tiles when
+// the fused path serves them all (3-D, f32 scalar pixels, isotropic radii,
+// non-empty requests), otherwise one forward_impl per piece.  Each piece is
+// filled exactly as its own launch would fill it (same arguments, same
+// arithmetic), only the z run and the tile shape come from the work of all
+// pieces together, so the results are bit-identical either way.
+int multi_forward(const std::vector<Plan>& plans, itkg_dtype dt, int ncomp, const itkg_view& in,
+                  const itkg_view* outs, itkg_arith arith, hipStream_t s) {
+    const int n = (int)plans.size();
+    const FusedEntry* fe = n > 1 ? fused_for(plans[0]) : nullptr;
+    bool one = fe && fe->batch && plans[0].D == 3 && fe->rz == fe->rxy && ncomp == 1 &&
+               use_fused(plans[0], dt, ncomp);
+    int64_t nz_max = 0;
+    for (int k = 0; k < n && one; ++k) {
+        const Plan& p = plans[k];
+        one = fused_for(p) == fe && fits_i32(p) && fused_src_ok(p, in, ncomp, false) &&
+              region_len(3, p.req) > 0;
+        nz_max = std::max<int64_t>(nz_max, p.req.size[2]);
+    }
+    if (!one) {
+        for (int k = 0; k < n; ++k)
+            if (int rc = forward_impl(plans[k], dt, ncomp, in, outs[k], arith, s, nullptr)) return rc;
+        return ITKG_OK;
+    }
+    const int rz = (int)plans[0].radii[2], rxy = (int)plans[0].radii[0];
+    // tile shape and z run from all pieces' tiles (as pick_shape and
+    // fused_tz model one launch)
+    auto tiles_xy = [&](int m) {
+        int64_t t = 0;
+        for (const Plan& p : plans)
+            t += ((p.req.size[0] + kFusedTileX - 1) / kFusedTileX) *
+                 ((p.req.size[1] + fe->ty[m] - 1) / fe->ty[m]);
+        return (int)std::min<int64_t>(t, INT32_MAX);
+    };
+    int mode = arith == ITKG_EXACT ? FUSED_FWD_EXACT : FUSED_FWD_FAST;
+    if (mode == FUSED_FWD_FAST) {
+        auto cost = [&](int m) {
+            const int txy = tiles_xy(m);
+            const int tz = fused_tz(txy, (int)nz_max, rz);
+            const int64_t tiles = (int64_t)txy * ((nz_max + tz - 1) / tz);
+            const int64_t cus = device_cus();
+            return (double)((tiles + cus - 1) / cus) * (double)(std::min<int64_t>(tz, nz_max) + 2 * rz + 3) *
+                   (3.0 * fe->ty[m] + 2.0 * rxy + 30.0);
+        };
+        if (const char* e = std::getenv("ITKG_FUSED_SHAPE"))
+            mode = std::atoi(e) == 32 ? FUSED_FWD_FAST_S : FUSED_FWD_FAST;
+        else if (cost(FUSED_FWD_FAST_S) < cost(FUSED_FWD_FAST))
+            mode = FUSED_FWD_FAST_S;
+    }
+    const int tz = fused_tz(tiles_xy(mode), (int)nz_max, rz);
+    std::vector<Fused3dArgs> args((size_t)n);
+    std::vector<int2> items;
+    for (int k = 0; k < n; ++k) {
+        fill_fused_args(plans[k], in, outs[k], mode, 0, tz, 0, 1, &args[k]);
+        const int nt = args[k].tiles_x * args[k].tiles_y * args[k].tiles_z;
+        for (int t = 0; t < nt; ++t) items.push_back(make_int2(k, t));
+    }
+    // workgroup b runs items b, b + G, ... on XCD b mod 8 (G a multiple of
+    // 8): 8 contiguous chunks keep neighbouring tiles on one XCD's L2
+    const int ni = (int)items.size(), q = (ni + 7) / 8;
+    std::vector<int2> work((size_t)q * 8, make_int2(-1, 0));
+    for (int i = 0; i < ni; ++i) work[(size_t)(i % q) * 8 + i / q] = items[i];
+    const size_t ab = args.size() * sizeof(Fused3dArgs), wb = work.size() * sizeof(int2);
+    void* block = nullptr;
+    hipError_t e = pool_alloc(&block, (ab + 255) / 256 * 256 + wb, s);
+    if (e != hipSuccess) return hip_fail(e, "pool_alloc(piece tables)");
+    char* base = static_cast<char*>(block);
+    // copies from pageable memory: the host buffers may go once they return
+    e = hipMemcpyAsync(base, args.data(), ab, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + (ab + 255) / 256 * 256, work.data(), wb, hipMemcpyHostToDevice, s);
+    int rc = ITKG_OK;
+    if (e != hipSuccess) rc = hip_fail(e, "piece tables");
+    if (!rc) {
+        const int per_cu = std::max(1, fe->occupancy(mode));
+        const int nwork = (int)work.size();
+        const int grid = std::min(nwork, ((device_cus() * per_cu + 7) / 8) * 8);
+        e = fe->batch(reinterpret_cast<const Fused3dArgs*>(base),
+                      reinterpret_cast<const int2*>(base + (ab + 255) / 256 * 256), nwork, grid,
+                      mode, s);
+        if (e != hipSuccess) rc = hip_fail(e, "piece launch");
+    }
+    (void)hipFreeAsync(block, s);
+    return rc;
+}
+
 // Host buffers, one piece: stage the input region in, run the plan, stage
 // the requested region out, on a private stream.
 int host_forward_serial(const Plan& p, itkg_dtype dtype, int ncomp, const void* input,
@@ -947,6 +1034,26 @@ int itkg_gaussian_generate_region(int32_t ndim, itkg_dtype dtype, int32_t ncomp,
     if ((rc = check_view(ndim, output, p.req, "output"))) return rc;
     return forward_impl(p, dtype, ncomp, *input, *output, arith,
                         static_cast<hipStream_t>(stream), nullptr);
+}
+
+int itkg_gaussian_generate_regions(int32_t ndim, itkg_dtype dtype, int32_t ncomp, double sigma,
+                                   const double* spacing, const itkg_region* largest,
+                                   const itkg_view* input, int32_t n, const itkg_view* outputs,
+                                   itkg_arith arith, void* stream) {
+    clear_error();
+    if (!args_ok(ndim, dtype, ncomp)) return fail(ITKG_ERR_INVALID_ARG, "bad ndim/dtype/ncomp");
+    if (n < 0) return fail(ITKG_ERR_INVALID_ARG, "negative count");
+    if (n == 0) return ITKG_OK;
+    if (!outputs) return fail(ITKG_ERR_INVALID_ARG, "null output views");
+    std::vector<Plan> plans((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        int rc = make_plan(ndim, sigma, spacing, largest, outputs[k].region, &plans[k]);
+        if (!rc) rc = check_view(ndim, input, plans[k].in_reg, "input");
+        if (!rc) rc = check_view(ndim, &outputs[k], plans[k].req, "output");
+        if (rc) return rc;
+    }
+    return multi_forward(plans, dtype, ncomp, *input, outputs, arith,
+                         static_cast<hipStream_t>(stream));
 }
 
 int itkg_gaussian_generate_region_noalloc(int32_t ndim, itkg_dtype dtype, int32_t ncomp,

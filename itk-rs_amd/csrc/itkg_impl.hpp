@@ -241,6 +241,9 @@ int forward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& in,
                  const itkg_view& out, itkg_arith arith, hipStream_t s, void* scratch);
 int backward_impl(const Plan& p, itkg_dtype dt, int ncomp, const itkg_view& gout,
                   const itkg_view& gin, int accumulate, hipStream_t s, void* scratch);
+// Several forward requests over one input view (itkg_gaussian_generate_regions).
+int multi_forward(const std::vector<Plan>& plans, itkg_dtype dt, int ncomp, const itkg_view& in,
+                  const itkg_view* outs, itkg_arith arith, hipStream_t s);
 
 // A piecewise computation between dense host buffers and dense device
 // buffers of the same byte layout (host_stage.cpp).  Piece k reads the input

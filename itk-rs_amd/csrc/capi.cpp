@@ -507,7 +507,8 @@ void fill_fused_args(const Plan& p, const itkg_view& src_view, const itkg_view& 
     A.accumulate = accumulate;
     const FusedEntry* fe = fused_for(p);
     const int TY = fe ? fe->ty[mode] : 32;
-    A.tiles_x = (A.n_dst[0] + kFusedTileX - 1) / kFusedTileX;
+    const int TX = fused_tile_x(mode);
+    A.tiles_x = (A.n_dst[0] + TX - 1) / TX;
     A.tiles_y = (A.n_dst[1] + TY - 1) / TY;
     A.tz = tz > 0 ? tz : fused_tz(A.tiles_x * A.tiles_y * ncomp, A.n_dst[2], rz);
     A.tiles_z = (A.n_dst[2] + A.tz - 1) / A.tz;
@@ -519,29 +520,44 @@ template void fill_fused_args<Fused3dArgsB>(const Plan&, const itkg_view&, const
 
 // Tile shape of a FAST or adjoint launch: 64-row tiles (fewer halo rows per
 // output) or 32-row tiles (twice the tiles, so longer z runs fill the GPU
-// on smaller volumes).  Modelled cost: tiles per CU x plane steps per tile x
-// work per step, which grows with the tile's rows plus their x halo plus a
-// fixed per-step part (loads, barrier); the z run of each shape as
-// fused_tz picks it.  ITKG_FUSED_SHAPE=64|32 overrides.
+// on smaller volumes), and for FAST at radii 1..5 also 128 x 32 tiles.
+// Modelled cost: tiles per CU x plane steps per tile x work per step, which
+// grows with the tile's rows plus their x halo plus a fixed per-step part
+// (loads, barrier), per 64 columns; the z run of each shape as fused_tz
+// picks it.  A 128-wide tile's step is weighted by kWideWalk: each source
+// row it loads is one longer contiguous segment, and its memory walk is
+// cheaper (measured at 512^3: 4.5%, 7%, 2%, 6% and 1% faster than 64 x 64
+// tiles at radii 1..5, where the kernel is bound by the walk; 1.7% slower
+// at radius 6, where the arithmetic of its 44-row x pass dominates, so it
+// is not built there).  ITKG_FUSED_SHAPE=64|32|128 overrides.
+constexpr double kWideWalk = 0.84;
 int pick_shape(const Plan& p, const FusedEntry* fe, int mode, int ncomp) {
     if (mode != FUSED_FWD_FAST && mode != FUSED_ADJ) return mode;
     const int small = mode == FUSED_FWD_FAST ? FUSED_FWD_FAST_S : FUSED_ADJ_S;
-    if (const char* e = std::getenv("ITKG_FUSED_SHAPE")) return std::atoi(e) == 32 ? small : mode;
+    const bool wide_ok = mode == FUSED_FWD_FAST && fe->ty[FUSED_FWD_FAST_W] > 0;
+    if (const char* e = std::getenv("ITKG_FUSED_SHAPE")) {
+        const int v = std::atoi(e);
+        return v == 32 ? small : (v == 128 && wide_ok ? FUSED_FWD_FAST_W : mode);
+    }
     const itkg_region& d = fused_is_adjoint(mode) ? p.in_reg : p.req;
     const int64_t nx = d.size[0], ny = p.D >= 2 ? d.size[1] : 1, nz = p.D == 3 ? d.size[2] : 1;
     const int rxy = (int)p.radii[0], rz = p.D == 3 ? (int)p.radii[2] : 0;
     auto cost = [&](int m) {
         const int ty = fe->ty[m];
-        const int64_t txy = ((nx + kFusedTileX - 1) / kFusedTileX) * ((ny + ty - 1) / ty) * ncomp;
+        const int tx = fused_tile_x(m);
+        const int64_t txy = ((nx + tx - 1) / tx) * ((ny + ty - 1) / ty) * ncomp;
         const int tz = fused_tz((int)std::min<int64_t>(txy, INT32_MAX), (int)nz, rz);
         const int64_t tiles = txy * ((nz + tz - 1) / tz);
         const int64_t cus = device_cus();
         const double per_cu = (double)((tiles + cus - 1) / cus);
-        return per_cu * (double)(std::min<int64_t>(tz, nz) + 2 * rz + 3) * (3.0 * ty + 2.0 * rxy + 30.0);
+        const double step = (3.0 * ty + 2.0 * rxy + 30.0) * (tx / 64) * (tx > 64 ? kWideWalk : 1.0);
+        return per_cu * (double)(std::min<int64_t>(tz, nz) + 2 * rz + 3) * step;
     };
-    const int pick = cost(small) < cost(mode) ? small : mode;
+    int pick = cost(small) < cost(mode) ? small : mode;
+    if (wide_ok && cost(FUSED_FWD_FAST_W) < cost(pick)) pick = FUSED_FWD_FAST_W;
     if (std::getenv("ITKG_DEBUG"))
-        std::fprintf(stderr, "itkg: fused shape %d-row tiles (mode %d)\n", fe->ty[pick], pick);
+        std::fprintf(stderr, "itkg: fused shape %d x %d tiles (mode %d)\n", fused_tile_x(pick),
+                     fe->ty[pick], pick);
     return pick;
 }
 

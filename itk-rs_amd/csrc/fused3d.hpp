@@ -15,12 +15,13 @@
 // (terms outside g_out are zero).  The transposed passes commute in exact
 // arithmetic, so they run in the same x, y, z order as the forward.
 //
-// Workgroup: NW waves.  It owns a 64 x TY destination tile in x/y (one
-// component of it, for interleaved pixels) and a run of TZ destination
-// planes, and walks the source planes the run needs in z order, one step
-// per plane (planes clamped at the image edge are filtered again; adjoint
-// planes outside the source are zeros, so every step is the same).  The
-// (TY+2r) x (64+2r) source window of a plane is loaded as a flattened array
+// Workgroup: NW waves.  It owns a TX x TY destination tile in x/y (64 x TY,
+// or 128 x 32 for FUSED_FWD_FAST_W; one component of it, for interleaved
+// pixels) and a run of TZ destination planes, and walks the source planes
+// the run needs in z order, one step per plane (planes clamped at the image
+// edge are filtered again; adjoint planes outside the source are zeros, so
+// every step is the same).  The (TY+2r) x (TX+2r) source window of a plane
+// is loaded as a flattened array
 // with per-lane offsets computed once per tile, into one of NSET register
 // sets (2, or 4 where the registers allow), NSET steps before it is stored
 // to LDS.  Per step, with both LDS buffers (the window and the
@@ -94,50 +95,61 @@ struct Cfg {
     static constexpr bool EXACT_MID = MODE == FUSED_FWD_EXACT && RZ >= 9 && RZ <= 14;
     static constexpr int NW = MODE != FUSED_FWD_EXACT ? 16 : (EXACT_MID ? 8 : 16);
     static constexpr int NTH = 64 * NW;
-    static constexpr int TX = 64;
+    // Tile width: 64 (a wave's lanes are the tile's columns) or, for
+    // FUSED_FWD_FAST_W, 128: the waves alternate between the two 64-column
+    // halves (wave w: half w % HALVES, row group w / HALVES), so each source
+    // row of the window is one 128 + 2r segment.
+    static constexpr int TX = fused_tile_x(MODE);
+    static constexpr int HALVES = TX / 64;
     static constexpr bool SMALL = MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S;
     // radii from 18 up (FMA modes): one column per lane, so that the ring
     // (2r + 1 registers a column) fits
     static constexpr int CPT = MODE == FUSED_FWD_EXACT
                                    ? ((EXACT_MID && RZ <= 10) || RZ <= 5 ? 4 : 2)
                                    : (RZ >= 18 ? 1 : (RZ <= 7 && !SMALL ? 4 : 2));
-    static constexpr int TY = NW * CPT;
+    static constexpr int TY = NW / HALVES * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows
     static constexpr int E = WX * HY;             // window elements per plane
     static constexpr int LJ = (E + NTH - 1) / NTH;  // window loads per lane and plane
-    // x pass: each lane produces Q consecutive outputs of one row; an item
-    // (one wave's 64 lanes) covers Q rows.  Q in {4, 2, 1} is chosen to
-    // balance the ceil(HY / Q) items over the NW waves (every wave waits for
-    // the slowest at the barrier that follows); Q-wide LDS reads need the
-    // row stride WX to be a multiple of Q.
+    // x pass: each lane produces Q consecutive outputs of one row; a row
+    // takes TX / Q lanes, so an item (one wave's 64 lanes) covers
+    // RPI = 64 Q / TX rows.  Q in {4, 2, 1} is chosen to balance the
+    // ceil(HY / RPI) items over the NW waves (every wave waits for the
+    // slowest at the barrier that follows); Q-wide LDS reads need the row
+    // stride WX to be a multiple of Q, and a row at most one wave's lanes.
+    static constexpr int rpi(int q) { return 64 * q / TX; }
     static constexpr int xcost(int q) {
-        const int items = (HY + q - 1) / q;
+        if (WX % q || rpi(q) < 1) return 1 << 30;
+        const int items = (HY + rpi(q) - 1) / rpi(q);
         const int per_wave = (items + NW - 1) / NW;
-        return (WX % q) ? 1 << 30 : per_wave * (q * (2 * RXY + 1) + 2 * ((q + 2 * RXY + q - 1) / q));
+        return per_wave * (q * (2 * RXY + 1) + 2 * ((q + 2 * RXY + q - 1) / q));
     }
     static constexpr int Q = xcost(4) <= xcost(2) ? (xcost(4) <= xcost(1) ? 4 : 1)
                                                   : (xcost(2) <= xcost(1) ? 2 : 1);
-    static constexpr int RG = (HY + Q - 1) / Q;   // x-pass items
+    static constexpr int RPI = rpi(Q);                // rows per x item
+    static constexpr int RG = (HY + RPI - 1) / RPI;   // x-pass items
     static constexpr int XQ = (Q + 2 * RXY + Q - 1) / Q;  // Q-wide reads per x item
-    // Split x pass: full rounds of Q-row items (every wave one item per
-    // round), then the rows left over one row per item (one output per
-    // lane), dealt to the waves; used when its slowest wave has less work.
-    static constexpr int NA = (HY / Q) / NW * NW;  // items of the full rounds
-    static constexpr int RA = Q * NA;              // rows they cover
-    static constexpr int RL = HY - RA;             // rows left over
+    // Split x pass: full rounds of RPI-row items (every wave one item per
+    // round), then the rows left over one row per item (QL = TX / 64
+    // outputs per lane), dealt to the waves; used when its slowest wave has
+    // less work.
+    static constexpr int QL = HALVES;
+    static constexpr int NA = (HY / RPI) / NW * NW;  // items of the full rounds
+    static constexpr int RA = RPI * NA;              // rows they cover
+    static constexpr int RL = HY - RA;               // rows left over
     static constexpr int xcost_split() {
         return (NA / NW) * (Q * (2 * RXY + 1) + 2 * XQ) +
-               ((RL + NW - 1) / NW) * 3 * (2 * RXY + 1);
+               ((RL + NW - 1) / NW) * (QL * (2 * RXY + 1) + 2 * (QL + 2 * RXY));
     }
     static constexpr bool XSPLIT = Q > 1 && NA > 0 && xcost_split() < xcost(Q);
     // One sIn plane: the window (row stride WX), room for the unconditional
     // stores of the last loads (LJ * NTH) and for the x pass's reads of the
     // rows past HY in its last item (their outputs are never used).
-    static constexpr int PLANE0 = LJ * NTH > Q * RG * WX ? LJ * NTH : Q * RG * WX;
+    static constexpr int PLANE0 = LJ * NTH > RPI * RG * WX ? LJ * NTH : RPI * RG * WX;
     static constexpr int PLANE = (PLANE0 + 3) / 4 * 4;
     static constexpr int SX = TX;                 // sX row stride (floats)
-    static constexpr int XPLANE = Q * RG * SX;    // one sX buffer
+    static constexpr int XPLANE = RPI * RG * SX;  // one sX buffer
     static constexpr int P = 2 * RZ + 1;
     // Adjoint at small radii (64-row tiles): the weights stay in scalar
     // registers for the whole tile and the first x-pass round's LDS reads
@@ -152,10 +164,12 @@ struct Cfg {
     // elsewhere.
     static constexpr int NSET =
         MODE != FUSED_FWD_EXACT && RXY <= 10 && RZ <= 10 &&
-                (CPT == 2 || (MODE == FUSED_FWD_FAST && RXY <= 6 && RZ <= 6))
+                (CPT == 2 || ((MODE == FUSED_FWD_FAST || MODE == FUSED_FWD_FAST_W) && RXY <= 6 && RZ <= 6))
             ? 4
             : 2;
-    static_assert(WX - 64 <= 64, "halo wider than one wave");
+    static_assert(WX - TX <= 64, "halo wider than one wave");
+    static_assert(HALVES == 1 || (MODE == FUSED_FWD_FAST_W && NW % HALVES == 0),
+                  "128-wide tiles: forward FAST only");
     static_assert((2 * PLANE + 2 * XPLANE) * 4 <= 160 * 1024, "LDS budget of one workgroup");
 };
 
@@ -274,6 +288,10 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     const int x0 = tx_i * C::TX, y0 = ty_i * C::TY, z0 = tz_i * a.tz;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // this wave's 64-column half of the tile and its row group (y pass, z
+    // ring and stores)
+    const int half = C::HALVES > 1 ? wave % C::HALVES : 0;
+    const int wrow = C::HALVES > 1 ? wave / C::HALVES : wave;
 
     const int nsx = a.n_src[0], nsy = a.n_src[1], nsz = a.n_src[2];
     const int ndx = a.n_dst[0], ndy = a.n_dst[1], ndz = a.n_dst[2];
@@ -351,7 +369,8 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     const bool x_edge = x0 == 0 || x0 + C::TX >= ndx;
 
     // ---- x pass: items g = wave, wave+NW, ...; lane -> (row in item, Q-group)
-    constexpr int Q = C::Q, LPR = 64 / Q;
+    constexpr int Q = C::Q, LPR = C::TX / Q, RPI = C::RPI, QL = C::QL;
+    static_assert(QL == 1 || (!STREAM && !ADJOINT), "two outputs per lane in the left-over rows");
     using vecq = std::conditional_t<Q == 4, float4, std::conditional_t<Q == 2, float2, float>>;
     const int xq = lane % LPR, xr = lane / LPR;
     auto x_pass = [&](const float* srcp, float* dstp) {
@@ -360,7 +379,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
         for (int g0 = 0; g0 < NG; g0 += C::NW) {
             const int g = g0 + wave;
             if (g < NG) {
-                const int row = Q * g + xr;
+                const int row = RPI * g + xr;
                 const vecq* srcq = reinterpret_cast<const vecq*>(&srcp[row * C::WX + Q * xq]);
                 float v[STREAM ? 1 : Q * C::XQ];
                 float o[Q];
@@ -438,9 +457,21 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
                 const int k = k0 + wave;
                 if (k < C::RL) {
                     const int row = C::RA + k;
-                    const float* rp = srcp + row * C::WX + lane;
+                    const float* rp = srcp + row * C::WX + QL * lane;
                     float o;
-                    if constexpr (STREAM) {
+                    if constexpr (QL == 2) {
+                        // 128-wide tiles: two outputs per lane, float2 reads
+                        float v[2 * RXY + 2];
+#pragma unroll
+                        for (int i = 0; i <= RXY; ++i) {
+                            const float2 t = reinterpret_cast<const float2*>(rp)[i];
+                            v[2 * i] = t.x;
+                            v[2 * i + 1] = t.y;
+                        }
+                        *reinterpret_cast<float2*>(&dstp[row * C::SX + 2 * lane]) =
+                            make_float2(tap_sum<RXY, MODE>(v, wx), tap_sum<RXY, MODE>(v + 1, wx));
+                        continue;
+                    } else if constexpr (STREAM) {
                         o = rp[0] * wx[0];
 #pragma unroll
                         for (int i = 1; i <= 2 * RXY; ++i) o = __builtin_fmaf(rp[i], wx[i], o);
@@ -476,7 +507,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     const bool x_first = NGE >= C::NW || wave < NGE;  // wave-uniform; constant if every wave has one
     [[maybe_unused]] float xv[Q * C::XQ];
     auto x_read_item = [&](const float* srcp, const int g, float* v) {
-        const vecq* srcq = reinterpret_cast<const vecq*>(&srcp[(Q * g + xr) * C::WX + Q * xq]);
+        const vecq* srcq = reinterpret_cast<const vecq*>(&srcp[(RPI * g + xr) * C::WX + Q * xq]);
 #pragma unroll
         for (int i = 0; i < C::XQ; ++i) {
             const vecq t = srcq[i];
@@ -516,7 +547,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
                 }
             }
         }
-        vecq* dq = reinterpret_cast<vecq*>(&dstp[(Q * g + xr) * C::SX + Q * xq]);
+        vecq* dq = reinterpret_cast<vecq*>(&dstp[(RPI * g + xr) * C::SX + Q * xq]);
         if constexpr (Q == 4) *dq = make_float4(o[0], o[1], o[2], o[3]);
         else if constexpr (Q == 2) *dq = make_float2(o[0], o[1]);
         else *dq = o[0];
@@ -565,8 +596,9 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
         }
     };
 
-    // ---- y pass: CPT outputs down column `lane`, rows wave*CPT ...
-    const int ybase = y0 + wave * CPT;
+    // ---- y pass: CPT outputs down tile column xcol, rows wrow*CPT ...
+    const int ybase = y0 + wrow * CPT;
+    const int xcol = half * 64 + lane;  // this lane's tile column
     // wave-uniform: this wave's rows hold the first or the last output row
     [[maybe_unused]] const bool y_edge = ybase <= 0 || ybase + CPT > ndy - 1;
     float val[CPT];
@@ -574,7 +606,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
         if constexpr (STREAM) {
 #pragma unroll
             for (int i = 0; i < CPT + 2 * RXY; ++i) {
-                const float t = xp[(wave * CPT + i) * C::SX + lane];
+                const float t = xp[(wrow * CPT + i) * C::SX + xcol];
 #pragma unroll
                 for (int c = 0; c < CPT; ++c) {
                     const int k = i - c;  // tap of input row i in output c
@@ -586,7 +618,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
         }
         float v[CPT + 2 * RXY];
 #pragma unroll
-        for (int i = 0; i < CPT + 2 * RXY; ++i) v[i] = xp[(wave * CPT + i) * C::SX + lane];
+        for (int i = 0; i < CPT + 2 * RXY; ++i) v[i] = xp[(wrow * CPT + i) * C::SX + xcol];
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
             if constexpr (EARLY) val[c] = tap_sum<RXY, MODE>(v + c, wyr);
@@ -617,11 +649,12 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     // ---- output column of this lane.  Dense rows (x stride 1) are written
     // non-temporally at a wave-uniform base (scalar registers) plus the
     // lane's byte offset.
-    const int ox = x0 + lane;
+    const int ox = x0 + xcol;
     const bool x_ok = ox < ndx;
     const int64_t dxs = a.dst_stride[0], dys = a.dst_stride[1], dzs = a.dst_stride[2];
     const bool store_nt = PNT && dxs == 1 && !(ADJOINT && a.accumulate);  // tile-uniform
-    const global_ptr<float> dwave = to_global(a.dst) + x0 + (int64_t)ybase * dys;  // used when dxs == 1
+    const global_ptr<float> dwave =
+        to_global(a.dst) + x0 + half * 64 + (int64_t)ybase * dys;  // used when dxs == 1
     uint32_t dlane = (uint32_t)lane * (uint32_t)sizeof(float);
 
     // ---- z ring: fixed slots, rotating weights (wz2 = wz repeated twice)
@@ -1009,6 +1042,16 @@ void mega_table(std::integer_sequence<int, I...>, int* lds, int* ty) {
      ...);
 }
 
+// The 128-wide FAST tiles (plain launches) are built for 3-D isotropic
+// radii 1..5, where they measured faster than 64 x 64 tiles.
+template <int RXY, int RZ>
+constexpr bool wide_built() { return RXY >= 1 && RXY <= 5 && RZ == RXY; }
+// Tile rows of the 128-wide shape, 0 where it is not built.
+template <int RXY, int RZ>
+constexpr int wide_ty() {
+    if constexpr (wide_built<RXY, RZ>()) return Cfg<RXY, RZ, FUSED_FWD_FAST_W>::TY;
+    else return 0;
+}
 // Workgroup size of mode `mode` for (RXY, RZ).
 template <int RXY, int RZ>
 int block_of(int mode) {
@@ -1017,6 +1060,7 @@ int block_of(int mode) {
         case FUSED_FWD_FAST: return Cfg<RXY, RZ, FUSED_FWD_FAST>::NTH;
         case FUSED_ADJ: return Cfg<RXY, RZ, FUSED_ADJ>::NTH;
         case FUSED_FWD_FAST_S: return Cfg<RXY, RZ, FUSED_FWD_FAST_S>::NTH;
+        case FUSED_FWD_FAST_W: return 1024;
         default: return Cfg<RXY, RZ, FUSED_ADJ_S>::NTH;
     }
 }
@@ -1041,6 +1085,12 @@ hipError_t launch_r(const FusedMulti& a, int mode, hipStream_t st) {
         case FUSED_ADJ_S:
             hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_ADJ_S>), grid, block, 0, st, a);
             break;
+        case FUSED_FWD_FAST_W:
+            if constexpr (wide_built<RXY, RZ>()) {
+                hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_FAST_W>), grid, block, 0, st, a);
+                break;
+            }
+            return hipErrorInvalidValue;
         default:
             return hipErrorInvalidValue;
     }
@@ -1105,7 +1155,7 @@ constexpr FusedEntry make_entry() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY},
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>()},
                       &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>, &occupancy_r<RXY, RZ>};
 }
 template <int RXY, int RZ>
@@ -1113,7 +1163,7 @@ constexpr FusedEntry make_entry_single() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY},
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>()},
                       &launch_r<RXY, RZ>, nullptr, nullptr};
 }
 

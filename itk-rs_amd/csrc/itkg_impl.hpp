@@ -124,15 +124,21 @@ size_t ptype_size(itkg_ptype t);
 // (forward: clamped into the source; adjoint: zero outside it).
 // The _S modes are FAST and adjoint on 32-row tiles (the others use 64-row
 // tiles), for volumes small enough that shorter tiles buy longer z runs.
+// FUSED_FWD_FAST_W is FAST on 128 x 32 tiles (plain launches, radii 1..5):
+// longer contiguous source rows, the same FMA chain per output, so the same
+// bits as the other FAST shapes.
 enum FusedMode {
     FUSED_FWD_EXACT = 0,
     FUSED_FWD_FAST = 1,
     FUSED_ADJ = 2,
     FUSED_FWD_FAST_S = 3,
     FUSED_ADJ_S = 4,
-    FUSED_NMODES = 5
+    FUSED_FWD_FAST_W = 5,
+    FUSED_NMODES = 6
 };
 constexpr bool fused_is_adjoint(int mode) { return mode == FUSED_ADJ || mode == FUSED_ADJ_S; }
+// Destination tile width (x) of a mode.
+constexpr int fused_tile_x(int mode) { return mode == FUSED_FWD_FAST_W ? 128 : 64; }
 constexpr int kFusedMaxR = 16;
 
 template <int MAXR>
@@ -171,13 +177,13 @@ struct FusedMulti {
 
 struct FusedEntry {
     int rxy, rz;
-    int ty[FUSED_NMODES];  // tile rows per mode (FusedMode index)
+    int ty[FUSED_NMODES];  // tile rows per mode (FusedMode index; 0: mode not built)
     hipError_t (*launch)(const FusedMulti& m, int mode, hipStream_t s);
     hipError_t (*batch)(const Fused3dArgs* vols, const int2* work, int nwork, int grid_blocks,
                         int mode, hipStream_t s);
     int (*occupancy)(int mode);  // resident workgroups per CU of the persistent form
 };
-constexpr int kFusedTileX = 64;
+constexpr int kFusedTileX = 64;  // tile width of every mode but FUSED_FWD_FAST_W
 // The instantiation for (rxy, rz), or null.
 const FusedEntry* fused_entry(int rxy, int rz);
 

@@ -249,24 +249,56 @@ def test_per_axis_strided_output_view(itk, orc, dev):
     assert torch.isnan(rest).sum().item() == rest.numel() - win.numel()  # nothing else written
 
 
-@pytest.mark.parametrize("sigma", [0.9, 2.0, 3.4])
+@pytest.mark.parametrize("sigma", [0.3, 0.4, 0.9, 1.2, 1.5, 2.0, 3.4])
 def test_fused_tile_shapes_agree(itk, orc, dev, monkeypatch, sigma):
-    """FAST forward and the adjoint on 64-row and on 32-row tiles: every voxel
-    follows the same FMA chain, so the two shapes agree bit for bit; both
-    within tolerance of the oracle."""
+    """FAST forward on 64 x 64, 64 x 32 and 128 x 32 tiles (the last built
+    for radii 1..5, here 1..5; at radius 6 and 11 the request falls back to
+    64 x 64) and the adjoint on the first two: every voxel follows the same
+    FMA chain, so the shapes agree bit for bit; all within tolerance of the
+    oracle.  x = 130: a 128-wide tile and a 2-column one; y = 90: a partial
+    last row of tiles."""
     rng = np.random.default_rng(31)
     arr = rng.random((37, 90, 130)).astype(np.float32)
     x = torch.from_numpy(arr).to(dev)
     outs, grads = {}, {}
-    for shape in ("64", "32"):
+    for shape in ("64", "32", "128"):
         monkeypatch.setenv("ITKG_FUSED_SHAPE", shape)
         outs[shape] = itk.gaussian(x, sigma, (1.0, 1.0, 1.0), arith=itk.FAST)
         grads[shape] = itk.gaussian_backward(x, sigma, (1.0, 1.0, 1.0))
     torch.cuda.synchronize()
     assert torch.equal(outs["64"], outs["32"])
+    assert torch.equal(outs["64"], outs["128"])
     assert torch.equal(grads["64"], grads["32"])
     ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, [0, 0, 0], [130, 90, 37])
     assert _rel_err(outs["64"].cpu().numpy(), ref) <= FAST_TOL
+
+
+@pytest.mark.parametrize("r", [1, 2, 3, 4, 5])
+def test_wide_tiles_subregion(itk, orc, dev, monkeypatch, r):
+    """128 x 32 FAST tiles on a sub-region request of a volume whose
+    extents are not multiples of the tile (x = 197: a last tile 69 columns
+    wide, both 64-column halves in use; y = 75): the same values as the
+    per-axis FAST kernels, and within tolerance of the oracle."""
+    sigma = r / 3.0 - 1e-3
+    rng = np.random.default_rng(100 + r)
+    arr = rng.random((33, 75, 197)).astype(np.float32)
+    img = _img(itk, arr, [0, 0, 0], [1.0, 1.0, 1.0], dev)
+    f = itk.GaussianFilter(img, sigma)
+    f.arith = itk.FAST
+    req = itk.Region([3, 4, 2], [190, 69, 29])
+    monkeypatch.setenv("ITKG_FUSED_SHAPE", "128")
+    itk.set_path(itk.PATH_FUSED)
+    try:
+        wide = f.generate_region(req).data.cpu().numpy()
+    finally:
+        itk.set_path(itk.PATH_PER_AXIS)
+    try:
+        per = f.generate_region(req).data.cpu().numpy()
+    finally:
+        itk.set_path(itk.PATH_AUTO)
+    assert np.array_equal(wide, per)  # same FMA chain; equal up to the sign of zero
+    ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, [3, 4, 2], [190, 69, 29])
+    assert _rel_err(wide, ref) <= FAST_TOL
 
 
 @pytest.mark.parametrize("sigma", [1.2, 2.0])
@@ -278,11 +310,12 @@ def test_fused_tile_shapes_agree_interleaved(itk, dev, monkeypatch, sigma):
     arr = rng.random((21, 40, 200, 3)).astype(np.float32)
     x = torch.from_numpy(arr).to(dev)
     outs = {}
-    for shape in ("64", "32"):
+    for shape in ("64", "32", "128"):
         monkeypatch.setenv("ITKG_FUSED_SHAPE", shape)
         outs[shape] = itk.gaussian(x, sigma, (1.0, 1.0, 1.0), ndim=3, arith=itk.FAST)
     torch.cuda.synchronize()
     assert torch.equal(outs["64"], outs["32"])
+    assert torch.equal(outs["64"], outs["128"])
     monkeypatch.delenv("ITKG_FUSED_SHAPE")
     per = {}
     itk.set_path(itk.PATH_PER_AXIS)

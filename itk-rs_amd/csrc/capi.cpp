@@ -518,26 +518,32 @@ template void fill_fused_args<Fused3dArgs>(const Plan&, const itkg_view&, const 
 template void fill_fused_args<Fused3dArgsB>(const Plan&, const itkg_view&, const itkg_view&, int,
                                             int, int, int, int, Fused3dArgsB*);
 
-// Tile shape of a FAST or adjoint launch: 64-row tiles (fewer halo rows per
-// output) or 32-row tiles (twice the tiles, so longer z runs fill the GPU
-// on smaller volumes), and for FAST at radii 1..5 also 128 x 32 tiles.
+// Tile shape of a FAST, EXACT or adjoint launch: 64-row tiles (fewer halo
+// rows per output) or, FAST and adjoint, 32-row tiles (twice the tiles, so
+// longer z runs fill the GPU on smaller volumes), and 128 x 32 tiles for
+// FAST at radii 1..5 and EXACT at radii 1..4.
 // Modelled cost: tiles per CU x plane steps per tile x work per step, which
 // grows with the tile's rows plus their x halo plus a fixed per-step part
 // (loads, barrier), per 64 columns; the z run of each shape as fused_tz
 // picks it.  A 128-wide tile's step is weighted by kWideWalk: each source
 // row it loads is one longer contiguous segment, and its memory walk is
-// cheaper (measured at 512^3: 4.5%, 7%, 2%, 6% and 1% faster than 64 x 64
-// tiles at radii 1..5, where the kernel is bound by the walk; 1.7% slower
-// at radius 6, where the arithmetic of its 44-row x pass dominates, so it
-// is not built there).  ITKG_FUSED_SHAPE=64|32|128 overrides.
+// cheaper (measured at 512^3, FAST: 4.5%, 7%, 2%, 6% and 1% faster than
+// 64 x 64 tiles at radii 1..5, where the kernel is bound by the walk; 1.7%
+// slower at radius 6, where the arithmetic of its 44-row x pass dominates,
+// so it is not built there; EXACT: 6%, 2.5%, 1% faster and even at radii
+// 1..4, 2% slower at 5).  ITKG_FUSED_SHAPE=64|32|128 overrides.
 constexpr double kWideWalk = 0.84;
 int pick_shape(const Plan& p, const FusedEntry* fe, int mode, int ncomp) {
-    if (mode != FUSED_FWD_FAST && mode != FUSED_ADJ) return mode;
-    const int small = mode == FUSED_FWD_FAST ? FUSED_FWD_FAST_S : FUSED_ADJ_S;
-    const bool wide_ok = mode == FUSED_FWD_FAST && fe->ty[FUSED_FWD_FAST_W] > 0;
+    if (mode != FUSED_FWD_FAST && mode != FUSED_ADJ && mode != FUSED_FWD_EXACT) return mode;
+    // the other shapes of this arithmetic (-1: none)
+    const int small = mode == FUSED_FWD_FAST ? FUSED_FWD_FAST_S : (mode == FUSED_ADJ ? FUSED_ADJ_S : -1);
+    const int wide = mode == FUSED_FWD_FAST ? FUSED_FWD_FAST_W : (mode == FUSED_FWD_EXACT ? FUSED_FWD_EXACT_W : -1);
+    const bool wide_ok = wide >= 0 && fe->ty[wide] > 0;
     if (const char* e = std::getenv("ITKG_FUSED_SHAPE")) {
         const int v = std::atoi(e);
-        return v == 32 ? small : (v == 128 && wide_ok ? FUSED_FWD_FAST_W : mode);
+        if (v == 32 && small >= 0) return small;
+        if (v == 128 && wide_ok) return wide;
+        return mode;
     }
     const itkg_region& d = fused_is_adjoint(mode) ? p.in_reg : p.req;
     const int64_t nx = d.size[0], ny = p.D >= 2 ? d.size[1] : 1, nz = p.D == 3 ? d.size[2] : 1;
@@ -553,8 +559,8 @@ int pick_shape(const Plan& p, const FusedEntry* fe, int mode, int ncomp) {
         const double step = (3.0 * ty + 2.0 * rxy + 30.0) * (tx / 64) * (tx > 64 ? kWideWalk : 1.0);
         return per_cu * (double)(std::min<int64_t>(tz, nz) + 2 * rz + 3) * step;
     };
-    int pick = cost(small) < cost(mode) ? small : mode;
-    if (wide_ok && cost(FUSED_FWD_FAST_W) < cost(pick)) pick = FUSED_FWD_FAST_W;
+    int pick = small >= 0 && cost(small) < cost(mode) ? small : mode;
+    if (wide_ok && cost(wide) < cost(pick)) pick = wide;
     if (std::getenv("ITKG_DEBUG"))
         std::fprintf(stderr, "itkg: fused shape %d x %d tiles (mode %d)\n", fused_tile_x(pick),
                      fe->ty[pick], pick);

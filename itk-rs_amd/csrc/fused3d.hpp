@@ -16,7 +16,7 @@
 // arithmetic, so they run in the same x, y, z order as the forward.
 //
 // Workgroup: NW waves.  It owns a TX x TY destination tile in x/y (64 x TY,
-// or 128 x 32 for FUSED_FWD_FAST_W; one component of it, for interleaved
+// or 128 x 32 for the _W modes; one component of it, for interleaved
 // pixels) and a run of TZ destination planes, and walks the source planes
 // the run needs in z order, one step per plane (planes clamped at the image
 // edge are filtered again; adjoint planes outside the source are zeros, so
@@ -92,11 +92,12 @@ __device__ __forceinline__ void lds_barrier_tracked() {
 //   VecPixel radii (13, 13, 4), bit-identical.
 template <int RXY, int RZ, int MODE>
 struct Cfg {
-    static constexpr bool EXACT_MID = MODE == FUSED_FWD_EXACT && RZ >= 9 && RZ <= 14;
-    static constexpr int NW = MODE != FUSED_FWD_EXACT ? 16 : (EXACT_MID ? 8 : 16);
+    static constexpr bool EXACT = fused_is_exact(MODE);
+    static constexpr bool EXACT_MID = EXACT && RZ >= 9 && RZ <= 14;
+    static constexpr int NW = !EXACT ? 16 : (EXACT_MID ? 8 : 16);
     static constexpr int NTH = 64 * NW;
-    // Tile width: 64 (a wave's lanes are the tile's columns) or, for
-    // FUSED_FWD_FAST_W, 128: the waves alternate between the two 64-column
+    // Tile width: 64 (a wave's lanes are the tile's columns) or, for the
+    // _W modes, 128: the waves alternate between the two 64-column
     // halves (wave w: half w % HALVES, row group w / HALVES), so each source
     // row of the window is one 128 + 2r segment.
     static constexpr int TX = fused_tile_x(MODE);
@@ -104,7 +105,7 @@ struct Cfg {
     static constexpr bool SMALL = MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S;
     // radii from 18 up (FMA modes): one column per lane, so that the ring
     // (2r + 1 registers a column) fits
-    static constexpr int CPT = MODE == FUSED_FWD_EXACT
+    static constexpr int CPT = EXACT
                                    ? ((EXACT_MID && RZ <= 10) || RZ <= 5 ? 4 : 2)
                                    : (RZ >= 18 ? 1 : (RZ <= 7 && !SMALL ? 4 : 2));
     static constexpr int TY = NW / HALVES * CPT;
@@ -163,13 +164,14 @@ struct Cfg {
     // columns per lane), forward and adjoint, up to radius 10 (2-3%); 2
     // elsewhere.
     static constexpr int NSET =
-        MODE != FUSED_FWD_EXACT && RXY <= 10 && RZ <= 10 &&
+        !EXACT && RXY <= 10 && RZ <= 10 &&
                 (CPT == 2 || ((MODE == FUSED_FWD_FAST || MODE == FUSED_FWD_FAST_W) && RXY <= 6 && RZ <= 6))
             ? 4
             : 2;
     static_assert(WX - TX <= 64, "halo wider than one wave");
-    static_assert(HALVES == 1 || (MODE == FUSED_FWD_FAST_W && NW % HALVES == 0),
-                  "128-wide tiles: forward FAST only");
+    static_assert(HALVES == 1 ||
+                      ((MODE == FUSED_FWD_FAST_W || MODE == FUSED_FWD_EXACT_W) && NW % HALVES == 0),
+                  "128-wide tiles: forward FAST and EXACT only");
     static_assert((2 * PLANE + 2 * XPLANE) * 4 <= 160 * 1024, "LDS budget of one workgroup");
 };
 
@@ -224,7 +226,7 @@ __device__ __forceinline__ float* ring_of(A& a0, A& a1, A& a2, A& a3, A& a4, A& 
 // One output of a 1-D pass over window values v[0 .. 2r].
 template <int R, int MODE>
 __device__ __forceinline__ float tap_sum(const float* v, const float* w) {
-    if constexpr (MODE == FUSED_FWD_EXACT) {
+    if constexpr (fused_is_exact(MODE)) {
         float acc = 0.0f;
 #pragma unroll
         for (int k = 0; k <= 2 * R; ++k) {
@@ -269,7 +271,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     // opaque load offsets in the FMA modes up to radius 6 and with 32-row
     // tiles up to radius 10; two load sets and the plain step order
     // elsewhere in 64-row tiles.
-    constexpr bool FMA = MODE != FUSED_FWD_EXACT;
+    constexpr bool FMA = !fused_is_exact(MODE);
     constexpr bool PFIT = FMA && ((RXY <= 6 && RZ <= 6) || (CPT == 2 && RXY <= 10 && RZ <= 10));
     constexpr int NSET = PERSIST && CPT == 4 ? 2 : C::NSET;
     constexpr bool EARLY = C::EARLY && !PERSIST;
@@ -281,7 +283,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     // of holding all 2r + Q (x) or 2r + CPT (y) inputs in registers first.
     // Every output still receives its taps k = 0..2r in order (the same FMA
     // chain as tap_sum).
-    constexpr bool STREAM = !ADJOINT && MODE != FUSED_FWD_EXACT && (RXY >= 16 || (PERSIST && RXY >= 12));
+    constexpr bool STREAM = !ADJOINT && FMA && (RXY >= 16 || (PERSIST && RXY >= 12));
     const int tx_i = tile % a.tiles_x;
     const int ty_i = (tile / a.tiles_x) % a.tiles_y;
     const int tz_i = tile / (a.tiles_x * a.tiles_y);
@@ -738,7 +740,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 float& aj = j < PL ? acc[j] : hac[j - PL];
-                if constexpr (MODE == FUSED_FWD_EXACT) {
+                if constexpr (fused_is_exact(MODE)) {
                     const float prod = val[c] * w[-j];
                     aj = aj + prod;
                 } else if constexpr (EARLY) {
@@ -1042,14 +1044,22 @@ void mega_table(std::integer_sequence<int, I...>, int* lds, int* ty) {
      ...);
 }
 
-// The 128-wide FAST tiles (plain launches) are built for 3-D isotropic
-// radii 1..5, where they measured faster than 64 x 64 tiles.
+// The 128-wide tiles (plain launches) are built for 3-D isotropic radii
+// where they measured at least as fast as 64 x 64 tiles: 1..5 in FAST, 1..4
+// in EXACT.
 template <int RXY, int RZ>
 constexpr bool wide_built() { return RXY >= 1 && RXY <= 5 && RZ == RXY; }
+template <int RXY, int RZ>
+constexpr bool wide_exact_built() { return RXY >= 1 && RXY <= 4 && RZ == RXY; }
 // Tile rows of the 128-wide shape, 0 where it is not built.
 template <int RXY, int RZ>
 constexpr int wide_ty() {
     if constexpr (wide_built<RXY, RZ>()) return Cfg<RXY, RZ, FUSED_FWD_FAST_W>::TY;
+    else return 0;
+}
+template <int RXY, int RZ>
+constexpr int wide_exact_ty() {
+    if constexpr (wide_exact_built<RXY, RZ>()) return Cfg<RXY, RZ, FUSED_FWD_EXACT_W>::TY;
     else return 0;
 }
 // Workgroup size of mode `mode` for (RXY, RZ).
@@ -1061,6 +1071,7 @@ int block_of(int mode) {
         case FUSED_ADJ: return Cfg<RXY, RZ, FUSED_ADJ>::NTH;
         case FUSED_FWD_FAST_S: return Cfg<RXY, RZ, FUSED_FWD_FAST_S>::NTH;
         case FUSED_FWD_FAST_W: return 1024;
+        case FUSED_FWD_EXACT_W: return 1024;
         default: return Cfg<RXY, RZ, FUSED_ADJ_S>::NTH;
     }
 }
@@ -1088,6 +1099,12 @@ hipError_t launch_r(const FusedMulti& a, int mode, hipStream_t st) {
         case FUSED_FWD_FAST_W:
             if constexpr (wide_built<RXY, RZ>()) {
                 hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_FAST_W>), grid, block, 0, st, a);
+                break;
+            }
+            return hipErrorInvalidValue;
+        case FUSED_FWD_EXACT_W:
+            if constexpr (wide_exact_built<RXY, RZ>()) {
+                hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT_W>), grid, block, 0, st, a);
                 break;
             }
             return hipErrorInvalidValue;
@@ -1155,7 +1172,7 @@ constexpr FusedEntry make_entry() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>()},
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>(), wide_exact_ty<RXY, RZ>()},
                       &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>, &occupancy_r<RXY, RZ>};
 }
 template <int RXY, int RZ>
@@ -1163,7 +1180,7 @@ constexpr FusedEntry make_entry_single() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>()},
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>(), wide_exact_ty<RXY, RZ>()},
                       &launch_r<RXY, RZ>, nullptr, nullptr};
 }
 

@@ -124,9 +124,9 @@ size_t ptype_size(itkg_ptype t);
 // (forward: clamped into the source; adjoint: zero outside it).
 // The _S modes are FAST and adjoint on 32-row tiles (the others use 64-row
 // tiles), for volumes small enough that shorter tiles buy longer z runs.
-// FUSED_FWD_FAST_W is FAST on 128 x 32 tiles (plain launches, radii 1..5):
-// longer contiguous source rows, the same FMA chain per output, so the same
-// bits as the other FAST shapes.
+// FUSED_FWD_FAST_W and FUSED_FWD_EXACT_W are FAST and EXACT on 128 x 32
+// tiles (plain launches, small radii): longer contiguous source rows, the
+// same arithmetic per output, so the same bits as the other shapes.
 enum FusedMode {
     FUSED_FWD_EXACT = 0,
     FUSED_FWD_FAST = 1,
@@ -134,11 +134,16 @@ enum FusedMode {
     FUSED_FWD_FAST_S = 3,
     FUSED_ADJ_S = 4,
     FUSED_FWD_FAST_W = 5,
-    FUSED_NMODES = 6
+    FUSED_FWD_EXACT_W = 6,
+    FUSED_NMODES = 7
 };
 constexpr bool fused_is_adjoint(int mode) { return mode == FUSED_ADJ || mode == FUSED_ADJ_S; }
+// EXACT arithmetic: every product and every sum rounded separately.
+constexpr bool fused_is_exact(int mode) { return mode == FUSED_FWD_EXACT || mode == FUSED_FWD_EXACT_W; }
 // Destination tile width (x) of a mode.
-constexpr int fused_tile_x(int mode) { return mode == FUSED_FWD_FAST_W ? 128 : 64; }
+constexpr int fused_tile_x(int mode) {
+    return mode == FUSED_FWD_FAST_W || mode == FUSED_FWD_EXACT_W ? 128 : 64;
+}
 constexpr int kFusedMaxR = 16;
 
 template <int MAXR>
@@ -183,7 +188,7 @@ struct FusedEntry {
                         int mode, hipStream_t s);
     int (*occupancy)(int mode);  // resident workgroups per CU of the persistent form
 };
-constexpr int kFusedTileX = 64;  // tile width of every mode but FUSED_FWD_FAST_W
+constexpr int kFusedTileX = 64;  // tile width of every mode but the _W ones
 // The instantiation for (rxy, rz), or null.
 const FusedEntry* fused_entry(int rxy, int rz);
 

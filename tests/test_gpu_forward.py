@@ -252,39 +252,48 @@ def test_per_axis_strided_output_view(itk, orc, dev):
 @pytest.mark.parametrize("sigma", [0.3, 0.4, 0.9, 1.2, 1.5, 2.0, 3.4])
 def test_fused_tile_shapes_agree(itk, orc, dev, monkeypatch, sigma):
     """FAST forward on 64 x 64, 64 x 32 and 128 x 32 tiles (the last built
-    for radii 1..5, here 1..5; at radius 6 and 11 the request falls back to
-    64 x 64) and the adjoint on the first two: every voxel follows the same
-    FMA chain, so the shapes agree bit for bit; all within tolerance of the
-    oracle.  x = 130: a 128-wide tile and a 2-column one; y = 90: a partial
-    last row of tiles."""
+    for radii 1..5; at radius 6 and 11 the request falls back to 64 x 64)
+    and the adjoint on the first two: every voxel follows the same FMA chain,
+    so the shapes agree bit for bit; all within tolerance of the oracle.
+    EXACT on 64 x 64 and 128 x 32 tiles (built for radii 1..4): bit for bit
+    the oracle.  x = 130: a 128-wide tile and a 2-column one; y = 90: a
+    partial last row of tiles."""
     rng = np.random.default_rng(31)
     arr = rng.random((37, 90, 130)).astype(np.float32)
     x = torch.from_numpy(arr).to(dev)
-    outs, grads = {}, {}
+    outs, grads, exact = {}, {}, {}
     for shape in ("64", "32", "128"):
         monkeypatch.setenv("ITKG_FUSED_SHAPE", shape)
         outs[shape] = itk.gaussian(x, sigma, (1.0, 1.0, 1.0), arith=itk.FAST)
         grads[shape] = itk.gaussian_backward(x, sigma, (1.0, 1.0, 1.0))
+        exact[shape] = itk.gaussian(x, sigma, (1.0, 1.0, 1.0), arith=itk.EXACT)
     torch.cuda.synchronize()
     assert torch.equal(outs["64"], outs["32"])
     assert torch.equal(outs["64"], outs["128"])
     assert torch.equal(grads["64"], grads["32"])
+    # EXACT (64 x 64 tiles, or 128 x 32 at radii 1..4) bit for bit against the oracle
+    ref_exact = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, [0, 0, 0], [130, 90, 37])
+    for shape in ("64", "128"):
+        assert _bits_equal(exact[shape].cpu().numpy(), ref_exact), shape
     ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, [0, 0, 0], [130, 90, 37])
     assert _rel_err(outs["64"].cpu().numpy(), ref) <= FAST_TOL
 
 
 @pytest.mark.parametrize("r", [1, 2, 3, 4, 5])
-def test_wide_tiles_subregion(itk, orc, dev, monkeypatch, r):
-    """128 x 32 FAST tiles on a sub-region request of a volume whose
-    extents are not multiples of the tile (x = 197: a last tile 69 columns
-    wide, both 64-column halves in use; y = 75): the same values as the
-    per-axis FAST kernels, and within tolerance of the oracle."""
+@pytest.mark.parametrize("arith", ["fast", "exact"])
+def test_wide_tiles_subregion(itk, orc, dev, monkeypatch, r, arith):
+    """128 x 32 tiles (FAST radii 1..5, EXACT radii 1..4; EXACT radius 5
+    falls back to 64 x 64) on a sub-region request of a volume whose extents
+    are not multiples of the tile (x = 197: a last tile 69 columns wide,
+    both 64-column halves in use; y = 75): FAST gives the same values as the
+    per-axis FAST kernels and is within tolerance of the oracle, EXACT is
+    bit for bit the oracle."""
     sigma = r / 3.0 - 1e-3
     rng = np.random.default_rng(100 + r)
     arr = rng.random((33, 75, 197)).astype(np.float32)
     img = _img(itk, arr, [0, 0, 0], [1.0, 1.0, 1.0], dev)
     f = itk.GaussianFilter(img, sigma)
-    f.arith = itk.FAST
+    f.arith = itk.FAST if arith == "fast" else itk.EXACT
     req = itk.Region([3, 4, 2], [190, 69, 29])
     monkeypatch.setenv("ITKG_FUSED_SHAPE", "128")
     itk.set_path(itk.PATH_FUSED)
@@ -296,9 +305,13 @@ def test_wide_tiles_subregion(itk, orc, dev, monkeypatch, r):
         per = f.generate_region(req).data.cpu().numpy()
     finally:
         itk.set_path(itk.PATH_AUTO)
-    assert np.array_equal(wide, per)  # same FMA chain; equal up to the sign of zero
     ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, [3, 4, 2], [190, 69, 29])
-    assert _rel_err(wide, ref) <= FAST_TOL
+    if arith == "fast":
+        assert np.array_equal(wide, per)  # same FMA chain; equal up to the sign of zero
+        assert _rel_err(wide, ref) <= FAST_TOL
+    else:
+        assert _bits_equal(wide, ref)
+        assert _bits_equal(per, ref)
 
 
 @pytest.mark.parametrize("sigma", [1.2, 2.0])

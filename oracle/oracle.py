@@ -64,6 +64,10 @@ def lib():
         L.orc_ref_faithful_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, i64p, i64p,
                                            f64p, ctypes.c_double, i64p, i64p, ctypes.c_void_p,
                                            ctypes.c_int]
+        L.orc_separable_opt_f32.restype = ctypes.c_int
+        L.orc_separable_opt_f32.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, f64p, ctypes.c_double,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.orc_box_kernel.restype = None
         L.orc_box_kernel.argtypes = [ctypes.c_uint64, f64p]
@@ -360,6 +364,34 @@ def ref_faithful(img: np.ndarray, img_index, spacing, sigma, req_index, req_size
     rc = lib().orc_ref_faithful_f32(D, ncomp, img.ctypes.data, _i64(img_index), _i64(img_size),
                                     _f64(spacing), float(sigma), _i64(req_index), _i64(req_size),
                                     out.ctypes.data, int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle rejected arguments")
+    return out
+
+
+def separable_opt(img: np.ndarray, spacing, sigma, nthreads=None, out=None,
+                  work=None) -> np.ndarray:
+    """The whole-image 3-D f32 Gaussian with the reference's arithmetic in a
+    cache-friendly loop order (rows along x in every pass, planes over
+    `nthreads` host threads): the optimised-CPU comparison of the bench.
+    `out` (the image's shape) and `work` (two images' worth of f32) may be
+    passed in so that repeated calls reuse their pages."""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    if img.ndim != 3:
+        raise ValueError("3-D scalar images only")
+    if nthreads is None:
+        nthreads = os.cpu_count() or 1
+    nz, ny, nx = img.shape
+    if out is None:
+        out = np.empty_like(img)
+    if work is None:
+        work = np.empty(2 * img.size, dtype=np.float32)
+    if out.shape != img.shape or out.dtype != np.float32 or not out.flags.c_contiguous:
+        raise ValueError("out: a contiguous f32 array of the image's shape")
+    if work.size < 2 * img.size or work.dtype != np.float32 or not work.flags.c_contiguous:
+        raise ValueError("work: a contiguous f32 array of twice the image's size")
+    rc = lib().orc_separable_opt_f32(img.ctypes.data, nx, ny, nz, _f64(spacing), float(sigma),
+                                     out.ctypes.data, work.ctypes.data, int(nthreads))
     if rc != 0:
         raise ValueError("oracle rejected arguments")
     return out

@@ -38,6 +38,8 @@
  *   generate_region (f64)        : dense Kronecker operator (1e-12);
  *   adjoint (f64 scatter)        : dense transpose of the Kronecker operator
  *                                  and <Ax, y> = <x, A^T y>;
+ *   separable_opt (bench's optimised CPU comparison): bit-exact against
+ *                                  generate_region;
  *   ref_faithful (threaded)      : bit-exact against generate_region.
  *
  * The other convolve_axis filters, restated the same way:

@@ -105,9 +105,17 @@ struct Cfg {
     static constexpr bool SMALL = MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S;
     // radii from 18 up (FMA modes): one column per lane, so that the ring
     // (2r + 1 registers a column) fits
+    // Anisotropic radii (x/y above z): the adjoint from x/y radius 11 with z
+    // radius 6 or 7, and EXACT at x/y radius 11 or more with 8 waves, take
+    // 2 columns where 4 would spill.
+    static constexpr bool WIDE_XY = RXY >= 11 && RXY > RZ;
     static constexpr int CPT = EXACT
-                                   ? ((EXACT_MID && RZ <= 10) || RZ <= 5 ? 4 : 2)
-                                   : (RZ >= 18 ? 1 : (RZ <= 7 && !SMALL ? 4 : 2));
+                                   ? ((EXACT_MID && RZ <= 10 && !WIDE_XY) || RZ <= 5 ? 4 : 2)
+                                   : (RZ >= 18 ? 1
+                                               : (RZ <= 7 && !SMALL &&
+                                                          !(MODE == FUSED_ADJ && WIDE_XY && RZ >= 6)
+                                                      ? 4
+                                                      : 2));
     static constexpr int TY = NW / HALVES * CPT;
     static constexpr int WX = TX + 2 * RXY;       // loaded window width = sIn row stride
     static constexpr int HY = TY + 2 * RXY;       // loaded window rows

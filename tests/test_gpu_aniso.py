@@ -1,0 +1,97 @@
+"""Anisotropic spacing on the fused kernel: slices thicker than the in-plane
+spacing give a z radius below the x/y radius (kernel_radius per axis,
+src/filters/gaussian.rs:24-27), and every such pair (x/y radius 2..12, z
+radius 1 .. x/y - 1) runs on the fused one-round-trip kernel
+(itk.PATH_FUSED, which fails where the fused kernel is not built).  EXACT is
+bit for bit the oracle (generate_region, gaussian.rs:59-90); FAST follows
+the per-axis FAST kernels' FMA chain bit for bit and is within 1e-5 (max abs
+error over max abs value) of the oracle; the adjoint is within 1e-5 of the
+oracle's f64 transpose."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+PAIRS = [(r, z) for r in range(2, 13) for z in range(1, r)]
+
+# (z, y, x) volume; a sub-region request (index x, y, z; size x, y, z) whose
+# input region is clipped at the image's low x/y/z edges and inside its high ones
+SHAPE = (23, 70, 90)
+REQ_INDEX, REQ_SIZE = [1, 4, 2], [80, 60, 17]
+
+
+def _rel(a, b):
+    return float(np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _sigma_spacing(rxy, rz):
+    """sigma and spacing (1, 1, sz) with radii ceil(3 sigma) = rxy and
+    ceil(3 sigma / sz) = rz."""
+    sigma = rxy / 3.0 - 1e-3
+    sz = 3.0 * sigma / (rz - 0.5)
+    return sigma, [1.0, 1.0, sz]
+
+
+def _run(itk, path, fn):
+    itk.set_path(path)
+    try:
+        out = fn()
+        torch.cuda.synchronize()
+        return out
+    finally:
+        itk.set_path(itk.PATH_AUTO)
+
+
+@pytest.mark.parametrize("rxy", range(2, 13))
+def test_anisotropic_radii_on_fused_kernel(itk, orc, dev, rxy):
+    rng = np.random.default_rng(600 + rxy)
+    arr = rng.random(SHAPE).astype(np.float32)
+    x = torch.from_numpy(arr).to(dev)
+    size = list(reversed(SHAPE))
+    req = itk.Region(REQ_INDEX, REQ_SIZE)
+    g = rng.standard_normal(tuple(reversed(REQ_SIZE))).astype(np.float32)
+    for rz in range(1, rxy):
+        sigma, sp = _sigma_spacing(rxy, rz)
+        assert [orc.kernel_radius(sigma, s) for s in sp] == [rxy, rxy, rz]
+        img = itk.Image(itk.Region([0, 0, 0], size), tuple(sp), (0.0,) * 3, x)
+        f = itk.GaussianFilter(img, sigma)
+        ref = orc.generate_region(arr, [0, 0, 0], sp, sigma, REQ_INDEX, REQ_SIZE)
+        # EXACT, fused: the oracle's bits
+        f.arith = itk.EXACT
+        ex = _run(itk, itk.PATH_FUSED, lambda: f.generate_region(req).data.cpu().numpy())
+        assert np.array_equal(ex.view(np.uint32), ref.view(np.uint32)), (rxy, rz)
+        # FAST, fused and per-axis: the same FMA chain
+        f.arith = itk.FAST
+        fa = _run(itk, itk.PATH_FUSED, lambda: f.generate_region(req).data.cpu().numpy())
+        pa = _run(itk, itk.PATH_PER_AXIS, lambda: f.generate_region(req).data.cpu().numpy())
+        assert np.array_equal(fa, pa), (rxy, rz)
+        assert _rel(fa, ref) <= TOL, (rxy, rz)
+        # adjoint, fused, against the f64 transpose
+        gimg = itk.Image(req, tuple(sp), (0.0,) * 3, torch.from_numpy(g).to(dev))
+        ga = _run(itk, itk.PATH_FUSED, lambda: f.backward(gimg))
+        gref, in_i, in_s = orc.adjoint_f64(g.astype(np.float64), [0, 0, 0], size, sp, sigma,
+                                           REQ_INDEX, REQ_SIZE)
+        assert ga.region == itk.Region(in_i, in_s)
+        assert _rel(ga.data.cpu().numpy(), gref) <= TOL, (rxy, rz)
+
+
+def test_anisotropic_streaming_and_64_row_tiles(itk, dev, monkeypatch):
+    """radii (6, 6, 3) (sigma 2 at spacing 1 x 1 x 2) over 192 x 128 x 97:
+    64-row and 32-row tiles agree bit for bit, and stream_full (4 pieces, each
+    its own fused launch) is bit-identical to the full request."""
+    g = torch.Generator(device=dev).manual_seed(12)
+    x = torch.rand((97, 128, 192), generator=g, device=dev)
+    img = itk.Image(itk.Region([0, 0, 0], [192, 128, 97]), (1.0, 1.0, 2.0), (0.0,) * 3, x)
+    for arith in (itk.FAST, itk.EXACT):
+        f = itk.GaussianFilter(img, 2.0)
+        f.arith = arith
+        full = _run(itk, itk.PATH_FUSED, lambda: f.generate_region(f.largest_region()).data)
+        streamed = _run(itk, itk.PATH_FUSED, lambda: itk.stream_full(f, 4).data)
+        assert torch.equal(full.view(torch.int32), streamed.view(torch.int32))
+        if arith == itk.FAST:
+            monkeypatch.setenv("ITKG_FUSED_SHAPE", "32")
+            small = _run(itk, itk.PATH_FUSED, lambda: f.generate_region(f.largest_region()).data)
+            monkeypatch.delenv("ITKG_FUSED_SHAPE")
+            assert torch.equal(full.view(torch.int32), small.view(torch.int32))

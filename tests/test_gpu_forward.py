@@ -337,3 +337,38 @@ def test_fused_tile_shapes_agree_interleaved(itk, dev, monkeypatch, sigma):
     finally:
         itk.set_path(itk.PATH_AUTO)
     assert torch.equal(outs["64"], per)  # the per-axis FAST kernels follow the same FMA chain
+
+
+@pytest.mark.parametrize("r", [1, 3, 6, 7, 9])
+def test_per_axis_row_and_group_passes(itk, orc, dev, r):
+    """The per-axis path's axis-0 pass (outputs staged through LDS, stored 64
+    consecutive pixels per instruction) and its strided passes over rows of
+    a multiple of 4 pixels (4 adjacent pixels per lane, below radius 8), on
+    a sub-region request: EXACT bit for bit the oracle, FAST equal to the
+    fused kernel's FMA chain, the adjoint within 1e-5 of the f64 transpose;
+    x = 128 takes the 4-pixel groups, x = 130 the one-pixel lanes."""
+    sigma = r / 3.0 - 1e-3
+    for nx in (128, 130):
+        rng = np.random.default_rng(700 + r + nx)
+        arr = rng.random((19, 40, nx)).astype(np.float32)
+        img = _img(itk, arr, [0, 0, 0], [1.0, 1.0, 1.0], dev)
+        req = itk.Region([4, 3, 2], [nx - 8, 33, 15])
+        f = itk.GaussianFilter(img, sigma)
+        ref = orc.generate_region(arr, [0, 0, 0], [1.0] * 3, sigma, list(req.index), list(req.size))
+        itk.set_path(itk.PATH_PER_AXIS)
+        try:
+            f.arith = itk.EXACT
+            ex = f.generate_region(req).data.cpu().numpy()
+            f.arith = itk.FAST
+            fa = f.generate_region(req).data.cpu().numpy()
+            g = rng.standard_normal((15, 33, nx - 8)).astype(np.float32)
+            gimg = itk.Image(req, (1.0,) * 3, (0.0,) * 3, torch.from_numpy(g).to(dev))
+            ga = f.backward(gimg).data.cpu().numpy()
+        finally:
+            itk.set_path(itk.PATH_AUTO)
+        assert _bits_equal(ex, ref), nx
+        fused = f.generate_region(req).data.cpu().numpy()
+        assert np.array_equal(fa, fused), nx
+        gref, _, _ = orc.adjoint_f64(g.astype(np.float64), [0, 0, 0], [nx, 40, 19], [1.0] * 3, sigma,
+                                     list(req.index), list(req.size))
+        assert _rel_err(ga, gref) <= FAST_TOL, nx

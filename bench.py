@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="headline loop only (profiling)")
     ap.add_argument("--extras", default="",
                     help="comma list of extras groups to run (default all): "
-                         "c2, c5, c4, c5_batch, c1_c3, filters, host")
+                         "c2, c5, c4, c5_batch, c1_c3, filters, aniso, host")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--keep-gc", action="store_true",
                     help="leave Python's garbage collector as it is (no collection before a "
@@ -537,6 +537,49 @@ def main():
                                       "ms": round(wc / 10 * 1e3, 4),
                                       "edge_pixels": int(oc.data.sum().item())}
             del xc, oc, ic, disc, xx, yy
+            torch.cuda.empty_cache()
+
+        if want("aniso"):
+            # anisotropic spacing: slices twice the in-plane spacing give radii
+            # (6, 6, 3) at sigma 2 (512 x 512 x 256 f32), on the fused kernel and
+            # on the per-axis path (three round trips) that served such
+            # volumes before; and the per-axis path on the 512^3 headline volume
+            xa = torch.rand((256, 512, 512), device=dev)
+            ya = torch.empty_like(xa)
+            spa = (1.0, 1.0, 2.0)
+            ia = itk.Image(itk.Region([0, 0, 0], [512, 512, 256]), spa, (0.0,) * 3, xa)
+            oa = itk.Image(ia.region, spa, (0.0,) * 3, ya)
+            na = xa.numel()
+            res = {}
+            for name, path in (("fused", itk.PATH_FUSED), ("per_axis", itk.PATH_PER_AXIS)):
+                fa = itk.GaussianFilter(ia, 2.0)
+                fa.arith = arith
+                itk.set_path(path)
+                try:
+                    wa, msa = timed_loop(lambda: fa.generate_into(ia, oa), 20, 3, stream)
+                finally:
+                    itk.set_path(itk.PATH_AUTO)
+                res[name] = {"gvox_s": round(na * 20 / wa / 1e9, 3),
+                             "kernel_ms": round(float(np.mean(msa)), 4),
+                             "hbm_frac": round(8.0 * na / (np.mean(msa) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            extras["aniso_r663_512x512x256"] = res
+            extras["aniso_max_rel_err_vs_f64"] = max_rel_err(ya, itk.gaussian(xa.double(), 2.0, spa))
+            del xa, ya, ia, oa
+            torch.cuda.empty_cache()
+            xp = torch.rand((512, 512, 512), device=dev)
+            yp = torch.empty_like(xp)
+            ip = itk.Image(itk.Region([0, 0, 0], [512] * 3), (1.0,) * 3, (0.0,) * 3, xp)
+            op = itk.Image(ip.region, (1.0,) * 3, (0.0,) * 3, yp)
+            fp = itk.GaussianFilter(ip, 2.0)
+            fp.arith = arith
+            itk.set_path(itk.PATH_PER_AXIS)
+            try:
+                wp, msp = timed_loop(lambda: fp.generate_into(ip, op), 10, 3, stream)
+            finally:
+                itk.set_path(itk.PATH_AUTO)
+            extras["per_axis_r6_512"] = {"gvox_s": round(xp.numel() * 10 / wp / 1e9, 3),
+                                         "kernel_ms": round(float(np.mean(msp)), 4)}
+            del xp, yp, ip, op
             torch.cuda.empty_cache()
 
         if want("host"):

@@ -1367,14 +1367,21 @@ int recursive_impl(int ndim, itkg_dtype dtype, double sigma, const double* spaci
                         "recursive Gaussian with no filtered line longer than one sample: axis 1 "
                         "or axes 2-3 exceed 65535 rows");
     }
+    // the f64 work image and, after it, the largest checkpoint table of the
+    // filtered axes (one block)
+    int64_t ck = 0;
+    for (int k = 0; k < naxes; ++k)
+        if (largest.size[axes[k]] > 1)
+            ck = std::max(ck, recursive_ckpt_doubles(total, largest.size[axes[k]], axes[k]));
     void* work = nullptr;
-    hipError_t e = pool_alloc(&work, (size_t)total * sizeof(double), s);
+    hipError_t e = pool_alloc(&work, (size_t)(total + ck) * sizeof(double), s);
     if (e != hipSuccess) return hip_fail(e, "pool_alloc(recursive work)");
     auto run = [&](auto* tag) {
         using T = std::remove_pointer_t<decltype(tag)>;
         RecursiveArgs<T> a{};
         a.in = static_cast<const T*>(in.data) + view_offset(ndim, in, largest.index);
         a.work = static_cast<double*>(work);
+        a.ckpt = static_cast<double*>(work) + total;
         a.out = static_cast<T*>(out.data);
         for (int d = 0; d < ITKG_MAX_DIM; ++d) {
             a.n[d] = d < ndim ? largest.size[d] : 1;

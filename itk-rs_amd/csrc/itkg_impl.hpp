@@ -74,9 +74,20 @@ struct RecursiveArgs {
     const T* in;
     double* work;
     T* out;
+    double* ckpt;  // causal state at each chunk start (recursive_ckpt_doubles)
     int64_t n[4], m[4], off[4];
     int64_t in_stride[4], out_stride[4];
 };
+// Line chunks of the recursive passes: rows (axis 0) and other axes.
+constexpr int kRecRowChunk = 32;
+constexpr int kRecLineChunk = 16;
+// Doubles of the checkpoint table of one filtered axis: 3 per line per chunk
+// after the first.
+inline int64_t recursive_ckpt_doubles(int64_t total, int64_t len, int axis) {
+    const int64_t K = axis == 0 ? kRecRowChunk : kRecLineChunk;
+    const int64_t nch = (len + K - 1) / K;
+    return nch > 1 ? total / len * (nch - 1) * 3 : 0;
+}
 // Widen, run each listed axis's causal + anticausal lines with its
 // coefficients (B, a1, a2, a3), narrow the sub-region.
 template <typename T>

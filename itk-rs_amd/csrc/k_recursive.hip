@@ -53,49 +53,57 @@ __global__ __launch_bounds__(256) void widen_kernel(const RecursiveArgs<T> a) {
 // writes the output, so no pass of its own widens or narrows.
 constexpr int kFromIn = 1, kToOut = 2;
 
-// Samples per chunk of a row (axis 0) line: 16 f64 samples are one whole
-// 128-byte line of the work image, so no chunk leaves half a cache line to
-// be fetched (or written back) again by the next.
-#ifndef ITKG_REC_CHUNK
-#define ITKG_REC_CHUNK 32
-#endif
-constexpr int kChunk = ITKG_REC_CHUNK;
-constexpr int kGroup = 16;  // samples per load group of a strided line
-static_assert(kChunk % 4 == 0, "chunks are whole float4 / double2 groups");
+// A line goes in chunks of K samples (the last one possibly shorter):
+// kRecRowChunk on rows (axis 0; 16-byte loads and stores, each lane through
+// its own contiguous row) and kRecLineChunk on the other axes (one load per
+// sample, coalesced across the lanes' neighbouring lines).
+static_assert(kRecRowChunk % 4 == 0 && kRecLineChunk % 4 == 0,
+              "chunks are whole float4 / double2 groups");
 
-template <typename T>
-__device__ __forceinline__ void load_chunk(const T* q, double* c) {
+template <typename T, int K>
+__device__ __forceinline__ void load_vec(const T* q, double* c) {
     if constexpr (std::is_same_v<T, float>) {
 #pragma unroll
-        for (int j = 0; j < kChunk / 4; ++j) {
+        for (int j = 0; j < K / 4; ++j) {
             const float4 u = reinterpret_cast<const float4*>(q)[j];
             c[4 * j] = u.x, c[4 * j + 1] = u.y, c[4 * j + 2] = u.z, c[4 * j + 3] = u.w;
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kChunk / 2; ++j) {
+        for (int j = 0; j < K / 2; ++j) {
             const double2 t = reinterpret_cast<const double2*>(q)[j];
             c[2 * j] = t.x, c[2 * j + 1] = t.y;
         }
     }
 }
 
-__device__ __forceinline__ void store_chunk(double* q, const double* c) {
+template <int K>
+__device__ __forceinline__ void store_vec(double* q, const double* c) {
 #pragma unroll
-    for (int j = 0; j < kChunk / 2; ++j)
+    for (int j = 0; j < K / 2; ++j)
         reinterpret_cast<double2*>(q)[j] = double2{c[2 * j], c[2 * j + 1]};
 }
 
 // One line per thread along `axis`: lines enumerate the other axes, the
 // lowest of them fastest.  With kToOut, lines outside the request (in the
-// other axes) are not needed and not run.  ROWS (axis 0) compiles the chunked
-// row path; the other axes' kernels leave it out, and with it the chunk's
-// registers, which would cost them occupancy.
+// other axes) are not needed and not run.
+//
+// The causal pass stores nothing but, at the start of every chunk after the
+// first, its state (y[n-1], y[n-2], y[n-3]) in the checkpoint table
+// a.ckpt[((c - 1) * 3 + k) * lines + line] (neighbouring lines adjacent).
+// The anticausal pass walks the chunks backwards: it loads the chunk's
+// input again, recomputes the chunk's causal values from the checkpoint
+// (the same sequence of f64 operations from the same state, so the same
+// values the causal pass produced), runs the anticausal recursion over them
+// and stores the results.  Per sample that is two reads of the input and one
+// write, instead of one read, a causal write, its read and a write.  Each
+// chunk's successor (and its checkpoint) is loaded before the chunk's steps.
 template <typename T, int IO, bool ROWS>
 __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, const int axis,
                                                    const double B, const double a1, const double a2,
                                                    const double a3) {
     constexpr bool from_in = (IO & kFromIn) != 0, to_out = (IO & kToOut) != 0;
+    constexpr int K = ROWS ? kRecRowChunk : kRecLineChunk;
     int64_t pitch[4];
     pitch[0] = 1;
     for (int d = 1; d < 4; ++d) pitch[d] = pitch[d - 1] * a.n[d - 1];
@@ -132,108 +140,99 @@ __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, con
             p[i * stride] = v;
         }
     };
-    double init = src(0);
-    double ym1 = init, ym2 = init, ym3 = init;
-    auto step = [&](double x) {
-        const double v = B * x + a1 * ym1 + a2 * ym2 + a3 * ym3;
-        ym3 = ym2;
-        ym2 = ym1;
-        ym1 = v;
+    auto step = [&](double x, double& m1, double& m2, double& m3) {
+        const double v = B * x + a1 * m1 + a2 * m2 + a3 * m3;
+        m3 = m2;
+        m2 = m1;
+        m1 = v;
         return v;
     };
-    // Rows (stride 1) that start 16-byte aligned go in chunks of kChunk
-    // samples with 16-byte loads and stores, each lane through its own
-    // contiguous run; the samples still pass through `step` one by one.
-    bool rows = ROWS && stride == 1 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
-    if constexpr (from_in) rows = rows && is == 1 && (reinterpret_cast<uintptr_t>(q) & 15) == 0;
-    const int64_t body = rows ? len / kChunk * kChunk : 0;
-    // Each chunk's successor is loaded before the chunk's steps (two chunks
-    // of loads in flight per wave).
-    if constexpr (ROWS) {
-        double nxt[kChunk];
-        if (body > 0) {
-            if constexpr (from_in) load_chunk(q, nxt);
-            else load_chunk(p, nxt);
+    // rows (stride 1) that start 16-byte aligned load and store whole chunks
+    // as 16-byte vectors
+    bool vec = ROWS && stride == 1 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+    if constexpr (from_in) vec = vec && is == 1 && (reinterpret_cast<uintptr_t>(q) & 15) == 0;
+    const int64_t nch = (len + K - 1) / K;
+    double* const ck = a.ckpt + l;
+    auto load = [&](int64_t c, double (&x)[K]) {
+        const int64_t i0 = c * K;
+        if (ROWS && vec && i0 + K <= len) {
+            if constexpr (from_in) load_vec<T, K>(q + i0, x);
+            else load_vec<double, K>(p + i0, x);
+            return;
         }
-        for (int64_t i = 0; i < body; i += kChunk) {
-            double c[kChunk];
+        // the last chunk may be short: its missing samples read the last
+        // sample again (in bounds) and are not used
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) c[j] = nxt[j];
-            if (i + kChunk < body) {
-                if constexpr (from_in) load_chunk(q + i + kChunk, nxt);
-                else load_chunk(p + i + kChunk, nxt);
-            }
+        for (int j = 0; j < K; ++j) x[j] = src(i0 + j < len ? i0 + j : len - 1);
+    };
+    const double init = src(0);
+    // causal pass
+    double ym1 = init, ym2 = init, ym3 = init;
+    double nxt[K];
+    load(0, nxt);
+    for (int64_t c = 0; c < nch; ++c) {
+        double x[K];
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) c[j] = step(c[j]);
-            store_chunk(p + i, c);
+        for (int j = 0; j < K; ++j) x[j] = nxt[j];
+        if (c + 1 < nch) load(c + 1, nxt);
+        if (c > 0) {
+            ck[((c - 1) * 3 + 0) * lines] = ym1;
+            ck[((c - 1) * 3 + 1) * lines] = ym2;
+            ck[((c - 1) * 3 + 2) * lines] = ym3;
         }
-    }
-    // Samples outside the chunks (every sample of a strided line) go in
-    // groups of kGroup: the group's loads are issued together, ahead of its
-    // steps and stores, so each wave keeps kGroup loads in flight (one at a
-    // time would leave HBM latency-bound at this occupancy).
-    // The next group is loaded before the current group's steps (two groups
-    // in flight); it holds samples this pass has not written yet.
-    int64_t i = body;
-    if (i + kGroup <= len) {
-        double nxt[kGroup];
+        const int64_t cnt = len - c * K;
+        if (cnt >= K) {
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) nxt[j] = src(i + j);
-        for (; i + kGroup <= len; i += kGroup) {
-            double c[kGroup];
+            for (int j = 0; j < K; ++j) step(x[j], ym1, ym2, ym3);
+        } else {
 #pragma unroll
-            for (int j = 0; j < kGroup; ++j) c[j] = nxt[j];
-            if (i + 2 * kGroup <= len) {
-#pragma unroll
-                for (int j = 0; j < kGroup; ++j) nxt[j] = src(i + kGroup + j);
-            }
-#pragma unroll
-            for (int j = 0; j < kGroup; ++j) c[j] = step(c[j]);
-#pragma unroll
-            for (int j = 0; j < kGroup; ++j) p[(i + j) * stride] = c[j];
+            for (int j = 0; j < K; ++j)
+                if (j < cnt) step(x[j], ym1, ym2, ym3);
         }
     }
-    for (; i < len; ++i) p[i * stride] = step(src(i));
-    init = ym1;  // the line's last value after the causal pass
-    ym1 = ym2 = ym3 = init;
-    // backwards: the samples past the last whole chunk first (groups from
-    // the top, then single samples), then the chunks
-    i = len;
-    if (i - kGroup >= body) {
-        double nxt[kGroup];
+    // anticausal pass, from the line's last causal value
+    double am1 = ym1, am2 = ym1, am3 = ym1;
+    double nck[3] = {init, init, init};
+    load(nch - 1, nxt);
+    if (nch > 1)
 #pragma unroll
-        for (int j = 0; j < kGroup; ++j) nxt[j] = p[(i - kGroup + j) * stride];
-        for (; i - kGroup >= body; i -= kGroup) {
-            double c[kGroup];
+        for (int k = 0; k < 3; ++k) nck[k] = ck[((nch - 2) * 3 + k) * lines];
+    for (int64_t c = nch - 1; c >= 0; --c) {
+        double x[K];
 #pragma unroll
-            for (int j = 0; j < kGroup; ++j) c[j] = nxt[j];
-            if (i - 2 * kGroup >= body) {
+        for (int j = 0; j < K; ++j) x[j] = nxt[j];
+        double s1 = nck[0], s2 = nck[1], s3 = nck[2];  // causal state at the chunk's start
+        if (c > 0) {
+            load(c - 1, nxt);
+            if (c > 1) {
 #pragma unroll
-                for (int j = 0; j < kGroup; ++j) nxt[j] = p[(i - 2 * kGroup + j) * stride];
-            }
-#pragma unroll
-            for (int j = kGroup - 1; j >= 0; --j) c[j] = step(c[j]);
-#pragma unroll
-            for (int j = 0; j < kGroup; ++j) put(i - kGroup + j, c[j]);
-        }
-    }
-    for (; i > body; --i) put(i - 1, step(p[(i - 1) * stride]));
-    if constexpr (ROWS) {
-        double nxt[kChunk];
-        if (body > 0) load_chunk(p + body - kChunk, nxt);
-        for (int64_t i = body - kChunk; i >= 0; i -= kChunk) {
-            double c[kChunk];
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) c[j] = nxt[j];
-            if (i - kChunk >= 0) load_chunk(p + i - kChunk, nxt);
-#pragma unroll
-            for (int j = kChunk - 1; j >= 0; --j) c[j] = step(c[j]);
-            if constexpr (to_out) {
-#pragma unroll
-                for (int j = 0; j < kChunk; ++j) put(i + j, c[j]);
+                for (int k = 0; k < 3; ++k) nck[k] = ck[((c - 2) * 3 + k) * lines];
             } else {
-                store_chunk(p + i, c);
+                nck[0] = nck[1] = nck[2] = init;
             }
+        }
+        const int64_t i0 = c * K, cnt = len - i0;
+        if (cnt >= K) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) x[j] = step(x[j], s1, s2, s3);
+#pragma unroll
+            for (int j = K - 1; j >= 0; --j) x[j] = step(x[j], am1, am2, am3);
+            if (!to_out && ROWS && vec) {
+                store_vec<K>(p + i0, x);
+            } else {
+#pragma unroll
+                for (int j = 0; j < K; ++j) put(i0 + j, x[j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (j < cnt) x[j] = step(x[j], s1, s2, s3);
+#pragma unroll
+            for (int j = K - 1; j >= 0; --j)
+                if (j < cnt) x[j] = step(x[j], am1, am2, am3);
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (j < cnt) put(i0 + j, x[j]);
         }
     }
 }

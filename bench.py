@@ -512,14 +512,19 @@ def main():
                 "gvox_s": round(nm * 20 / wu / 1e9, 3), "ms": round(float(np.mean(msu)), 4),
                 "hbm_frac_20B": round(20.0 * nm / (np.mean(msu) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             # RecursiveGaussianFilter, sigma 2: a causal and an anticausal
-            # line pass per axis over an f64 work image, the first pass
-            # reading the f32 input and the last writing the f32 output; each
-            # pass streams every line twice: 28 + 32 + 28 = 88 B per voxel
+            # line pass per axis over an f64 work image, the first reading the
+            # f32 input and the last writing the f32 output; each pass reads
+            # its input twice (the anticausal pass recomputes the causal
+            # values chunk by chunk from checkpointed states: 48 B written and
+            # read per chunk of 32 samples) and writes once:
+            # (4 + 4 + 8 + 1.5) + (8 + 8 + 8 + 1.5) + (8 + 8 + 4 + 1.5)
+            # = 64.5 B per voxel
             rf = itk.RecursiveGaussianFilter(im, 2.0)
             wr, msr = timed_loop(lambda: rf.generate_into(im, om), 10, 2, stream)
             extras["recursive_s2_512"] = {
                 "gvox_s": round(nm * 10 / wr / 1e9, 3), "ms": round(float(np.mean(msr)), 4),
-                "hbm_frac_88B": round(88.0 * nm / (np.mean(msr) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                "bytes_per_voxel": 64.5,
+                "hbm_frac": round(64.5 * nm / (np.mean(msr) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             del xm, ym, im, om
             torch.cuda.empty_cache()
             # CannyEdgeDetectionFilter on a 2048^2 disc with noise (sigma 1.5):

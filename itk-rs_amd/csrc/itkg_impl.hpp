@@ -80,7 +80,7 @@ struct RecursiveArgs {
 };
 // Line chunks of the recursive passes: rows (axis 0) and other axes.
 constexpr int kRecRowChunk = 32;
-constexpr int kRecLineChunk = 16;
+constexpr int kRecLineChunk = 32;
 // Doubles of the checkpoint table of one filtered axis: 3 per line per chunk
 // after the first.
 inline int64_t recursive_ckpt_doubles(int64_t total, int64_t len, int axis) {

@@ -5,10 +5,13 @@
 // (recursive_gaussian.rs:55-86), each started from the line's first
 // (causal) or last (anticausal) value; then the requested region narrowed to
 // the pixel type.  The widening happens as the first pass reads the input
-// and the narrowing as the last pass writes the output.  One thread runs one line; consecutive threads take
-// neighbouring lines, which are adjacent in memory (coalesced) for every
-// axis but axis 0.  Compiled with -ffp-contract=off: each product and sum
-// rounds on its own, in the reference's order.
+// and the narrowing as the last pass writes the output.  One thread runs one
+// line; consecutive threads take neighbouring lines, which are adjacent in
+// memory (coalesced) for every axis but axis 0, whose rows a wave moves
+// through LDS.  The causal values are not stored: the anticausal pass
+// recomputes them chunk by chunk from checkpointed states.  Compiled with
+// -ffp-contract=off: each product and sum rounds on its own, in the
+// reference's order.
 #include <type_traits>
 
 #include "itkg_impl.hpp"
@@ -84,6 +87,224 @@ __device__ __forceinline__ void store_vec(double* q, const double* c) {
         reinterpret_cast<double2*>(q)[j] = double2{c[2 * j], c[2 * j + 1]};
 }
 
+// Rows (axis 0, the first filtered axis: the input is read) with the
+// chunks of a wave's 64 rows moved through LDS: the wave loads each chunk
+// of its rows cooperatively, 16 bytes per lane (an instruction covers a few
+// whole row segments instead of 64 scattered pieces), each lane then reads
+// its own row's chunk from a padded tile, and the results for the f64 work
+// image go back the same way.  The recursion, the checkpoints and the
+// order of every operation are those of line_kernel.  Returns false (and
+// does nothing) where a wave's rows are not 16-byte aligned; every wave of
+// the launch decides alike (the test depends on the strides and the base
+// alone), so a launch runs one path or the other.
+template <typename T, int IO>
+__device__ __forceinline__ bool rows_cooperative(const RecursiveArgs<T>& a, const int64_t len,
+                                                 const int64_t lines, const double B,
+                                                 const double a1, const double a2,
+                                                 const double a3) {
+    constexpr bool to_out = (IO & kToOut) != 0;
+    constexpr int K = kRecRowChunk;
+    constexpr int S = K + 1;  // tile row stride (doubles)
+    constexpr int VPR = std::is_same_v<T, float> ? K / 4 : K / 2;  // 16-byte vectors per row chunk
+    constexpr int RPI = 64 / VPR;                                   // rows per instruction
+    static_assert((K == 16 || K == 32) && 64 % VPR == 0, "16- or 32-sample row chunks");
+    // every row of the input and of the work image starts 16-byte aligned
+    // (the row pitch and the bases), the input's samples are contiguous
+    const bool aligned = a.in_stride[0] == 1 && (reinterpret_cast<uintptr_t>(a.in) & 15) == 0 &&
+                         (a.in_stride[1] * (int64_t)sizeof(T)) % 16 == 0 &&
+                         (a.in_stride[2] * (int64_t)sizeof(T)) % 16 == 0 &&
+                         (a.in_stride[3] * (int64_t)sizeof(T)) % 16 == 0 &&
+                         (to_out || ((reinterpret_cast<uintptr_t>(a.work) & 15) == 0 && a.n[0] % 2 == 0));
+    if (!aligned) return false;
+    __shared__ double tiles[4][64 * S];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* const tile = tiles[wave];
+    const int64_t lw = (int64_t)blockIdx.x * 256 + wave * 64;  // the wave's first line
+    if (lw >= lines) return true;
+    const int nrow = (int)(lines - lw < 64 ? lines - lw : 64);
+    // this lane's row: the line index's coordinates on axes 1..3 (axis 1
+    // fastest), its input row and, with kToOut, its output row
+    const int64_t l = lw + lane;
+    int64_t rem = l < lines ? l : lines - 1, ibase = 0, obase = 0;
+    bool active = l < lines;
+    for (int d = 1; d < 4; ++d) {
+        const int64_t c = rem % a.n[d];
+        rem /= a.n[d];
+        ibase += c * a.in_stride[d];
+        if constexpr (to_out) {
+            const int64_t o = c - a.off[d];
+            active = active && o >= 0 && o < a.m[d];
+            obase += (o < 0 ? 0 : o) * a.out_stride[d];
+        }
+    }
+    const T* q = a.in + ibase;
+    T* o = a.out + obase;
+    double* p = a.work + (l < lines ? l : lines - 1) * a.n[0];
+    const int64_t os = a.out_stride[0];
+    const int64_t lo = a.off[0], hi = lo + a.m[0];
+    auto step = [&](double x, double& m1, double& m2, double& m3) {
+        const double v = B * x + a1 * m1 + a2 * m2 + a3 * m3;
+        m3 = m2;
+        m2 = m1;
+        m1 = v;
+        return v;
+    };
+    // another lane's row start (a 64-bit pointer moved as two 32-bit halves)
+    auto row_ptr = [](auto* ptr, int src_lane) {
+        const uint64_t u = reinterpret_cast<uint64_t>(ptr);
+        const uint32_t lo32 = __shfl((uint32_t)u, src_lane), hi32 = __shfl((uint32_t)(u >> 32), src_lane);
+        return reinterpret_cast<decltype(ptr)>(((uint64_t)hi32 << 32) | lo32);
+    };
+    auto sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // cooperative load of chunk c: vector v of row r = RPI i + lane / VPR,
+    // part lane % VPR; rows past the wave's lines and samples past the line
+    // end are not loaded (the short last chunk loads whole vectors while
+    // they lie inside the row)
+    const int vpart = lane % VPR, vrow = lane / VPR;
+    const int64_t nch = (len + K - 1) / K;
+    using V = std::conditional_t<std::is_same_v<T, float>, float4, double2>;
+    constexpr int EPV = std::is_same_v<T, float> ? 4 : 2;  // samples per vector
+    auto vload = [&](int64_t c, V (&v)[64 / RPI]) {
+#pragma unroll
+        for (int i = 0; i < 64 / RPI; ++i) {
+            const int r = RPI * i + vrow;
+            const int64_t s0 = c * K + (int64_t)EPV * vpart;
+            const T* qr = row_ptr(q, r < nrow ? r : nrow - 1);
+            v[i] = V{};
+            if (r < nrow && s0 + EPV <= len) v[i] = *reinterpret_cast<const V*>(qr + s0);
+        }
+    };
+    auto vput = [&](const V (&v)[64 / RPI]) {
+#pragma unroll
+        for (int i = 0; i < 64 / RPI; ++i) {
+            double* t = tile + (RPI * i + vrow) * S + EPV * vpart;
+            if constexpr (std::is_same_v<T, float>) {
+                t[0] = v[i].x, t[1] = v[i].y, t[2] = v[i].z, t[3] = v[i].w;
+            } else {
+                t[0] = v[i].x, t[1] = v[i].y;
+            }
+        }
+    };
+    // a short last chunk's samples that no whole vector covers, one by one
+    auto tail_fix = [&](int64_t c) {
+        const int64_t i0 = c * K;
+        for (int64_t s0 = (len - i0) / EPV * EPV; s0 < len - i0; ++s0)
+            if (l < lines) tile[lane * S + s0] = (double)q[i0 + s0];
+    };
+    double x[K];
+    auto read_row = [&] {
+#pragma unroll
+        for (int j = 0; j < K; ++j) x[j] = tile[lane * S + j];
+    };
+    V nxt[64 / RPI];
+    const double init = (double)q[0];
+    // causal pass: only the checkpoints are stored
+    double ym1 = init, ym2 = init, ym3 = init;
+    double* const ck = a.ckpt + (l < lines ? l : lines - 1);
+    vload(0, nxt);
+    for (int64_t c = 0; c < nch; ++c) {
+        sync();  // every lane is done reading the tile
+        vput(nxt);
+        if (c * K + K > len) {
+            sync();
+            tail_fix(c);
+        }
+        if (c + 1 < nch) vload(c + 1, nxt);
+        sync();
+        read_row();
+        if (c > 0 && l < lines) {
+            ck[((c - 1) * 3 + 0) * lines] = ym1;
+            ck[((c - 1) * 3 + 1) * lines] = ym2;
+            ck[((c - 1) * 3 + 2) * lines] = ym3;
+        }
+        const int64_t cnt = len - c * K;
+        if (cnt >= K) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) step(x[j], ym1, ym2, ym3);
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (j < cnt) step(x[j], ym1, ym2, ym3);
+        }
+    }
+    // anticausal pass
+    double am1 = ym1, am2 = ym1, am3 = ym1;
+    double nck[3] = {init, init, init};
+    if (nch > 1 && l < lines)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) nck[k] = ck[((nch - 2) * 3 + k) * lines];
+    vload(nch - 1, nxt);
+    for (int64_t c = nch - 1; c >= 0; --c) {
+        sync();
+        vput(nxt);
+        if (c * K + K > len) {
+            sync();
+            tail_fix(c);
+        }
+        double s1 = nck[0], s2 = nck[1], s3 = nck[2];  // causal state at the chunk's start
+        if (c > 0) {
+            vload(c - 1, nxt);
+            if (c > 1 && l < lines) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) nck[k] = ck[((c - 2) * 3 + k) * lines];
+            } else {
+                nck[0] = nck[1] = nck[2] = init;
+            }
+        }
+        sync();
+        read_row();
+        const int64_t i0 = c * K, cnt = len - i0;
+        if (cnt >= K) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) x[j] = step(x[j], s1, s2, s3);
+#pragma unroll
+            for (int j = K - 1; j >= 0; --j) x[j] = step(x[j], am1, am2, am3);
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (j < cnt) x[j] = step(x[j], s1, s2, s3);
+#pragma unroll
+            for (int j = K - 1; j >= 0; --j)
+                if (j < cnt) x[j] = step(x[j], am1, am2, am3);
+        }
+        if constexpr (to_out) {
+            if (active) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const int64_t i = i0 + j;
+                    if (j < cnt && i >= lo && i < hi) o[(i - lo) * os] = (T)x[j];
+                }
+            }
+        } else {
+            // the results back through the tile, stored 16 bytes per lane
+            sync();  // every lane is done reading the tile
+#pragma unroll
+            for (int j = 0; j < K; ++j) tile[lane * S + j] = x[j];
+            sync();
+            // K / 2 lanes per row (16 bytes = 2 doubles each), 128 / K rows
+            // per instruction
+            constexpr int LPR = K / 2, RPS = 64 / LPR;
+#pragma unroll
+            for (int i = 0; i < 64 / RPS; ++i) {
+                const int r = RPS * i + lane / LPR, part = lane % LPR;
+                const int64_t s0 = i0 + 2 * part;
+                double* const pr = row_ptr(p, r < nrow ? r : 0);
+                const double* t = tile + r * S + 2 * part;
+                if (r < nrow && s0 + 2 <= len) {
+                    *reinterpret_cast<double2*>(pr + s0) = double2{t[0], t[1]};
+                } else if (r < nrow && s0 < len) {
+                    pr[s0] = t[0];
+                }
+            }
+        }
+    }
+    return true;
+}
+
 // One line per thread along `axis`: lines enumerate the other axes, the
 // lowest of them fastest.  With kToOut, lines outside the request (in the
 // other axes) are not needed and not run.
@@ -109,6 +330,9 @@ __global__ __launch_bounds__(256) void line_kernel(const RecursiveArgs<T> a, con
     for (int d = 1; d < 4; ++d) pitch[d] = pitch[d - 1] * a.n[d - 1];
     const int64_t len = a.n[axis], stride = pitch[axis];
     const int64_t lines = a.n[0] * a.n[1] * a.n[2] * a.n[3] / len;
+    if constexpr (ROWS && from_in) {
+        if (rows_cooperative<T, IO>(a, len, lines, B, a1, a2, a3)) return;
+    }
     const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (l >= lines) return;
     int64_t rem = l, base = 0, ibase = 0, obase = 0;

@@ -257,6 +257,37 @@ def test_recursive_matches_oracle(itk, orc, dev, size, spacing, sigma, axis, req
     assert np.array_equal(out.data.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("width,pitch,axis,dtype,req", [
+    (83, 100, -1, np.float32, None),
+    (83, 100, 0, np.float32, ((5, 2, 1), (70, 30, 8))),
+    (50, 52, -1, np.float32, ((3, 0, 0), (45, 37, 11))),
+    (17, 20, 0, np.float32, None),
+    (83, 96, -1, np.float64, ((1, 1, 1), (80, 35, 9))),
+    (69, 70, 0, np.float64, None),
+    (128, 128, -1, np.float32, None),
+])
+def test_recursive_rows_of_a_window(itk, orc, dev, width, pitch, axis, dtype, req):
+    """The input a window of `width` samples per row into rows of `pitch`
+    samples (16-byte-aligned rows whose length is not a multiple of 4): axis
+    0 goes through the wave-cooperative LDS path, with whole 16-byte vectors
+    up to the row end and the last samples one by one; a short last chunk;
+    all axes or axis 0 alone (which reads the input and writes the output
+    itself); bit for bit the oracle."""
+    rng = np.random.default_rng(800 + width + pitch)
+    full = rng.random((11, 37, pitch)).astype(dtype)
+    x = full[:, :, :width]
+    src = itk.Image(itk.Region([0, 0, 0], [width, 37, 11]), (1.0, 1.0, 1.0), (0.0,) * 3,
+                    torch.from_numpy(full).to(dev)[:, :, :width])
+    f = itk.RecursiveGaussianFilter(src, 1.9) if axis < 0 else \
+        itk.RecursiveGaussianAxisFilter(src, 1.9, axis)
+    reg = src.region if req is None else itk.Region(list(req[0]), list(req[1]))
+    out = f.generate_region(reg)
+    torch.cuda.synchronize()
+    ref = orc.recursive_generate_region(np.ascontiguousarray(x), [0, 0, 0], [1.0, 1.0, 1.0], 1.9,
+                                        axis, list(reg.index), list(reg.size))
+    assert np.array_equal(out.data.cpu().numpy(), ref)
+
+
 def test_recursive_axis_1_beyond_65535(itk, orc, dev):
     """A 16 x 100000 image: the line kernels take 1-D grids, so axis 1 may
     exceed the 65535 rows of a grid dimension."""

@@ -155,6 +155,61 @@ __global__ __launch_bounds__(kConvBlock) void convert_rows4_kernel(const Convert
     }
 }
 
+// Views that are one dense run of elements on both sides (whole contiguous
+// images, the usual CastFilter call): a flat range of quads, kConvFlatU per
+// lane, kConvFlatU * kConvBlock per block, so a wave moves several rows'
+// worth of bytes where a row-per-block grid (a 512-pixel row is 128 quads)
+// would leave half of each block idle and spend its time in workgroup
+// dispatch.  The last n % 4 elements, if any, go one per lane in block 0.
+constexpr int kConvFlatU = 2;
+
+template <class S, class D, bool ROUND>
+__global__ __launch_bounds__(kConvBlock) void convert_flat4_kernel(const S* in, D* out,
+                                                                   const uint32_t quads,
+                                                                   const uint32_t tail) {
+    const uint32_t q0 = blockIdx.x * (uint32_t)(kConvFlatU * kConvBlock) + threadIdx.x;
+    const Quad<S>* in4 = reinterpret_cast<const Quad<S>*>(in);
+    Quad<D>* out4 = reinterpret_cast<Quad<D>*>(out);
+    Quad<S> v[kConvFlatU];
+#pragma unroll
+    for (int k = 0; k < kConvFlatU; ++k) {
+        const uint32_t q = q0 + (uint32_t)(k * kConvBlock);
+        if (q < quads) v[k] = in4[q];
+    }
+#pragma unroll
+    for (int k = 0; k < kConvFlatU; ++k) {
+        const uint32_t q = q0 + (uint32_t)(k * kConvBlock);
+        if (q < quads) {
+            Quad<D> r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                S t = v[k].v[i];
+                if constexpr (ROUND && std::is_floating_point_v<S>) t = round_half_away(t);
+                r.v[i] = rust_as<D>(t);
+            }
+            out4[q] = r;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < tail) {
+        const int64_t j = (int64_t)quads * 4 + threadIdx.x;
+        S t = in[j];
+        if constexpr (ROUND && std::is_floating_point_v<S>) t = round_half_away(t);
+        out[j] = rust_as<D>(t);
+    }
+}
+
+// True when the view's elements (pixels times components) are one dense
+// run in the order x, then axes 1..3: every axis of more than one pixel
+// sits at the stride of the axes below it.
+inline bool dense_run(const int64_t* n, const int64_t* stride) {
+    int64_t p = 1;
+    for (int d = 0; d < ITKG_MAX_DIM; ++d) {
+        if (n[d] > 1 && stride[d] != p) return false;
+        p *= n[d];
+    }
+    return true;
+}
+
 template <class S, class D>
 bool quad_ok(const ConvertArgs& a) {
     if (a.ncomp != 1 || a.in_stride[0] != 1 || a.out_stride[0] != 1 || a.n[0] % 4) return false;
@@ -167,6 +222,23 @@ bool quad_ok(const ConvertArgs& a) {
 template <class S, class D>
 hipError_t launch_sd(const ConvertArgs& a, bool round, hipStream_t s) {
     const int64_t row_elems = a.n[0] * a.ncomp;
+    const int64_t elems = row_elems * a.n[1] * a.n[2] * a.n[3];
+    if (dense_run(a.n, a.in_stride) && dense_run(a.n, a.out_stride) &&
+        (uintptr_t)a.in % (4 * sizeof(S)) == 0 && (uintptr_t)a.out % (4 * sizeof(D)) == 0 &&
+        elems / 4 < (INT64_C(1) << 32) - kConvFlatU * kConvBlock) {
+        const int64_t quads = elems / 4;
+        const int64_t per_block = (int64_t)kConvFlatU * kConvBlock;
+        const unsigned g = (unsigned)std::max<int64_t>(1, (quads + per_block - 1) / per_block);
+        const S* in = static_cast<const S*>(a.in);
+        D* out = static_cast<D*>(a.out);
+        if (round)
+            hipLaunchKernelGGL((convert_flat4_kernel<S, D, true>), dim3(g), dim3(kConvBlock), 0, s,
+                               in, out, (uint32_t)quads, (uint32_t)(elems % 4));
+        else
+            hipLaunchKernelGGL((convert_flat4_kernel<S, D, false>), dim3(g), dim3(kConvBlock), 0, s,
+                               in, out, (uint32_t)quads, (uint32_t)(elems % 4));
+        return hipGetLastError();
+    }
     if (row_elems < (INT64_C(1) << 31) && a.n[1] <= 65535 && a.n[2] * a.n[3] <= 65535 &&
         quad_ok<S, D>(a)) {
         const int64_t quads = a.n[0] / 4;

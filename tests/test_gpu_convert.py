@@ -15,6 +15,7 @@ NAN, INF = float("nan"), float("inf")
 TYPES = [np.uint8, np.int16, np.int32, np.float32, np.float64]
 TORCH = {np.uint8: torch.uint8, np.int16: torch.int16, np.int32: torch.int32,
          np.float32: torch.float32, np.float64: torch.float64}
+_PT = {np.uint8: 0, np.int16: 2, np.int32: 3, np.float32: 4, np.float64: 5}  # itkg_ptype
 
 
 def _values(dt, rng, n=4096):
@@ -45,8 +46,8 @@ def _same(a, b):
 def test_every_type_pair_matches_rust_casts(itk, orc, dev, src, mode):
     rng = np.random.default_rng(2 * TYPES.index(src) + mode)
     full = _values(src, rng)
-    # an odd length (one element per lane) and a multiple of four (dense
-    # aligned rows: four elements per lane)
+    # a dense run of odd length (four elements per lane, the last one in
+    # block 0) and one of a multiple of four
     for x in (full, np.ascontiguousarray(full[: full.size // 4 * 4])):
         xin = itk.Image(itk.Region([0], [x.size]), (1.0,), (0.0,), torch.from_numpy(x).to(dev))
         for dst in TYPES:
@@ -56,6 +57,37 @@ def test_every_type_pair_matches_rust_casts(itk, orc, dev, src, mode):
             v = orc.rust_round(x) if (mode == 1 and np.issubdtype(np.dtype(src), np.floating)) else x
             ref = orc.rust_as(v, dst)
             assert _same(out.data.cpu().numpy(), ref), (np.dtype(src), np.dtype(dst), mode, x.size)
+
+
+@pytest.mark.parametrize("src", TYPES)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_every_type_pair_on_windows(itk, orc, dev, src, mode):
+    """64 rows of a wider buffer into 64 rows of another wider buffer: rows
+    of a multiple of four elements at strides of multiples of four (four
+    elements per lane, row by row) and rows of 63 at odd strides (one
+    element per lane); the padding of the output buffer stays untouched."""
+    rng = np.random.default_rng(40 + 2 * TYPES.index(src) + mode)
+    vals = _values(src, rng, 5000)
+    for w, pin, pout in ((64, 68, 72), (63, 65, 67)):
+        xb = np.ascontiguousarray(vals[: 64 * pin].reshape(64, pin))
+        img = itk.Image(itk.Region([0, 0], [pin, 64]), (1.0, 1.0), (0.0, 0.0),
+                        torch.from_numpy(xb).to(dev))
+        req = itk.Region([0, 0], [w, 64])
+        win_in = itk.Image(req, (1.0, 1.0), (0.0, 0.0), img.data[img._slices(req)])
+        x = xb[:, :w]
+        v = orc.rust_round(x) if (mode == 1 and np.issubdtype(np.dtype(src), np.floating)) else x
+        for dst in TYPES:
+            ob = torch.zeros((64, pout), dtype=TORCH[dst], device=dev)
+            big = itk.Image(itk.Region([0, 0], [pout, 64]), (1.0, 1.0), (0.0, 0.0), ob)
+            out = itk.Image(req, (1.0, 1.0), (0.0, 0.0), big.data[big._slices(req)])
+            rc = itk.lib.itkg_convert_pixels(2, _PT[src], _PT[dst], 1, ctypes.byref(win_in.view()),
+                                             ctypes.byref(out.view()), mode, itk._stream_ptr(dev))
+            assert rc == 0, itk.lib.itkg_last_error()
+            torch.cuda.synchronize()
+            got = ob.cpu().numpy()
+            assert _same(np.ascontiguousarray(got[:, :w]), orc.rust_as(v, dst)), \
+                (np.dtype(src), np.dtype(dst), mode, w)
+            assert (got[:, w:] == 0).all()
 
 
 def test_strided_subregion_and_components(itk, orc, dev):
@@ -115,12 +147,16 @@ def test_host_chain_u16_to_f32(itk, orc):
 
 
 def test_rows_beyond_the_grid_limit(itk, orc, dev):
-    # axis 1 longer than 65535 rows: the generic conversion kernel
+    # axis 1 longer than 65535 rows: the whole image is one dense run (the
+    # flat kernel, 350005 elements: a tail of one); its first four columns
+    # are not (the generic conversion kernel)
     rng = np.random.default_rng(4)
     x = (rng.standard_normal((70001, 5)) * 200).astype(np.float32)
     img = itk.Image(itk.Region([0, 0], [5, 70001]), (1.0, 1.0), (0.0, 0.0), torch.from_numpy(x).to(dev))
-    for dst, mode in ((torch.uint8, itk.CONVERT_ROUND), (torch.int16, itk.CONVERT_AS)):
-        out = itk.CastFilter(img, dst, mode).generate_region(img.region)
-        v = orc.rust_round(x) if mode == itk.CONVERT_ROUND else x
-        ref = orc.rust_as(v, np.uint8 if dst == torch.uint8 else np.int16)
-        assert _same(out.data.cpu().numpy(), ref), dst
+    for req in (img.region, itk.Region([0, 0], [4, 70001])):
+        for dst, mode in ((torch.uint8, itk.CONVERT_ROUND), (torch.int16, itk.CONVERT_AS)):
+            out = itk.CastFilter(img, dst, mode).generate_region(req)
+            xs = x[:, : req.size[0]]
+            v = orc.rust_round(xs) if mode == itk.CONVERT_ROUND else xs
+            ref = orc.rust_as(v, np.uint8 if dst == torch.uint8 else np.int16)
+            assert _same(out.data.cpu().numpy(), ref), (dst, req)

@@ -238,7 +238,8 @@ int itkg_index_gaussian_kernel(double sigma, double* out, uint64_t cap, uint64_t
  * on every axis, then out = orig + amount * (orig - blur) where
  * |orig - blur| > threshold, else orig, in f64, narrowed to the pixel type.
  * input covers output->region padded by the radius, clipped to `largest`.
- * arith selects the blur's arithmetic (EXACT: bit-identical). */
+ * arith selects the blur's arithmetic (EXACT: bit-identical).  Output and
+ * input must not overlap. */
 int itkg_unsharp_mask_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, double amount,
                                       double threshold, const itkg_region* largest,
                                       const itkg_view* input, const itkg_view* output,

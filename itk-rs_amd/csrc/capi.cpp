@@ -568,12 +568,14 @@ int pick_shape(const Plan& p, const FusedEntry* fe, int mode, int ncomp) {
 }
 
 int run_fused(const Plan& p, int ncomp, const itkg_view& src_view, const itkg_view& dst_view,
-              int mode, int accumulate, hipStream_t s) {
+              int mode, int accumulate, hipStream_t s, double amount = 0.0, double threshold = 0.0) {
     const FusedEntry* fe = fused_for(p);
     mode = pick_shape(p, fe, mode, ncomp);
     FusedMulti m;
     std::memset(&m, 0, sizeof m);
     m.nc = ncomp;  // one launch over (tile, component) items
+    m.amount = amount;
+    m.threshold = threshold;
     for (int c = 0; c < ncomp; ++c)
         fill_fused_args(p, src_view, dst_view, mode, accumulate, 0, c, ncomp, &m.a[c]);
     hipError_t e = fe->launch(m, mode, s);
@@ -1216,6 +1218,12 @@ int unsharp_impl(int ndim, itkg_dtype dtype, double sigma, double amount, double
     if ((rc = check_view(ndim, &out, p.req, "output"))) return rc;
     const int64_t n = region_len(ndim, p.req);
     if (n == 0) return ITKG_OK;
+    // f32 on the fused kernel: the step in the blur's own launch, one round
+    // trip (the blur never reaches memory)
+    const int umode = arith == ITKG_FAST ? FUSED_FWD_FAST_U : FUSED_FWD_EXACT_U;
+    if (ndim >= 2 && use_fused(p, dtype, 1) && fused_src_ok(p, in, 1, false) &&
+        fused_for(p)->ty[umode] > 0)
+        return run_fused(p, 1, in, out, umode, 0, s, amount, threshold);
     const size_t px = dtype_size(dtype);
     void* blur = nullptr;
     hipError_t e = pool_alloc(&blur, (size_t)n * px, s);

@@ -92,6 +92,8 @@ __device__ __forceinline__ void lds_barrier_tracked() {
 //   VecPixel radii (13, 13, 4), bit-identical.
 template <int RXY, int RZ, int MODE>
 struct Cfg {
+    // the unsharp modes take the tiles and registers of their plain mode
+    static constexpr int BM = fused_base_mode(MODE);
     static constexpr bool EXACT = fused_is_exact(MODE);
     static constexpr bool EXACT_MID = EXACT && RZ >= 9 && RZ <= 14;
     static constexpr int NW = !EXACT ? 16 : (EXACT_MID ? 8 : 16);
@@ -100,9 +102,9 @@ struct Cfg {
     // _W modes, 128: the waves alternate between the two 64-column
     // halves (wave w: half w % HALVES, row group w / HALVES), so each source
     // row of the window is one 128 + 2r segment.
-    static constexpr int TX = fused_tile_x(MODE);
+    static constexpr int TX = fused_tile_x(BM);
     static constexpr int HALVES = TX / 64;
-    static constexpr bool SMALL = MODE == FUSED_FWD_FAST_S || MODE == FUSED_ADJ_S;
+    static constexpr bool SMALL = BM == FUSED_FWD_FAST_S || BM == FUSED_ADJ_S;
     // radii from 18 up (FMA modes): one column per lane, so that the ring
     // (2r + 1 registers a column) fits
     // Anisotropic radii (x/y above z): the adjoint from x/y radius 11 with z
@@ -113,7 +115,7 @@ struct Cfg {
                                    ? ((EXACT_MID && RZ <= 10 && !WIDE_XY) || RZ <= 5 ? 4 : 2)
                                    : (RZ >= 18 ? 1
                                                : (RZ <= 7 && !SMALL &&
-                                                          !(MODE == FUSED_ADJ && WIDE_XY && RZ >= 6)
+                                                          !(BM == FUSED_ADJ && WIDE_XY && RZ >= 6)
                                                       ? 4
                                                       : 2));
     static constexpr int TY = NW / HALVES * CPT;
@@ -164,7 +166,7 @@ struct Cfg {
     // registers for the whole tile and the first x-pass round's LDS reads
     // are issued before the z step.  Measured 4-6% faster for the adjoint;
     // no gain for forward FAST, and larger radii would spill.
-    static constexpr bool EARLY = MODE == FUSED_ADJ && RXY <= 7 && RZ <= 7;
+    static constexpr bool EARLY = BM == FUSED_ADJ && RXY <= 7 && RZ <= 7;
     // Register sets of window loads: a plane is loaded NSET steps before it
     // is stored to LDS.  4 where the registers fit without spilling and it
     // measured faster: forward FAST with 64-row tiles up to radius 6 (126
@@ -173,12 +175,12 @@ struct Cfg {
     // elsewhere.
     static constexpr int NSET =
         !EXACT && RXY <= 10 && RZ <= 10 &&
-                (CPT == 2 || ((MODE == FUSED_FWD_FAST || MODE == FUSED_FWD_FAST_W) && RXY <= 6 && RZ <= 6))
+                (CPT == 2 || ((BM == FUSED_FWD_FAST || BM == FUSED_FWD_FAST_W) && RXY <= 6 && RZ <= 6))
             ? 4
             : 2;
     static_assert(WX - TX <= 64, "halo wider than one wave");
     static_assert(HALVES == 1 ||
-                      ((MODE == FUSED_FWD_FAST_W || MODE == FUSED_FWD_EXACT_W) && NW % HALVES == 0),
+                      ((BM == FUSED_FWD_FAST_W || BM == FUSED_FWD_EXACT_W) && NW % HALVES == 0),
                   "128-wide tiles: forward FAST and EXACT only");
     static_assert((2 * PLANE + 2 * XPLANE) * 4 <= 160 * 1024, "LDS budget of one workgroup");
 };
@@ -270,8 +272,9 @@ __device__ __forceinline__ float tap_sum(const float* v, const Weights<N>& w) {
 // loop holds more registers: there 64-row tiles keep two load sets, the
 // adjoint keeps the plain step order and all outputs take plain stores.
 template <int RXY, int RZ, int MODE, bool PERSIST, class ARGS = Fused3dArgs>
-__device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, float* sIn,
-                                             float* sX) {
+__device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, float* sIn, float* sX,
+                                             [[maybe_unused]] const double amount = 0.0,
+                                             [[maybe_unused]] const double threshold = 0.0) {
     using C = Cfg<RXY, RZ, MODE>;
     constexpr int CPT = C::CPT, P = C::P, LJ = C::LJ;
     // The persistent kernel uses the register-hungry options only where its
@@ -283,6 +286,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     constexpr bool PFIT = FMA && ((RXY <= 6 && RZ <= 6) || (CPT == 2 && RXY <= 10 && RZ <= 10));
     constexpr int NSET = PERSIST && CPT == 4 ? 2 : C::NSET;
     constexpr bool EARLY = C::EARLY && !PERSIST;
+    static_assert(!EARLY || !fused_is_unsharp(MODE), "the unsharp loads follow the plain step order");
     constexpr bool PNT = !PERSIST || PFIT;   // non-temporal dense stores
     constexpr bool PASM = !PERSIST || PFIT;  // opaque load offsets (scalar-base loads)
     constexpr bool ADJOINT = fused_is_adjoint(MODE);
@@ -713,6 +717,25 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
         for (int j = 0; j < P; ++j) wzs[j] = a.wz2[s + P - j];
         pin_sgprs(wzs);
     };
+    // Unsharp modes: the source pixels at the output pixels of step u (the
+    // centre tap of each; indexes clamped into the tile, so that the loads
+    // are unconditional and in bounds), loaded one step ahead and before
+    // that step's window loads: the wait for them then never waits for the
+    // newest window set.  At z radius 15 the registers held a step longer
+    // would spill: there they are loaded where they are used.
+    constexpr bool OG_AHEAD = fused_is_unsharp(MODE) && RZ < 15;
+    [[maybe_unused]] float og[CPT];
+    auto load_orig = [&](int u) {
+        if constexpr (fused_is_unsharp(MODE)) {
+            const int t = min(max(u - 2 * RZ, 0), tz_act - 1);
+            const global_ptr<const float> scol =
+                to_global(a.src) + (int64_t)(min(ox, ndx - 1) + a.shift[0]) * a.src_stride[0] +
+                (int64_t)(z0 + t + a.shift[2]) * a.src_stride[2];
+#pragma unroll
+            for (int c = 0; c < CPT; ++c)
+                og[c] = scol[(int64_t)(min(ybase + c, ndy - 1) + a.shift[1]) * a.src_stride[1]];
+        }
+    };
     auto step = [&](int u, float* r, const int h) {
         const int t = u - 2 * RZ;  // output completed at this step (if 0 <= t < tz_act)
         y_pass(sX + h * C::XPLANE);
@@ -783,6 +806,19 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
         });
         s = sn;
         if (t >= 0 && t < tz_act) {
+            if constexpr (fused_is_unsharp(MODE)) {
+                // UnsharpMaskFilter's step: the source pixel at the output
+                // pixel's own index against its blurred value, in f64, each
+                // product and sum rounded on its own
+                if constexpr (!OG_AHEAD) load_orig(u);
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const double orig = (double)og[c];
+                    const double edge = orig - (double)o[c];
+                    const double scaled = amount * edge;
+                    o[c] = (float)(fabs(edge) > threshold ? orig + scaled : orig);
+                }
+            }
             if (store_nt) {
                 // dense output, non-temporal (keeps L2 for the input halos);
                 // the lane offset is opaque here so that its widening stays
@@ -827,8 +863,10 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
                 // reload of `r`
                 x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
                 __builtin_amdgcn_sched_barrier(0);
+                if constexpr (OG_AHEAD) load_orig(u + 1);
                 load_plane(r, u + 2 + NSET);
             } else {
+                if constexpr (OG_AHEAD) load_orig(u + 1);
                 load_plane(r, u + 4);
                 x_pass(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
             }
@@ -860,6 +898,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     } else {
         load_plane(rb, 3);
     }
+    if constexpr (OG_AHEAD) load_orig(0);
     lds_barrier();
     if constexpr (EARLY) {
         x_read(sIn);
@@ -907,7 +946,7 @@ void fused3d_kernel([[maybe_unused]] const FusedMulti args) {
     const int item = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
     if (item >= nt) return;
     const int comp = item % nc;
-    fused3d_tile<RXY, RZ, MODE, false>(m.a[comp], item / nc, sIn, sX);
+    fused3d_tile<RXY, RZ, MODE, false>(m.a[comp], item / nc, sIn, sX, m.amount, m.threshold);
 }
 
 // Persistent form over a packed work list of (volume, tile) items for many
@@ -1070,6 +1109,19 @@ constexpr int wide_exact_ty() {
     if constexpr (wide_exact_built<RXY, RZ>()) return Cfg<RXY, RZ, FUSED_FWD_EXACT_W>::TY;
     else return 0;
 }
+// The unsharp modes: the radii UnsharpMaskFilter's index-space kernel
+// gives, the same on every axis (3-D) or on x and y (2-D, z radius 0).
+// EXACT at radius 15 would spill (12 bytes a lane): that radius keeps the
+// blur and the step as two launches.
+template <int RXY, int RZ, int MODE>
+constexpr bool unsharp_built() {
+    return (RZ == RXY || RZ == 0) && !(MODE == FUSED_FWD_EXACT_U && RZ == 15);
+}
+template <int RXY, int RZ, int MODE>
+constexpr int unsharp_ty() {
+    if constexpr (unsharp_built<RXY, RZ, MODE>()) return Cfg<RXY, RZ, MODE>::TY;
+    else return 0;
+}
 // Workgroup size of mode `mode` for (RXY, RZ).
 template <int RXY, int RZ>
 int block_of(int mode) {
@@ -1080,6 +1132,8 @@ int block_of(int mode) {
         case FUSED_FWD_FAST_S: return Cfg<RXY, RZ, FUSED_FWD_FAST_S>::NTH;
         case FUSED_FWD_FAST_W: return 1024;
         case FUSED_FWD_EXACT_W: return 1024;
+        case FUSED_FWD_EXACT_U: return Cfg<RXY, RZ, FUSED_FWD_EXACT_U>::NTH;
+        case FUSED_FWD_FAST_U: return Cfg<RXY, RZ, FUSED_FWD_FAST_U>::NTH;
         default: return Cfg<RXY, RZ, FUSED_ADJ_S>::NTH;
     }
 }
@@ -1113,6 +1167,18 @@ hipError_t launch_r(const FusedMulti& a, int mode, hipStream_t st) {
         case FUSED_FWD_EXACT_W:
             if constexpr (wide_exact_built<RXY, RZ>()) {
                 hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT_W>), grid, block, 0, st, a);
+                break;
+            }
+            return hipErrorInvalidValue;
+        case FUSED_FWD_EXACT_U:
+            if constexpr (unsharp_built<RXY, RZ, FUSED_FWD_EXACT_U>()) {
+                hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_EXACT_U>), grid, block, 0, st, a);
+                break;
+            }
+            return hipErrorInvalidValue;
+        case FUSED_FWD_FAST_U:
+            if constexpr (unsharp_built<RXY, RZ, FUSED_FWD_FAST_U>()) {
+                hipLaunchKernelGGL((fused3d_kernel<RXY, RZ, FUSED_FWD_FAST_U>), grid, block, 0, st, a);
                 break;
             }
             return hipErrorInvalidValue;
@@ -1180,7 +1246,8 @@ constexpr FusedEntry make_entry() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>(), wide_exact_ty<RXY, RZ>()},
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>(), wide_exact_ty<RXY, RZ>(),
+                       unsharp_ty<RXY, RZ, FUSED_FWD_EXACT_U>(), unsharp_ty<RXY, RZ, FUSED_FWD_FAST_U>()},
                       &launch_r<RXY, RZ>, &launch_batch_r<RXY, RZ>, &occupancy_r<RXY, RZ>};
 }
 template <int RXY, int RZ>
@@ -1188,7 +1255,8 @@ constexpr FusedEntry make_entry_single() {
     return FusedEntry{RXY, RZ,
                       {Cfg<RXY, RZ, FUSED_FWD_EXACT>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST>::TY,
                        Cfg<RXY, RZ, FUSED_ADJ>::TY, Cfg<RXY, RZ, FUSED_FWD_FAST_S>::TY,
-                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>(), wide_exact_ty<RXY, RZ>()},
+                       Cfg<RXY, RZ, FUSED_ADJ_S>::TY, wide_ty<RXY, RZ>(), wide_exact_ty<RXY, RZ>(),
+                       unsharp_ty<RXY, RZ, FUSED_FWD_EXACT_U>(), unsharp_ty<RXY, RZ, FUSED_FWD_FAST_U>()},
                       &launch_r<RXY, RZ>, nullptr, nullptr};
 }
 

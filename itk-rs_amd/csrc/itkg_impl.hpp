@@ -146,11 +146,24 @@ enum FusedMode {
     FUSED_ADJ_S = 4,
     FUSED_FWD_FAST_W = 5,
     FUSED_FWD_EXACT_W = 6,
-    FUSED_NMODES = 7
+    // UnsharpMaskFilter's per-pixel step on the forward result, in the same
+    // launch (64-row tiles, EXACT or FAST blur): with o the source pixel at
+    // the output pixel's index and b its blurred value, both widened to f64,
+    // the output is o + amount * (o - b) where |o - b| > threshold, else o.
+    FUSED_FWD_EXACT_U = 7,
+    FUSED_FWD_FAST_U = 8,
+    FUSED_NMODES = 9
 };
 constexpr bool fused_is_adjoint(int mode) { return mode == FUSED_ADJ || mode == FUSED_ADJ_S; }
 // EXACT arithmetic: every product and every sum rounded separately.
-constexpr bool fused_is_exact(int mode) { return mode == FUSED_FWD_EXACT || mode == FUSED_FWD_EXACT_W; }
+constexpr bool fused_is_exact(int mode) {
+    return mode == FUSED_FWD_EXACT || mode == FUSED_FWD_EXACT_W || mode == FUSED_FWD_EXACT_U;
+}
+constexpr bool fused_is_unsharp(int mode) { return mode == FUSED_FWD_EXACT_U || mode == FUSED_FWD_FAST_U; }
+// The plain forward mode whose tiles, registers and arithmetic a mode shares.
+constexpr int fused_base_mode(int mode) {
+    return mode == FUSED_FWD_EXACT_U ? FUSED_FWD_EXACT : (mode == FUSED_FWD_FAST_U ? FUSED_FWD_FAST : mode);
+}
 // Destination tile width (x) of a mode.
 constexpr int fused_tile_x(int mode) {
     return mode == FUSED_FWD_FAST_W || mode == FUSED_FWD_EXACT_W ? 128 : 64;
@@ -189,6 +202,7 @@ constexpr int kFusedMaxComp = 4;
 struct FusedMulti {
     Fused3dArgs a[kFusedMaxComp];
     int32_t nc;
+    double amount, threshold;  // the unsharp modes' step
 };
 
 struct FusedEntry {

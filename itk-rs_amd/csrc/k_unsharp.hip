@@ -3,7 +3,10 @@
 // edge = orig - blur, and the output is orig + amount * edge where
 // |edge| > threshold, else orig, narrowed back to the pixel type.  This file
 // is compiled with -ffp-contract=off: the multiply and the add round
-// separately, as the reference computes them.
+// separately, as the reference computes them.  f32 3-D and 2-D images with
+// a fused kernel for their radius apply the same step in the blur's own
+// launch (fused3d.hpp, the unsharp modes); this kernel serves the rest
+// (f64, 1-D and 4-D images, the per-axis path).
 #include "itkg_impl.hpp"
 
 namespace itkg {

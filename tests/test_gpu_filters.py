@@ -192,6 +192,39 @@ def test_unsharp_exact_matches_oracle(itk, orc, dev, path, size, sigma, amount, 
     assert np.array_equal(out.data.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("size,sigma,thr,req", [
+    ((70, 50, 30), 2.0, 0.03, ((5, 0, 3), (40, 50, 20))),
+    ((130, 129, 67), 1.0, 0.0, None),
+    ((64, 70, 40), 5.0, 0.01, None),     # radius 15: EXACT as two launches, FAST fused
+    ((50, 60, 33), 4.5, 0.0, None),      # radius 14
+    ((96, 80), 1.4, 0.02, ((0, 10), (96, 30))),
+    ((300, 211), 3.3, 0.0, None),
+])
+def test_unsharp_fused_step_equals_two_launches(itk, dev, size, sigma, thr, req):
+    """f32 volumes and planes: the step in the blur's own launch (the fused
+    kernel's unsharp modes) against the blur on the per-axis path followed
+    by the per-pixel step kernel, bit for bit in EXACT and in FAST (the FAST
+    blur has the same bits on both paths); thresholds that split the pixels,
+    sub-requests, radii 3..15."""
+    D = len(size)
+    x = torch.from_numpy(rand(tuple(reversed(size)), 1, np.float32)).to(dev)
+    src = itk.Image(itk.Region([0] * D, list(size)), (1.0,) * D, (0.0,) * D, x)
+    reg = src.region if req is None else itk.Region(list(req[0]), list(req[1]))
+    for arith in (itk.EXACT, itk.FAST):
+        f = itk.UnsharpMaskFilter(src, sigma).with_amount(0.7)
+        f.threshold = thr
+        f.arith = arith
+        fused = f.generate_region(reg).data
+        itk.set_path(itk.PATH_PER_AXIS)
+        try:
+            two = f.generate_region(reg).data
+        finally:
+            itk.set_path(itk.PATH_AUTO)
+        torch.cuda.synchronize()
+        assert torch.equal(fused.view(torch.int32), two.view(torch.int32)), arith
+        assert float((fused - x[src._slices(reg)]).abs().max()) > 0  # the step changed pixels
+
+
 def test_unsharp_fast_and_host_entry(itk, dev):
     x = torch.rand((80, 96, 112), device=dev)
     src = itk.Image(itk.Region([0, 0, 0], [112, 96, 80]), (1.0,) * 3, (0.0,) * 3, x)

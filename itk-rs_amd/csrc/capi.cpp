@@ -1499,24 +1499,22 @@ int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, dou
     Plan p;
     if ((rc = make_plan_weights(2, radii, w, largest, L, &p, false))) return rc;
     const size_t px = dtype_size(dtype);
-    // the six scratch buffers carved from one allocation: each stream-ordered
+    // the scratch buffers (smoothed image, weak and strong edge maps, the
+    // hysteresis flags) carved from one allocation: each stream-ordered
     // free costs the host about 0.1 ms, more than this filter's kernels at
     // moderate sizes
     auto up = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
-    const size_t o_mag = up((size_t)n * px), o_nms = o_mag + up((size_t)n * 8);
-    const size_t o_bin = o_nms + up((size_t)n * 8), o_edges = o_bin + up((size_t)n);
+    const size_t o_weak = up((size_t)n * px), o_edges = o_weak + up((size_t)n);
     const size_t o_flag = o_edges + up((size_t)n), total_bytes = o_flag + 256;
     void* block = nullptr;
-    void *sm = nullptr, *mag = nullptr, *nms = nullptr, *bin = nullptr, *edges = nullptr, *flag = nullptr;
+    void *sm = nullptr, *weak = nullptr, *edges = nullptr, *flag = nullptr;
     hipError_t e = pool_alloc(&block, total_bytes, s);
     if (e != hipSuccess) {
         rc = hip_fail(e, "pool_alloc(Canny scratch)");
     } else {
         char* base = static_cast<char*>(block);
         sm = base;
-        mag = base + o_mag;
-        nms = base + o_nms;
-        bin = base + o_bin;
+        weak = base + o_weak;
         edges = base + o_edges;
         flag = base + o_flag;
     }
@@ -1528,9 +1526,7 @@ int itkg_canny_generate_region(int32_t ndim, itkg_dtype dtype, double sigma, dou
             using T = std::remove_pointer_t<decltype(tag)>;
             CannyArgs<T> a{};
             a.smoothed = static_cast<const T*>(sm);
-            a.mag = static_cast<double*>(mag);
-            a.nms = static_cast<double*>(nms);
-            a.bin = static_cast<uint8_t*>(bin);
+            a.weak = static_cast<uint8_t*>(weak);
             a.edges = static_cast<uint8_t*>(edges);
             a.changed = static_cast<int*>(flag);
             a.out = static_cast<T*>(output->data);

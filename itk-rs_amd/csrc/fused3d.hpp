@@ -285,7 +285,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     constexpr bool FMA = !fused_is_exact(MODE);
     constexpr bool PFIT = FMA && ((RXY <= 6 && RZ <= 6) || (CPT == 2 && RXY <= 10 && RZ <= 10));
     constexpr int NSET = PERSIST && CPT == 4 ? 2 : C::NSET;
-    constexpr bool EARLY = C::EARLY && !PERSIST;
+    constexpr bool EARLY = C::EARLY && !PERSIST && RZ > 0;
     static_assert(!EARLY || !fused_is_unsharp(MODE), "the unsharp loads follow the plain step order");
     constexpr bool PNT = !PERSIST || PFIT;   // non-temporal dense stores
     constexpr bool PASM = !PERSIST || PFIT;  // opaque load offsets (scalar-base loads)
@@ -736,7 +736,8 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
                 og[c] = scol[(int64_t)(min(ybase + c, ndy - 1) + a.shift[1]) * a.src_stride[1]];
         }
     };
-    auto step = [&](int u, float* r, const int h) {
+    // last: the tile's only step (a plane), which ends with its stores
+    auto step = [&](int u, float* r, const int h, const bool last = false) {
         const int t = u - 2 * RZ;  // output completed at this step (if 0 <= t < tz_act)
         y_pass(sX + h * C::XPLANE);
         // EARLY: the first x-pass round's LDS reads go out now, so that the
@@ -850,6 +851,7 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
                 }
             }
         }
+        if (last) return;
         if constexpr (EARLY) {
             x_finish(sIn + (1 - h) * C::PLANE, sX + (1 - h) * C::XPLANE);
             store_plane(r, h, u + 2);
@@ -878,6 +880,20 @@ __device__ __forceinline__ void fused3d_tile(const ARGS& a, const int tile, floa
     if constexpr (EARLY) {
         pin_sgprs(wxr.w);
         pin_sgprs(wyr.w);
+    }
+    if constexpr (RZ == 0) {
+        // a plane (2-D image): one step and nothing to prefetch, where whole
+        // rounds of NSET steps would repeat the loads, the x pass and the
+        // y pass of the same clamped plane
+        lds_barrier();
+        load_plane(ra, 0);
+        if constexpr (OG_AHEAD) load_orig(0);
+        store_plane(ra, 0, 0);
+        lds_barrier();
+        x_pass(sIn, sX);
+        lds_barrier();
+        step(0, ra, 0, true);
+        return;
     }
     lds_barrier();  // the previous tile of a persistent workgroup is done with LDS
     load_plane(ra, 0);

@@ -100,10 +100,8 @@ hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxe
 template <typename T>
 struct CannyArgs {
     const T* smoothed;
-    double* mag;
-    double* nms;
-    uint8_t* bin;
-    uint8_t* edges;
+    uint8_t* weak;   // nms >= lower (suppressed pixels as 0.0)
+    uint8_t* edges;  // nms >= upper, then grown by hysteresis
     int* changed;
     T* out;
     int64_t nx, ny, ox, oy, mx, my;

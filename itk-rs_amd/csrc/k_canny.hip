@@ -46,43 +46,140 @@ __device__ __forceinline__ int canny_bin(const double gx, const double gy) {
     return b;
 }
 
+// canny_bin, with the common case decided without atan2: on the upper
+// half-plane (the gradient negated when gy < 0, which keeps its angle
+// modulo 180 degrees) the angle is below 22.5 degrees or at least 157.5
+// exactly when |gy| < |gx| tan(22.5), between 67.5 and 112.5 when
+// |gy| > |gx| tan(67.5), and otherwise in bin 1 or 3 by the sign of gx.
+// Both differences are within a few ulps of (|gx| + |gy|); where either
+// lies within 1e-12 of it (the angle within about 6e-11 degrees of an
+// edge, far wider than the rounding of the reference's atan2 and degree
+// conversion), and for NaN, infinite or huge gradients, canny_bin decides
+// (one call, not inlined).  gy = 0 is bin 0 (atan2 gives 0 or +-180).
+__device__ __noinline__ int canny_bin_call(const double gx, const double gy) {
+    return canny_bin(gx, gy);
+}
+
+__device__ __forceinline__ int canny_bin_fast(const double gx, const double gy) {
+    const double ax = fabs(gx), ay = fabs(gy), ag = ax + ay;
+    if (!(ag <= 0x1p+1000)) return canny_bin_call(gx, gy);  // NaN, infinite or huge
+    if (gy == 0.0) return 0;
+    const double d1 = ay - ax * 0x1.a827999fcef32p-2;  // tan(22.5 degrees)
+    const double d2 = ay - ax * 0x1.3504f333f9de6p+1;  // tan(67.5 degrees)
+    const double tol = 1e-12 * ag;
+    if (!(fabs(d1) > tol && fabs(d2) > tol)) return canny_bin_call(gx, gy);
+    if (d1 < 0.0) return 0;
+    if (d2 > 0.0) return 2;
+    return (gy > 0.0) == (gx > 0.0) ? 1 : 3;
+}
+
 __global__ __launch_bounds__(256) void canny_bins_kernel(const double* gx, const double* gy,
                                                          uint8_t* bins, const int64_t n) {
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * 256)
-        bins[k] = (uint8_t)canny_bin(gx[k], gy[k]);
+        bins[k] = (uint8_t)canny_bin_fast(gx[k], gy[k]);
 }
+
+// Gradient, direction bin and non-maximum suppression of a TW x TH tile in
+// one pass.  The smoothed tile and a two-pixel ring go to LDS first (one
+// round of independent loads, each at the image coordinate clamped as the
+// central differences clamp it); the f64 magnitudes of the tile and of its
+// one-pixel ring (the suppression's neighbours) follow in LDS, so the
+// magnitude never reaches memory.  The outputs are the strong-edge map
+// (nms >= upper) and the weak map (nms >= lower, suppressed pixels counting
+// as 0.0) that hysteresis reads.
+constexpr int kGradW = 64, kGradH = 16;
 
 template <typename T>
-__global__ __launch_bounds__(256) void canny_grad_kernel(const T* sm, double* mag, uint8_t* bin,
-                                                         const int64_t nx, const int64_t ny) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
-    if (i >= nx) return;
-    const int64_t ip = i + 1 > nx - 1 ? nx - 1 : i + 1, im = i - 1 < 0 ? 0 : i - 1;
-    const int64_t jp = j + 1 > ny - 1 ? ny - 1 : j + 1, jm = j - 1 < 0 ? 0 : j - 1;
-    const double gx = (double)sm[ip + j * nx] - (double)sm[im + j * nx];
-    const double gy = (double)sm[i + jp * nx] - (double)sm[i + jm * nx];
-    mag[i + j * nx] = sqrt(gx * gx + gy * gy);
-    bin[i + j * nx] = (uint8_t)canny_bin(gx, gy);
-}
-
-__global__ __launch_bounds__(256) void canny_nms_kernel(const double* mag, const uint8_t* bin,
-                                                        double* nms, uint8_t* edges, const int64_t nx,
-                                                        const int64_t ny, const double upper) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
-    if (i >= nx) return;
-    const int b = bin[i + j * nx];
-    const int64_t di = b == 0 ? 1 : (b == 1 ? 1 : (b == 2 ? 0 : -1));
-    const int64_t dj = b == 0 ? 0 : 1;
-    auto at = [&](int64_t a, int64_t c) {
-        a = a < 0 ? 0 : (a > nx - 1 ? nx - 1 : a);
-        c = c < 0 ? 0 : (c > ny - 1 ? ny - 1 : c);
-        return mag[a + c * nx];
+__global__ __launch_bounds__(256) void canny_grad_nms_kernel(const T* sm, uint8_t* edges,
+                                                             uint8_t* weak, const int64_t nx,
+                                                             const int64_t ny, const double lower,
+                                                             const double upper) {
+    constexpr int SW = kGradW + 4, SH = kGradH + 4;  // smoothed tile + 2-pixel ring
+    constexpr int MW = kGradW + 2, MH = kGradH + 2;  // magnitudes: tile + 1-pixel ring
+    __shared__ double s_sm[SH][SW];
+    __shared__ double s_mag[MH][MW];
+    const int64_t x0 = (int64_t)blockIdx.x * kGradW, y0 = (int64_t)blockIdx.y * kGradH;
+    auto clampx = [&](int64_t i) { return i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i); };
+    auto clampy = [&](int64_t j) { return j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j); };
+    // cell (ty, tx) holds the pixel at the clamped coordinate of
+    // (x0 - 2 + tx, y0 - 2 + ty): the gradient at an image pixel reads its
+    // clamped neighbours from the cells next to its own
+    constexpr int SN = SW * SH, SPER = (SN + 255) / 256;
+    T v[SPER];
+#pragma unroll
+    for (int m = 0; m < SPER; ++m) {
+        const int k = min((int)threadIdx.x + 256 * m, SN - 1);
+        const int ty = k / SW, tx = k - ty * SW;
+        v[m] = sm[clampx(x0 - 2 + tx) + clampy(y0 - 2 + ty) * nx];
+    }
+#pragma unroll
+    for (int m = 0; m < SPER; ++m) {
+        const int k = (int)threadIdx.x + 256 * m;
+        if (k < SN) s_sm[k / SW][k % SW] = (double)v[m];
+    }
+    __syncthreads();
+    // the gradient of tile-relative pixel (px, py), px in -1 .. TW, py in
+    // -1 .. TH
+    auto grad = [&](int px, int py, double& gx, double& gy) {
+        gx = s_sm[py + 2][px + 3] - s_sm[py + 2][px + 1];
+        gy = s_sm[py + 3][px + 2] - s_sm[py + 1][px + 2];
     };
-    const double m = mag[i + j * nx];
-    const double v = (m >= at(i + di, j + dj) && m >= at(i - di, j - dj)) ? m : 0.0;
-    nms[i + j * nx] = v;
-    edges[i + j * nx] = v >= upper ? 1 : 0;
+    // the tile's magnitudes and bins (each thread keeps the bins of the
+    // pixels it suppresses below) ...
+    constexpr int PER = kGradW * kGradH / 256;
+    int bins[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int k = threadIdx.x + 256 * m;
+        const int tx = k % kGradW, ty = k / kGradW;
+        double gx, gy;
+        grad(tx, ty, gx, gy);
+        s_mag[ty + 1][tx + 1] = sqrt(gx * gx + gy * gy);
+        bins[m] = canny_bin_fast(gx, gy);
+    }
+    // ... and the magnitudes of its one-pixel ring: the top and bottom rows,
+    // then the left and right columns between them (cells outside the image
+    // hold values the suppression never reads: it clamps its neighbours
+    // into the image)
+    constexpr int RING = 2 * MW + 2 * kGradH;
+    if (threadIdx.x < RING) {
+        int r = threadIdx.x, tx, ty;
+        if (r < 2 * MW) {
+            ty = r < MW ? 0 : MH - 1;
+            tx = r < MW ? r : r - MW;
+        } else {
+            r -= 2 * MW;
+            ty = 1 + (r >> 1);
+            tx = (r & 1) ? MW - 1 : 0;
+        }
+        double gx, gy;
+        grad(tx - 1, ty - 1, gx, gy);
+        s_mag[ty][tx] = sqrt(gx * gx + gy * gy);
+    }
+    __syncthreads();
+    // the suppression's neighbours, clamped into the image, as tile-relative
+    // coordinates: lo (-1, or 0 on the image's first tile) .. hi (TW, or the
+    // image's last pixel)
+    const int lox = x0 == 0 ? 0 : -1, loy = y0 == 0 ? 0 : -1;
+    const int hix = (int)min<int64_t>(kGradW, nx - 1 - x0), hiy = (int)min<int64_t>(kGradH, ny - 1 - y0);
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int k = threadIdx.x + 256 * m;
+        const int tx = k % kGradW, ty = k / kGradW;
+        if (tx > hix || ty > hiy) continue;  // outside the image
+        const int b = bins[m];
+        const int di = b == 0 ? 1 : (b == 1 ? 1 : (b == 2 ? 0 : -1));
+        const int dj = b == 0 ? 0 : 1;
+        auto at = [&](int a, int c) {
+            return s_mag[min(max(c, loy), hiy) + 1][min(max(a, lox), hix) + 1];
+        };
+        const double mm = s_mag[ty + 1][tx + 1];
+        const double val = (mm >= at(tx + di, ty + dj) && mm >= at(tx - di, ty - dj)) ? mm : 0.0;
+        const int64_t o = (x0 + tx) + (y0 + ty) * nx;
+        edges[o] = val >= upper ? 1 : 0;
+        weak[o] = val >= lower ? 1 : 0;
+    }
 }
 
 // One hysteresis pass: each block grows the edges inside its 32 x 32 tile
@@ -90,9 +187,9 @@ __global__ __launch_bounds__(256) void canny_nms_kernel(const double* mag, const
 // tile is stable, writes them back and raises *changed if it added any.
 constexpr int kTile = 32;
 
-__global__ __launch_bounds__(256) void canny_hysteresis_kernel(const double* nms, uint8_t* edges,
+__global__ __launch_bounds__(256) void canny_hysteresis_kernel(const uint8_t* weakmap, uint8_t* edges,
                                                                const int64_t nx, const int64_t ny,
-                                                               const double lower, int* changed) {
+                                                               int* changed) {
     __shared__ uint8_t e[kTile + 2][kTile + 2];
     __shared__ uint8_t weak[kTile][kTile];
     __shared__ int grew;
@@ -105,7 +202,7 @@ __global__ __launch_bounds__(256) void canny_hysteresis_kernel(const double* nms
     for (int k = threadIdx.x; k < kTile * kTile; k += 256) {
         const int ty = k / kTile, tx = k % kTile;
         const int64_t gx = x0 + tx, gy = y0 + ty;
-        weak[ty][tx] = (gx < nx && gy < ny) ? (nms[gx + gy * nx] >= lower ? 1 : 0) : 0;
+        weak[ty][tx] = (gx < nx && gy < ny) ? weakmap[gx + gy * nx] : 0;
     }
     bool any = false;
     for (;;) {
@@ -137,14 +234,20 @@ __global__ __launch_bounds__(256) void canny_hysteresis_kernel(const double* nms
     if (threadIdx.x == 0) *changed = 1;
 }
 
+// The request as 1 / 0: kOutPer pixels of a row per thread, 256 apart.
+constexpr int kOutPer = 4;
+
 template <typename T>
 __global__ __launch_bounds__(256) void canny_out_kernel(const uint8_t* edges, T* out,
                                                         const int64_t nx, const int64_t ox,
                                                         const int64_t oy, const int64_t mx,
                                                         const int64_t s0, const int64_t s1) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
-    if (i >= mx) return;
-    out[i * s0 + j * s1] = edges[(i + ox) + (j + oy) * nx] ? (T)1.0 : (T)0.0;
+    const int64_t j = blockIdx.y;
+#pragma unroll
+    for (int k = 0; k < kOutPer; ++k) {
+        const int64_t i = (int64_t)blockIdx.x * (256 * kOutPer) + 256 * k + threadIdx.x;
+        if (i < mx) out[i * s0 + j * s1] = edges[(i + ox) + (j + oy) * nx] ? (T)1.0 : (T)0.0;
+    }
 }
 
 }  // namespace
@@ -160,11 +263,11 @@ hipError_t launch_canny_bins(const double* gx, const double* gy, uint8_t* bins, 
 template <typename T>
 hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     const int64_t nx = a.nx, ny = a.ny;
-    if (ny > 65535 || a.my > 65535 || (ny + kTile - 1) / kTile > 65535) return hipErrorInvalidValue;
-    const dim3 g((unsigned)((nx + 255) / 256), (unsigned)ny);
-    hipLaunchKernelGGL(canny_grad_kernel<T>, g, dim3(256), 0, s, a.smoothed, a.mag, a.bin, nx, ny);
-    hipLaunchKernelGGL(canny_nms_kernel, g, dim3(256), 0, s, a.mag, a.bin, a.nms, a.edges, nx, ny,
-                       a.upper);
+    if (a.my > 65535 || (ny + kTile - 1) / kTile > 65535 || (ny + kGradH - 1) / kGradH > 65535)
+        return hipErrorInvalidValue;
+    const dim3 g((unsigned)((nx + kGradW - 1) / kGradW), (unsigned)((ny + kGradH - 1) / kGradH));
+    hipLaunchKernelGGL(canny_grad_nms_kernel<T>, g, dim3(256), 0, s, a.smoothed, a.edges, a.weak, nx,
+                       ny, a.lower, a.upper);
     hipError_t e = hipGetLastError();
     const dim3 gt((unsigned)((nx + kTile - 1) / kTile), (unsigned)((ny + kTile - 1) / kTile));
     // passes until one adds nothing (each pass grows every tile to its own
@@ -177,13 +280,13 @@ hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     int batch = 1;
     while (e == hipSuccess) {
         for (int i = 0; i + 1 < batch; ++i)
-            hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.nms, a.edges, nx,
-                               ny, a.lower, a.changed + 1);
+            hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.weak, a.edges, nx,
+                               ny, a.changed + 1);
         int host = 0;
         e = hipMemsetAsync(a.changed, 0, sizeof(int), s);
         if (e != hipSuccess) break;
-        hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.nms, a.edges, nx, ny,
-                           a.lower, a.changed);
+        hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.weak, a.edges, nx, ny,
+                           a.changed);
         e = hipMemcpyAsync(&host, a.changed, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess || !host) break;
@@ -191,7 +294,8 @@ hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     }
     if (e != hipSuccess) return e;
     if (a.mx > 0 && a.my > 0)
-        hipLaunchKernelGGL(canny_out_kernel<T>, dim3((unsigned)((a.mx + 255) / 256), (unsigned)a.my),
+        hipLaunchKernelGGL(canny_out_kernel<T>,
+                           dim3((unsigned)((a.mx + 256 * kOutPer - 1) / (256 * kOutPer)), (unsigned)a.my),
                            dim3(256), 0, s, a.edges, a.out, nx, a.ox, a.oy, a.mx, a.out_stride[0],
                            a.out_stride[1]);
     return hipGetLastError();

@@ -127,7 +127,10 @@ def test_device_bins_equal_the_oracle(itk, orc, dev):
     rng = np.random.default_rng(6)
     special = [(0.0, 0.0), (-0.0, 0.0), (0.0, -0.0), (-0.0, -0.0), (1.0, 0.0), (-1.0, 0.0),
                (0.0, 1.0), (0.0, -1.0), (1.0, 1.0), (-1.0, 1.0), (1.0, -1.0), (-1.0, -1.0),
-               (3.0, 7.0), (1e-300, 1e-300), (1e300, -1e300), (5e-324, 0.0)]
+               (3.0, 7.0), (1e-300, 1e-300), (1e300, -1e300), (5e-324, 0.0), (0.0, 5e-324),
+               (-1.0, -0.0), (-0.0, -1.0), (math.nan, 1.0), (1.0, math.nan), (math.inf, 1.0),
+               (1.0, -math.inf), (math.inf, math.inf), (-math.inf, -math.inf), (-math.inf, 0.0),
+               (1e301, 3e301), (1e-271, -2e-271)]
     gx = np.concatenate([[t[0] for t in g], [s[0] for s in special], rng.standard_normal(50000)])
     gy = np.concatenate([[t[1] for t in g], [s[1] for s in special], rng.standard_normal(50000)])
     want = orc.canny_bins(gx, gy)

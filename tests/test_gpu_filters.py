@@ -417,6 +417,28 @@ def test_canny_matches_oracle(itk, orc, dev, case):
     assert float(f.generate_region(src.region).data.sum()) > 0, name   # the case has edges
 
 
+@pytest.mark.parametrize("shape,sigma,lower,upper", [
+    ((1, 37), 1.0, 0.02, 0.1),
+    ((45, 1), 1.0, 0.02, 0.1),
+    ((3, 2), 0.7, 0.0, 0.05),
+    ((17, 129), 1.3, 0.0, 0.15),     # lower 0: every suppressed pixel (nms 0.0) is weak
+    ((70, 65), 1.0, -1.0, 0.2),
+])
+def test_canny_thin_images_and_low_thresholds(itk, orc, dev, shape, sigma, lower, upper):
+    """Images one or a few pixels across (every gradient and suppression
+    neighbour clamped, the 64 x 16 tiles mostly outside the image) and lower
+    thresholds at or below zero, where suppressed pixels count as weak:
+    bit for bit the oracle."""
+    rng = np.random.default_rng(sum(shape))
+    x = rng.random(shape).astype(np.float32)
+    ny, nx = shape
+    src = itk.Image(itk.Region([0, 0], [nx, ny]), (1.0, 1.0), (0.0, 0.0), torch.from_numpy(x).to(dev))
+    out = itk.CannyEdgeDetectionFilter(src, sigma, lower, upper).generate_region(src.region)
+    torch.cuda.synchronize()
+    ref = orc.canny_generate_region(x, [0, 0], sigma, lower, upper, [0, 0], [nx, ny])
+    assert np.array_equal(out.data.cpu().numpy(), ref), shape
+
+
 def test_canny_f64_and_errors(itk, orc, dev):
     rng = np.random.default_rng(5)
     x = rng.random((50, 60))

@@ -95,8 +95,11 @@ hipError_t launch_recursive(const RecursiveArgs<T>& a, const int* axes, int naxe
                             const double (*coef)[4], hipStream_t s);
 
 // Canny after smoothing: `smoothed` is the dense nx x ny smoothed image;
-// mag, nms (f64), bin, edges (u8) are nx x ny scratch and `changed` two ints;
-// out (strides in pixels) receives the mx x my request at offset (ox, oy).
+// weak and edges are nx x ny byte maps of scratch and `changed` two device
+// ints (the flag of a hysteresis batch's earlier passes, which nobody
+// reads; the last pass of a batch raises a pinned host flag instead); out
+// (strides in pixels) receives the mx x my request at offset (ox, oy).  The
+// call synchronises the stream after each batch of hysteresis passes.
 template <typename T>
 struct CannyArgs {
     const T* smoothed;

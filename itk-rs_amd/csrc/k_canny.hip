@@ -7,6 +7,7 @@
 // every pixel with nms >= lower 8-connected to an edge).  Compiled with
 // -ffp-contract=off: the products and sums round as the reference's do.
 #include <algorithm>
+#include <mutex>
 
 #include "itkg_impl.hpp"
 
@@ -260,6 +261,19 @@ hipError_t launch_canny_bins(const double* gx, const double* gy, uint8_t* bins, 
     return hipGetLastError();
 }
 
+// The flag the last pass of each hysteresis batch raises, in pinned
+// coherent host memory (one per device, allocated on first use and kept
+// until the process ends): the host clears it before the batch and reads it
+// after the stream synchronisation, with no copy or fill on the stream.
+// Calls on one device take turns at it.
+constexpr int kFlagDevices = 64;
+struct HostFlag {
+    std::mutex mu;
+    int* host = nullptr;  // host address
+    int* dev = nullptr;   // the same memory as the kernels address it
+};
+HostFlag g_host_flag[kFlagDevices];
+
 template <typename T>
 hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     const int64_t nx = a.nx, ny = a.ny;
@@ -277,19 +291,34 @@ hipError_t launch_canny(const CannyArgs<T>& a, hipStream_t s) {
     // nothing, so a chain across k tiles costs about log2(k) waits on the
     // host instead of k, for at most twice the passes.  The batch's other
     // passes raise a second flag nobody reads.
+    int dev = 0;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kFlagDevices) return hipErrorInvalidDevice;
+    HostFlag& hf = g_host_flag[dev];
+    std::lock_guard<std::mutex> lock(hf.mu);
+    if (!hf.host) {
+        void* p = nullptr;
+        e = hipHostMalloc(&p, sizeof(int), hipHostMallocCoherent);
+        if (e != hipSuccess) return e;
+        void* d = nullptr;
+        e = hipHostGetDevicePointer(&d, p, 0);
+        if (e != hipSuccess) return e;  // p stays allocated; the next call retries the mapping
+        hf.host = static_cast<int*>(p);
+        hf.dev = static_cast<int*>(d);
+    }
+    volatile int* const flag = hf.host;
     int batch = 1;
     while (e == hipSuccess) {
         for (int i = 0; i + 1 < batch; ++i)
             hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.weak, a.edges, nx,
                                ny, a.changed + 1);
-        int host = 0;
-        e = hipMemsetAsync(a.changed, 0, sizeof(int), s);
-        if (e != hipSuccess) break;
+        *flag = 0;  // the last batch's passes have completed (synchronised below)
         hipLaunchKernelGGL(canny_hysteresis_kernel, gt, dim3(256), 0, s, a.weak, a.edges, nx, ny,
-                           a.changed);
-        e = hipMemcpyAsync(&host, a.changed, sizeof(int), hipMemcpyDeviceToHost, s);
+                           hf.dev);
+        e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess || !host) break;
+        if (e != hipSuccess || !*flag) break;
         batch = batch < 64 ? 2 * batch : 64;
     }
     if (e != hipSuccess) return e;

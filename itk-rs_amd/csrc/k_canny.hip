@@ -195,15 +195,40 @@ __global__ __launch_bounds__(256) void canny_hysteresis_kernel(const uint8_t* we
     __shared__ uint8_t weak[kTile][kTile];
     __shared__ int grew;
     const int64_t x0 = (int64_t)blockIdx.x * kTile, y0 = (int64_t)blockIdx.y * kTile;
-    for (int k = threadIdx.x; k < (kTile + 2) * (kTile + 2); k += 256) {
-        const int ty = k / (kTile + 2), tx = k % (kTile + 2);
-        const int64_t gx = x0 + tx - 1, gy = y0 + ty - 1;
-        e[ty][tx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? edges[gx + gy * nx] : 0;
-    }
-    for (int k = threadIdx.x; k < kTile * kTile; k += 256) {
+    // the tile's own pixels first: a tile with no weak pixel that is not an
+    // edge yet (most of an image) has nothing to grow and stops here
+    static_assert(kTile * kTile == 4 * 256, "four pixels a thread");
+    bool cand = false;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int k = threadIdx.x + 256 * m;
         const int ty = k / kTile, tx = k % kTile;
         const int64_t gx = x0 + tx, gy = y0 + ty;
-        weak[ty][tx] = (gx < nx && gy < ny) ? weakmap[gx + gy * nx] : 0;
+        uint8_t w = 0, ev = 0;
+        if (gx < nx && gy < ny) {
+            w = weakmap[gx + gy * nx];
+            ev = edges[gx + gy * nx];
+        }
+        weak[ty][tx] = w;
+        e[ty + 1][tx + 1] = ev;
+        cand = cand || (w && !ev);
+    }
+    if (!__syncthreads_or(cand)) return;
+    // the one-pixel ring: the top and bottom rows, then the left and right
+    // columns between them
+    constexpr int RING = 4 * kTile + 4;
+    if (threadIdx.x < RING) {
+        int r = threadIdx.x, tx, ty;
+        if (r < 2 * (kTile + 2)) {
+            ty = r < kTile + 2 ? 0 : kTile + 1;
+            tx = r < kTile + 2 ? r : r - (kTile + 2);
+        } else {
+            r -= 2 * (kTile + 2);
+            ty = 1 + (r >> 1);
+            tx = (r & 1) ? kTile + 1 : 0;
+        }
+        const int64_t gx = x0 + tx - 1, gy = y0 + ty - 1;
+        e[ty][tx] = (gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? edges[gx + gy * nx] : 0;
     }
     bool any = false;
     for (;;) {

@@ -275,6 +275,10 @@ def test_unsharp_streamed_and_vector_refused(itk, dev):
     ((81, 40, 9), (1.0, 1.0, 1.0), 2.2, 0, None, np.float64),
     ((113, 7), (1.0, 1.0), 2.6, -1, None, np.float32),
     ((95, 5, 3), (1.0, 1.0, 1.0), 1.1, 2, ((0, 1, 0), (95, 3, 3)), np.float32),
+    # even rows: the in-place passes of the strided axes take two lines a
+    # thread (f64 pixels, a 4-D image with two such passes)
+    ((64, 40, 12), (1.0, 1.0, 1.0), 1.5, -1, ((2, 3, 1), (60, 35, 10)), np.float64),
+    ((20, 9, 7, 5), (1.0, 1.0, 1.0, 1.0), 1.2, -1, None, np.float32),
 ])
 def test_recursive_matches_oracle(itk, orc, dev, size, spacing, sigma, axis, req, dtype):
     D = len(size)

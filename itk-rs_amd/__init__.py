@@ -15,6 +15,7 @@ from __future__ import annotations
 import contextlib
 import ctypes
 import os
+import threading
 from dataclasses import dataclass, field
 from typing import Iterator, List, Optional, Sequence, Tuple
 
@@ -1122,12 +1123,46 @@ def gaussian(x: torch.Tensor, sigma: float, spacing=None, ndim: Optional[int] = 
         x = x.contiguous()
     dt, nc = _pixel_info(x, nd)
     out = torch.empty_like(x)
-    vin, vout = _whole_view(x, nd, nc), _whole_view(out, nd, nc)
+    vin, vout = _whole_views(x, out, nd, nc)
     with _on_device(x.device):
         _check(lib.itkg_gaussian_generate_region(
-            nd, dt, nc, float(sigma), _doubles(sp), ctypes.byref(vin.region), ctypes.byref(vin),
+            nd, dt, nc, float(sigma), _spacing_array(sp), ctypes.byref(vin.region), ctypes.byref(vin),
             ctypes.byref(vout), arith, _stream_ptr(x.device)))
     return out
+
+
+# Per-thread view structs of gaussian(), by layout: only the data pointers
+# change from call to call (building the structs field by field costs a
+# small image's call more host time than its kernel takes).  The library
+# reads them during the call only.
+_VIEWS = threading.local()
+_SPACINGS: dict = {}
+
+
+def _whole_views(x: torch.Tensor, out: torch.Tensor, nd: int, nc: int) -> Tuple[c_view, c_view]:
+    cache = getattr(_VIEWS, "d", None)
+    if cache is None:
+        cache = _VIEWS.d = {}
+    key = (x.shape, x.stride(), nd, nc)
+    pair = cache.get(key)
+    if pair is None:
+        if len(cache) >= 64:
+            cache.clear()
+        pair = cache[key] = (_whole_view(x, nd, nc), _whole_view(out, nd, nc))
+    vin, vout = pair
+    vin.data = x.data_ptr() if x.numel() else None
+    vout.data = out.data_ptr() if out.numel() else None
+    return vin, vout
+
+
+def _spacing_array(sp: Tuple[float, ...]):
+    """The spacing as a C double array, one per distinct spacing (read only)."""
+    a = _SPACINGS.get(sp)
+    if a is None:
+        if len(_SPACINGS) >= 256:
+            _SPACINGS.clear()
+        a = _SPACINGS[sp] = _doubles(sp)
+    return a
 
 
 def _whole_view(t: torch.Tensor, nd: int, nc: int) -> c_view:

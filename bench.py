@@ -457,6 +457,25 @@ def main():
             w1, ms1 = timed_loop(lambda: itk.gaussian(x1, s1, sp1, arith=arith), 20, 3, stream)
             extras["c1_512sq"] = {"gvox_s": round(x1.numel() * 20 / w1 / 1e9, 3),
                                   "ms_median": round(float(np.median(ms1)), 4)}
+            # the same calls back to back: wall time between two
+            # synchronisations, without timed_loop's two events a call (for
+            # an image this small they are most of a step's host time)
+            import gc
+            if not KEEP_GC:
+                gc.collect()
+                gc.disable()
+            try:
+                torch.cuda.synchronize()
+                tb0 = time.perf_counter()
+                for _ in range(500):
+                    itk.gaussian(x1, s1, sp1, arith=arith)
+                torch.cuda.synchronize()
+                tb = (time.perf_counter() - tb0) / 500
+            finally:
+                if not KEEP_GC:
+                    gc.enable()
+            extras["c1_512sq"]["us_per_call_back_to_back"] = round(tb * 1e6, 2)
+            extras["c1_512sq"]["gvox_s_back_to_back"] = round(x1.numel() / tb / 1e9, 3)
             extras["c1_max_rel_err_vs_f64"] = max_rel_err(itk.gaussian(x1, s1, sp1, arith=arith),
                                                           itk.gaussian(x1.double(), s1, sp1))
             v3, s3, sp3 = wl.c3(dev)

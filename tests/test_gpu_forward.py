@@ -372,3 +372,49 @@ def test_per_axis_row_and_group_passes(itk, orc, dev, r):
         gref, _, _ = orc.adjoint_f64(g.astype(np.float64), [0, 0, 0], [nx, 40, 19], [1.0] * 3, sigma,
                                      list(req.index), list(req.size))
         assert _rel_err(ga, gref) <= FAST_TOL, nx
+
+
+def test_gaussian_from_threads_and_alternating_layouts(itk, orc, dev):
+    """gaussian() keeps its view structs per thread and layout: four threads
+    filtering distinct images of one layout at once, and one thread
+    alternating layouts (2-D, 3-D, interleaved components, f64), each equal
+    bit for bit to the result of a lone call; the 2-D image against the
+    oracle."""
+    import threading
+    g = torch.Generator(device=dev).manual_seed(31)
+    imgs = [torch.rand((96, 130), generator=g, device=dev) for _ in range(4)]
+    want = [itk.gaussian(x, 1.5, arith=itk.EXACT) for x in imgs]
+    torch.cuda.synchronize()
+    ref = orc.generate_region(imgs[0].cpu().numpy(), [0, 0], [1.0, 1.0], 1.5, [0, 0], [130, 96])
+    assert np.array_equal(want[0].cpu().numpy(), ref)
+    got = [None] * 4
+    errors = []
+
+    def run(k):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for _ in range(20):
+                    got[k] = itk.gaussian(imgs[k], 1.5, arith=itk.EXACT)
+            s.synchronize()
+        except Exception as e:  # reported below, in the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for k in range(4):
+        assert torch.equal(got[k], want[k]), k
+    others = [torch.rand((20, 30, 40), generator=g, device=dev),
+              torch.rand((30, 40, 3), generator=g, device=dev),
+              torch.rand((96, 130), generator=g, device=dev, dtype=torch.float64)]
+    alone = [itk.gaussian(others[0], 1.2), itk.gaussian(others[1], 1.2, ndim=2),
+             itk.gaussian(others[2], 1.2)]
+    for _ in range(3):
+        for x, a in zip([imgs[0]] + others, [want[0]] + alone):
+            nd = 2 if x.dim() == 3 and x.shape[-1] == 3 else None
+            sigma = 1.5 if x is imgs[0] else 1.2
+            assert torch.equal(itk.gaussian(x, sigma, ndim=nd, arith=itk.EXACT), a)

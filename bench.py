@@ -522,16 +522,16 @@ def main():
                 "ms_median": round(float(np.median(msb)), 4), "ms_min": round(float(np.min(msb)), 4),
                 "hbm_frac_4_round_trips": round(4 * 8.0 * nm / (np.mean(msb) * 1e-3) / 1e9
                                                 / HBM_PEAK_GBS, 4)}
-            # UnsharpMaskFilter, sigma 2 in pixels (radius 6): one fused
-            # round trip, the per-pixel step in the blur's own launch (the
-            # source pixel at each output's index read again there, from
-            # cache): 4 + 4 B per voxel
+            # UnsharpMaskFilter, sigma 2 in pixels (radius 6): one launch, the
+            # per-pixel step in the blur's epilogue; the source pixel at each
+            # output's index is read again there, from memory (the window
+            # that held it left L2 steps before): 4 + 4 + 4 B per voxel
             uf = itk.UnsharpMaskFilter(im, 2.0)
             uf.arith = arith
             wu, msu = timed_loop(lambda: uf.generate_into(im, om), 20, 3, stream)
             extras["unsharp_s2_512"] = {
                 "gvox_s": round(nm * 20 / wu / 1e9, 3), "ms": round(float(np.mean(msu)), 4),
-                "hbm_frac_8B": round(8.0 * nm / (np.mean(msu) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                "hbm_frac_12B": round(12.0 * nm / (np.mean(msu) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             # RecursiveGaussianFilter, sigma 2: a causal and an anticausal
             # line pass per axis over an f64 work image, the first reading the
             # f32 input and the last writing the f32 output; each pass reads
